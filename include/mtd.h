@@ -1,0 +1,162 @@
+/* mtd.h -- C ABI of the MI355X-native off-axis preprocessing and Fourier
+ * (Ewald-cap) reconstruction of madeba/MTD_transmission.
+ *
+ * The reference has no library layer: its hot path is two executables,
+ * tomo_pretraitement (Tomo_prog/pretraitement_hors_axe_CPU/main.cpp) and
+ * tomo_reconstruction (Tomo_prog/Reconstruction/main.cpp), whose internal
+ * C++ functions take std::vector buffers. Each entry point below replaces one
+ * of those internal functions or loops; the reference line it replaces is
+ * cited next to it. The drop-in executables built from
+ * mtd_transmission_amd/tools/ are thin mains over this ABI.
+ *
+ * Conventions
+ *  - Every call returns 0 (MTD_OK) or an error code; mtd_last_error() gives
+ *    the message of the last failure on that context.
+ *  - Array arguments are DEVICE pointers (hipMalloc / torch CUDA tensors)
+ *    unless the comment says "host". The caller owns them; the context owns
+ *    its scratch workspaces, twiddle tables and fit plans.
+ *  - Calls are asynchronous on the context's stream (mtd_ctx_set_stream);
+ *    mtd_ctx_sync() waits. One context per host thread.
+ *  - Complex arrays are interleaved float pairs (re, im) -- "c32".
+ *  - posspec arrays hold 2*A int32: all x first, then all y (tab_posSpec.raw
+ *    order, pretraitement_hors_axe_CPU/main.cpp:250-251), in crop pixels.
+ *  - Images are row-major, x fastest (cpt = x + y*dim); volumes
+ *    x + y*V + z*V*V.
+ */
+#ifndef MTD_H
+#define MTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mtd_status {
+  MTD_OK = 0,
+  MTD_ERR_ARG = 1,          /* invalid argument or geometry            */
+  MTD_ERR_UNSUPPORTED = 2,  /* size/mode not built (e.g. DEROUL 1)     */
+  MTD_ERR_HIP = 3,          /* HIP runtime error                       */
+  MTD_ERR_IO = 4            /* file could not be read or written       */
+};
+
+enum mtd_dtype { MTD_U8 = 0, MTD_U16 = 1, MTD_F32 = 2 };
+
+typedef struct mtd_ctx mtd_ctx;
+
+/* Acquisition geometry: the values both `manip` constructors derive
+ * (pretraitement_hors_axe_CPU/manip.cpp:9-157,
+ * Reconstruction/src/manip.cpp:10-163). Fill it with mtd_geom_derive(). */
+typedef struct mtd_geom {
+  int32_t n;           /* CCD_ROIX: hologram side N                        */
+  int32_t nxmax;       /* NXMAX: crop side C = 2*NXMAX                     */
+  int32_t dim_final;   /* DIM_FINAL: volume side V                         */
+  int32_t circle_cx;   /* CIRCLE_CX: carrier column of the +1 order        */
+  int32_t circle_cy;   /* CIRCLE_CY                                        */
+  int32_t born;        /* BORN 1: Born field, 0: Rytov                     */
+  int32_t c_aber;      /* C_ABER: polynomial aberration correction         */
+  int32_t deroul;      /* DEROUL: phase unwrapping (only 0 is supported)   */
+  double n0;           /* N0, float-rounded                                */
+  double na;           /* NA, float-rounded                                */
+  double lambda;       /* LAMBDA (m), float-rounded                        */
+  double pix_holo;     /* tailleTheoPixelHolo = TPCAM/Gt                   */
+  double rayon;        /* round(NXMAX*n0/NA)                               */
+  double pix_uborn;    /* tailleTheoPixelUborn = pix_holo*N/(2*NXMAX)      */
+  double delta_f;      /* Delta_fUborn = 1/(pix_uborn*2*NXMAX)             */
+  double coef_na;      /* COEF_NA_OBJ_LIMIT (default 1)                    */
+  double alpha_holo;   /* Tukey alpha of the hologram window (0.05)        */
+  double alpha_field;  /* Tukey alpha of the field window (0.1)            */
+} mtd_geom;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the FFTW_init / FFT_encaps plan objects
+ * (pretraitement_hors_axe_CPU/src/FFTW_init.cpp, Reconstruction/src/FFT_encaps.cpp). */
+int mtd_ctx_create(int device, mtd_ctx** out);
+int mtd_ctx_destroy(mtd_ctx* ctx);
+/* hip_stream: a hipStream_t (NULL = the legacy default stream). */
+int mtd_ctx_set_stream(mtd_ctx* ctx, void* hip_stream);
+int mtd_ctx_sync(mtd_ctx* ctx);
+const char* mtd_last_error(const mtd_ctx* ctx);
+const char* mtd_version(void);
+
+/* ---- host-only helpers (no GPU needed) -------------------------------- */
+/* extract_val: last "KEY value" line, value through atof into a C float
+ * (pretraitement_hors_axe_CPU/src/IO_fonctions.cpp:81-118; default form
+ * Reconstruction/src/fonctions.cpp:137-180). has_default=0: missing -> 0.
+ * Returns 1 if the key was found, 0 if not, -1 if the file is unreadable. */
+int mtd_config_extract_val(const char* path, const char* token, int has_default,
+                           double default_value, float* out);
+/* extract_string (IO_fonctions.cpp:44-79). Copies at most cap-1 chars. */
+int mtd_config_extract_string(const char* path, const char* token, char* out,
+                              size_t cap);
+/* Derived geometry from config values (manip.cpp:32-69,104;
+ * Reconstruction/src/manip.cpp:52-94). Inputs are the raw config values;
+ * they are rounded through float as extract_val does. */
+int mtd_geom_derive(double ccd_roix, double nxmax, double dim_final,
+                    double circle_cx, double circle_cy, double na, double n0,
+                    double lambda, double tpcam, double f_tube, double f_obj,
+                    double rf, double coef_na, int born, int c_aber, int deroul,
+                    mtd_geom* out);
+/* tukey2D 1-D factor (FFT_fonctions.cpp:61-87), host arrays of n doubles. */
+int mtd_tukey1d(int n, double alpha, double* out);
+
+/* ---- preprocessing (tomo_pretraitement) -------------------------------- */
+/* holo2TF_UBorn over a stack (fonctions.cpp:279-304, loop main.cpp:130-165):
+ * holos[A][N][N] (dtype) -> crops[A][C][C] c32, the centred spectrum
+ * /N^2 cropped at (CIRCLE_CX-NXMAX, CIRCLE_CY-NXMAX). */
+int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                     int dtype, int A, float* crops);
+
+/* Field formation from crops (main.cpp:230-342): coordSpec, calc_Uborn2,
+ * atan2 phase, aberCorr2/ampliCorr2 polynomial fits (FP64), amplitude clamp,
+ * Born/Rytov (or the C_ABER 0 branch). mask: host uint8 [C][C]
+ * (Image_mask.pgm) or NULL (all 255, Correction_aberration.cpp:48-52).
+ * Outputs posspec[2A] int32 and fields[A][C][C] c32 (UBornfinal order). */
+int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops,
+                      int A, const uint8_t* mask, int32_t* posspec,
+                      float* fields);
+
+/* Both preprocessing loops (main.cpp:130-342) from holograms. */
+int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                   int dtype, int A, const uint8_t* mask, int32_t* posspec,
+                   float* fields);
+
+/* ---- reconstruction (tomo_reconstruction) ------------------------------ */
+/* Per-angle spectrum (Reconstruction/main.cpp:207-217): U_B*tukey(C,0.1),
+ * shift by NXMAX, forward FFT * pixUBorn^2, shift by +posSpec.
+ * spectra[A][C][C] c32 in the frame retroPropag_Born reads. */
+int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
+                          const int32_t* posspec, int A, float* spectra);
+
+/* retroPropag_Born over all angles in order (Reconstruction/src/fonctions.cpp:875-935,
+ * loop main.cpp:203-225): deterministic Ewald-cap accumulation of spectra
+ * (frame of mtd_field_to_spectrum) into vol[V^3] c32 and counts[V^3] int32.
+ * Both outputs are fully overwritten. */
+int mtd_ewald_scatter(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                      const int32_t* posspec, int A, float* vol,
+                      int32_t* counts);
+
+/* main.cpp:203-225 from fields: window + FFT + Ewald accumulation. */
+int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
+                         const int32_t* posspec, int A, float* vol,
+                         int32_t* counts);
+
+/* main.cpp:243-352: counts 0->1 and divide, fftshift3D, backward 3-D FFT
+ * * delta_f^3, fftshift3D, * -1/(2 n0 kv^2). index_re/index_im: float V^3
+ * (indice.tif / absorption.tif). otf: uint8 V^3 (Re != 0, OTF3D.tif) or NULL. */
+int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
+                 const int32_t* counts, float* index_re, float* index_im,
+                 uint8_t* otf);
+
+/* The whole plain method, holograms to index volume, fused where stages
+ * allow (both programs' compute: pretraitement main.cpp:130-342 and
+ * Reconstruction/main.cpp:203-352). posspec may be NULL. */
+int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                          int dtype, int A, const uint8_t* mask,
+                          int32_t* posspec, float* index_re, float* index_im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTD_H */

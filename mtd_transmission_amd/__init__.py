@@ -1,0 +1,260 @@
+"""MI355X-native off-axis preprocessing and Ewald-cap Fourier reconstruction.
+
+Python binding of the C ABI in ``include/mtd.h`` (ctypes over the in-tree
+``libmtd.so``). Device arrays are torch CUDA tensors (HIP on ROCm); the
+library launches on torch's current stream. There is no CPU fallback: every
+compute call goes through the gfx950 kernels and fails loudly when the
+library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmtd.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+MTD_U8, MTD_U16, MTD_F32 = 0, 1, 2
+
+
+class MTDError(RuntimeError):
+    pass
+
+
+class Geom(ctypes.Structure):
+    """mtd_geom (include/mtd.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int32), ("nxmax", ctypes.c_int32),
+        ("dim_final", ctypes.c_int32), ("circle_cx", ctypes.c_int32),
+        ("circle_cy", ctypes.c_int32), ("born", ctypes.c_int32),
+        ("c_aber", ctypes.c_int32), ("deroul", ctypes.c_int32),
+        ("n0", ctypes.c_double), ("na", ctypes.c_double),
+        ("lambda_", ctypes.c_double), ("pix_holo", ctypes.c_double),
+        ("rayon", ctypes.c_double), ("pix_uborn", ctypes.c_double),
+        ("delta_f", ctypes.c_double), ("coef_na", ctypes.c_double),
+        ("alpha_holo", ctypes.c_double), ("alpha_field", ctypes.c_double),
+    ]
+
+    @property
+    def C(self):
+        return 2 * self.nxmax
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_GP = ctypes.POINTER(Geom)
+SIGNATURES = {
+    "mtd_version": (ctypes.c_char_p, []),
+    "mtd_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "mtd_ctx_destroy": (_I, [_P]),
+    "mtd_ctx_set_stream": (_I, [_P, _P]),
+    "mtd_ctx_sync": (_I, [_P]),
+    "mtd_last_error": (ctypes.c_char_p, [_P]),
+    "mtd_config_extract_val": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _D,
+                                    ctypes.POINTER(ctypes.c_float)]),
+    "mtd_config_extract_string": (_I, [ctypes.c_char_p, ctypes.c_char_p,
+                                       ctypes.c_char_p, ctypes.c_size_t]),
+    "mtd_geom_derive": (_I, [_D] * 13 + [_I, _I, _I, _GP]),
+    "mtd_tukey1d": (_I, [_I, _D, ctypes.POINTER(_D)]),
+    "mtd_holo_to_crop": (_I, [_P, _GP, _P, _I, _I, _P]),
+    "mtd_crop_to_field": (_I, [_P, _GP, _P, _I, _P, _P, _P]),
+    "mtd_preprocess": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P]),
+    "mtd_field_to_spectrum": (_I, [_P, _GP, _P, _P, _I, _P]),
+    "mtd_ewald_scatter": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
+    "mtd_ewald_accumulate": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
+    "mtd_finalize": (_I, [_P, _GP, _P, _P, _P, _P, _P]),
+    "mtd_reconstruct_plain": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P, _P]),
+}
+
+
+def lib():
+    """Load libmtd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MTDError(f"{LIB_PATH} is missing: build it first "
+                           "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------
+# host-only helpers
+# --------------------------------------------------------------------------
+def geom_derive(ccd_roix, nxmax, dim_final, circle_cx, circle_cy, na, n0, lam,
+                tpcam, f_tube=1.0, f_obj=1.0, rf=1.0, coef_na=1.0, born=1,
+                c_aber=1, deroul=0) -> Geom:
+    g = Geom()
+    rc = lib().mtd_geom_derive(ccd_roix, nxmax, dim_final, circle_cx, circle_cy,
+                               na, n0, lam, tpcam, f_tube, f_obj, rf, coef_na,
+                               born, c_aber, deroul, ctypes.byref(g))
+    if rc:
+        raise MTDError(f"mtd_geom_derive failed ({rc})")
+    return g
+
+
+def extract_val(path, token, default=None):
+    """Returns (value, found) with the reference's float semantics."""
+    out = ctypes.c_float()
+    rc = lib().mtd_config_extract_val(str(path).encode(), token.encode(),
+                                      int(default is not None),
+                                      float(default or 0.0), ctypes.byref(out))
+    if rc < 0:
+        raise MTDError(f"cannot read {path}")
+    return float(out.value), bool(rc)
+
+
+def extract_string(path, token):
+    buf = ctypes.create_string_buffer(4096)
+    rc = lib().mtd_config_extract_string(str(path).encode(), token.encode(), buf, 4096)
+    if rc < 0:
+        raise MTDError(f"cannot read {path}")
+    return buf.value.decode()
+
+
+def tukey1d(n, alpha):
+    arr = (ctypes.c_double * n)()
+    if lib().mtd_tukey1d(n, alpha, arr):
+        raise MTDError("mtd_tukey1d failed")
+    return [arr[i] for i in range(n)]
+
+
+# --------------------------------------------------------------------------
+# device context
+# --------------------------------------------------------------------------
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise MTDError("device arrays must be GPU tensors")
+    if not t.is_contiguous():
+        raise MTDError("device arrays must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dtype_code(t):
+    import torch
+    if t.dtype == torch.uint8:
+        return MTD_U8
+    if t.dtype in (torch.uint16, torch.int16):
+        return MTD_U16
+    if t.dtype == torch.float32:
+        return MTD_F32
+    raise MTDError(f"unsupported hologram dtype {t.dtype}")
+
+
+def _mask_ptr(mask, C):
+    if mask is None:
+        return None, None
+    import numpy as np
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    if m.shape != (C, C):
+        raise MTDError(f"mask must be {C}x{C}, got {m.shape}")
+    return m, m.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One mtd_ctx bound to a device; launches on torch's current stream."""
+
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise MTDError("no GPU visible: the MI355X path has no CPU fallback")
+        self.device = device
+        h = ctypes.c_void_p()
+        torch.cuda.set_device(device)
+        rc = lib().mtd_ctx_create(device, ctypes.byref(h))
+        if rc:
+            raise MTDError(f"mtd_ctx_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mtd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        lib().mtd_ctx_set_stream(self.h, ctypes.c_void_p(s))
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().mtd_last_error(self.h).decode()
+            raise MTDError(f"{what}: {msg} (code {rc})")
+
+    def sync(self):
+        self._check(lib().mtd_ctx_sync(self.h), "sync")
+
+    # ---- stages ---------------------------------------------------------
+    def holo_to_crop(self, g: Geom, holos, crops):
+        self._stream()
+        self._check(lib().mtd_holo_to_crop(self.h, ctypes.byref(g), _ptr(holos),
+                                           _dtype_code(holos), holos.shape[0],
+                                           _ptr(crops)), "holo_to_crop")
+
+    def crop_to_field(self, g: Geom, crops, mask, posspec, fields):
+        self._stream()
+        keep, mp = _mask_ptr(mask, g.C)
+        self._check(lib().mtd_crop_to_field(self.h, ctypes.byref(g), _ptr(crops),
+                                            crops.shape[0], mp, _ptr(posspec),
+                                            _ptr(fields)), "crop_to_field")
+
+    def preprocess(self, g: Geom, holos, mask, posspec, fields):
+        self._stream()
+        keep, mp = _mask_ptr(mask, g.C)
+        self._check(lib().mtd_preprocess(self.h, ctypes.byref(g), _ptr(holos),
+                                         _dtype_code(holos), holos.shape[0], mp,
+                                         _ptr(posspec), _ptr(fields)), "preprocess")
+
+    def field_to_spectrum(self, g: Geom, fields, posspec, spectra):
+        self._stream()
+        self._check(lib().mtd_field_to_spectrum(self.h, ctypes.byref(g), _ptr(fields),
+                                                _ptr(posspec), fields.shape[0],
+                                                _ptr(spectra)), "field_to_spectrum")
+
+    def ewald_scatter(self, g: Geom, spectra, posspec, vol, counts):
+        self._stream()
+        A = spectra.shape[0] if spectra is not None else 0
+        self._check(lib().mtd_ewald_scatter(self.h, ctypes.byref(g), _ptr(spectra),
+                                            _ptr(posspec), A, _ptr(vol), _ptr(counts)),
+                    "ewald_scatter")
+
+    def ewald_accumulate(self, g: Geom, fields, posspec, vol, counts):
+        self._stream()
+        self._check(lib().mtd_ewald_accumulate(self.h, ctypes.byref(g), _ptr(fields),
+                                               _ptr(posspec), fields.shape[0], _ptr(vol),
+                                               _ptr(counts)), "ewald_accumulate")
+
+    def finalize(self, g: Geom, vol, counts, index_re, index_im, otf=None):
+        self._stream()
+        self._check(lib().mtd_finalize(self.h, ctypes.byref(g), _ptr(vol), _ptr(counts),
+                                       _ptr(index_re), _ptr(index_im), _ptr(otf)),
+                    "finalize")
+
+    def reconstruct_plain(self, g: Geom, holos, mask, index_re, index_im, posspec=None):
+        self._stream()
+        keep, mp = _mask_ptr(mask, g.C)
+        self._check(lib().mtd_reconstruct_plain(self.h, ctypes.byref(g), _ptr(holos),
+                                                _dtype_code(holos), holos.shape[0], mp,
+                                                _ptr(posspec), _ptr(index_re),
+                                                _ptr(index_im)), "reconstruct_plain")

@@ -1,0 +1,116 @@
+// Internal declarations shared by the C-ABI layer and the kernel units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mtd.h"
+
+namespace mtd {
+
+// Per-angle Ewald geometry (computed on the device from posspec, FP64 heights
+// from a host table): lateral specular offset fm0 = posSpec - NXMAX, its cap
+// height fzm0 = sqrt(R^2 - |fm0|^2) and the skip flag of
+// Reconstruction/src/fonctions.cpp:897.
+struct AngleGeo {
+  int fx, fy;
+  int skip;
+  int pad;
+  double fz;
+};
+
+// Angle-independent least-squares plan (Correction_aberration.cpp:108-217):
+// sample list, LU factors of the normal matrix A^T A with row pivots.
+struct LsqPlan {
+  int M = 0;
+  int C = 0;
+  std::vector<uint8_t> mask;  // host copy used to detect changes
+  int2* d_samples = nullptr;  // (x, y) field-frame coordinates
+  double* d_lu = nullptr;     // 15x15 row-major, L unit-lower + U
+  int* d_piv = nullptr;       // 15 sequential row swaps
+};
+
+struct Buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace mtd
+
+struct mtd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int, float2*> twiddles;                       // per FFT length
+  std::map<std::pair<int, long long>, double*> tukey;     // (n, alpha*1e9)
+  std::map<std::pair<long long, long long>, double*> htab; // heights per (R, r2max)
+  std::map<long long, float*> betatab;                    // per geometry hash
+  std::map<int, mtd::Buffer> ws;                          // scratch slots
+  mtd::LsqPlan lsq;
+};
+
+namespace mtd {
+
+// scratch slots
+enum WsSlot {
+  WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
+  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP
+};
+
+int set_err(mtd_ctx* ctx, int code, const std::string& msg);
+int hip_check(mtd_ctx* ctx, hipError_t e, const char* what);
+void* ws_get(mtd_ctx* ctx, int slot, size_t bytes);
+const float2* twiddle(mtd_ctx* ctx, int L);
+const double* tukey_table(mtd_ctx* ctx, int n, double alpha);
+void tukey1d_host(int n, double alpha, double* out);
+int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask);
+
+bool fft_len_supported(int L);
+
+// ---- kernel launchers (return hipError_t of the launch) -------------------
+hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
+                            const double* win, const float2* tw, int x0,
+                            int C, float2* t1, hipStream_t s);
+hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
+                            int C, int x0, int y0, float2* crop,
+                            hipStream_t s);
+
+hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
+                         int* posy, hipStream_t s);
+hipError_t launch_demod(int C, const float2* crop, const int* posx,
+                        const int* posy, int A, const float2* tw,
+                        float2* work, float2* iu, hipStream_t s);
+hipError_t launch_lsq(int C, const float2* iu, int A, const LsqPlan& plan,
+                      double* coef, hipStream_t s);
+// mode: bit0 born, bit1 c_aber
+hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
+                             const int* posx, const int* posy, int A,
+                             const double* coef, int mode, float2* fields,
+                             hipStream_t s);
+// Per-angle forward spectrum. from_iu=0: src are fields (U frame);
+// from_iu=1: src is IU and the field formation runs inside the load.
+// Output frame: shift_pos=0 -> unshifted F, 1 -> shifted by +posSpec.
+hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
+                               const float2* crop, const double* coef,
+                               int mode, const int* posx, const int* posy,
+                               int A, const double* win, const float2* tw,
+                               float dx2, int shift_pos, float2* work,
+                               float2* out, hipStream_t s);
+
+hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
+                            double R2, const double* htab, int r2tab,
+                            AngleGeo* ageo, hipStream_t s);
+hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
+                         const int* posy, const AngleGeo* ageo, int A, int C,
+                         int nxmax, int V, const double* htab,
+                         const float* beta, int r2max, float2* vol,
+                         int32_t* cnt, hipStream_t s);
+hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
+                           float2* work, float df3, float cte, float gsign,
+                           float* out_re, float* out_im, uint8_t* otf,
+                           const float2* tw, hipStream_t s);
+
+}  // namespace mtd

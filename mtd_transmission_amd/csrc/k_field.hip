@@ -1,0 +1,430 @@
+// Field stage on the C x C crops (C = 2*NXMAX), batched over angles.
+//
+//   argmax     coordSpec (pretraitement_hors_axe_CPU/fonctions.cpp:88-101)
+//   demod      calc_Uborn2 (fonctions.cpp:348-374): the -posSpec shift is an
+//              index remap on load, then an unnormalised 2-D backward FFT.
+//              The result IU is kept in the frame before the (NXMAX, NXMAX)
+//              shift: U[y][x] = IU[(y+NXMAX)%C][(x+NXMAX)%C].
+//   lsq        aberCorr2 / ampliCorr2 right-hand sides and solves in FP64
+//              (Correction_aberration.cpp:166-217) with the LU factors of the
+//              angle-independent normal matrix.
+//   form       main.cpp:289-316 (Born / Rytov) or main.cpp:332-337 (C_ABER 0).
+//   spectrum   Reconstruction/main.cpp:207-217: window, shift by NXMAX,
+//              forward 2-D FFT * pixUBorn^2, optional shift by +posSpec.
+#include "fft_lds.hpp"
+#include "internal.hpp"
+#include "passes.hpp"
+
+namespace mtd {
+
+// ---------------------------------------------------------------------------
+// argmax with the slot-0 squared quirk; first index of the maximum.
+// ---------------------------------------------------------------------------
+template <int C>
+__global__ void __launch_bounds__(256)
+k_argmax(const float2* __restrict__ crop, int* __restrict__ posx,
+         int* __restrict__ posy) {
+  const int a = blockIdx.x;
+  const float2* p = crop + (size_t)a * C * C;
+  float best = -1.f;
+  int bi = 0x7fffffff;
+  for (int i = threadIdx.x; i < C * C; i += 256) {
+    const float2 v = p[i];
+    const float m = (i == 0) ? __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y))
+                             : hypotf(v.x, v.y);
+    if (m > best) { best = m; bi = i; }
+  }
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float v2 = sv[threadIdx.x + s];
+      const int i2 = si[threadIdx.x + s];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sv[threadIdx.x] = v2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    posx[a] = si[0] % C;
+    posy[a] = si[0] / C;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// demodulation: rows then columns of the backward FFT
+// ---------------------------------------------------------------------------
+template <int C>
+struct DemodRowOp {
+  const float2* crop;
+  const int* posx;
+  const int* posy;
+  float2* out;
+  const float2* tw;
+  int px, py;
+  __device__ void begin(int a) { px = posx[a]; py = posy[a]; }
+  __device__ float2 load(int a, int v, int u) const {
+    return crop[((size_t)a * C + (v + py) % C) * C + (u + px) % C];
+  }
+  __device__ void store(int a, int v, int u, float2 x) const {
+    out[((size_t)a * C + v) * C + u] = x;
+  }
+};
+
+template <int C>
+struct ColInPlaceOp {  // column pass reading and writing one stack
+  const float2* in;
+  float2* out;
+  const float2* tw;
+  __device__ void begin(int) {}
+  __device__ float2 load(int a, int col, int row) const {
+    return in[((size_t)a * C + row) * C + col];
+  }
+  __device__ void store(int a, int col, int row, float2 x) const {
+    out[((size_t)a * C + row) * C + col] = x;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// least squares: per angle, FP64 right-hand sides A^T b for the phase and
+// amplitude fits over the sample set, then the LU solve.
+// ---------------------------------------------------------------------------
+constexpr int NCOEF = 15;
+
+__device__ __forceinline__ void monomials(double X, double Y, double* mono) {
+  double xp[5], yp[5];
+  xp[0] = 1.0; yp[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i < 5; ++i) { xp[i] = xp[i - 1] * X; yp[i] = yp[i - 1] * Y; }
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i <= 4; ++i)
+#pragma unroll
+    for (int j = 0; j <= 4 - i; ++j) mono[k++] = xp[i] * yp[j];
+}
+
+__device__ __forceinline__ void lu_solve15(const double* __restrict__ lu,
+                                           const int* __restrict__ piv,
+                                           double* b) {
+  for (int k = 0; k < NCOEF; ++k) {
+    const int p = piv[k];
+    if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+  }
+  for (int i = 1; i < NCOEF; ++i) {
+    double s = b[i];
+    for (int j = 0; j < i; ++j) s -= lu[i * NCOEF + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = NCOEF - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < NCOEF; ++j) s -= lu[i * NCOEF + j] * b[j];
+    b[i] = s / lu[i * NCOEF + i];
+  }
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+k_lsq(const float2* __restrict__ iu, const int2* __restrict__ samples, int M,
+      const double* __restrict__ lu, const int* __restrict__ piv,
+      double* __restrict__ coef) {
+  const int a = blockIdx.x;
+  const float2* u = iu + (size_t)a * C * C;
+  double rp[NCOEF], ra[NCOEF];
+#pragma unroll
+  for (int k = 0; k < NCOEF; ++k) { rp[k] = 0.0; ra[k] = 0.0; }
+  for (int m = threadIdx.x; m < M; m += 256) {
+    const int2 s = samples[m];
+    const float2 v = u[((s.y + C / 2) % C) * C + (s.x + C / 2) % C];
+    const double ph = (double)atan2f(v.y, v.x);
+    const double am = (double)hypotf(v.x, v.y);
+    double mono[NCOEF];
+    monomials((double)s.x, (double)s.y, mono);
+#pragma unroll
+    for (int k = 0; k < NCOEF; ++k) {
+      rp[k] += ph * mono[k];
+      ra[k] += am * mono[k];
+    }
+  }
+  // deterministic reduction: butterfly within the wave, then waves in order
+  __shared__ double red[4][2 * NCOEF];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NCOEF; ++k) {
+    double x = rp[k], y = ra[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+    }
+    if (lane == 0) { red[wave][k] = x; red[wave][NCOEF + k] = y; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double b[NCOEF];
+    for (int k = 0; k < NCOEF; ++k) {
+      const int idx = threadIdx.x * NCOEF + k;
+      b[k] = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+    }
+    lu_solve15(lu, piv, b);
+    for (int k = 0; k < NCOEF; ++k)
+      coef[((size_t)a * 2 + threadIdx.x) * NCOEF + k] = b[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// field formation for one pixel of the U frame at (X, Y)
+// ---------------------------------------------------------------------------
+struct FieldMode {
+  int born;
+  int c_aber;
+};
+
+__device__ __forceinline__ float2 form_pixel(float2 u, int X, int Y,
+                                             const double* __restrict__ cph,
+                                             const double* __restrict__ cam,
+                                             float2 spec_norm, int mode) {
+  if (!(mode & 2)) {  // C_ABER 0: U * conj(s)/|s|
+    return make_float2(u.x * spec_norm.x - u.y * spec_norm.y,
+                       u.x * spec_norm.y + u.y * spec_norm.x);
+  }
+  const float phf = atan2f(u.y, u.x);
+  const float amf = hypotf(u.x, u.y);
+  double mono[NCOEF];
+  monomials((double)X, (double)Y, mono);
+  double bgp = 0.0, bga = 0.0;
+#pragma unroll
+  for (int k = 0; k < NCOEF; ++k) {
+    bgp += cph[k] * mono[k];
+    bga += cam[k] * mono[k];
+  }
+  const double phic = (double)phf - bgp;
+  double A = (double)amf / (bga + 0.00000001);
+  if (A > 10) A = 1;
+  if (A < -10) A = 1;
+  const float pc = (float)phic;
+  const float aa = (float)fabs(A);
+  if (mode & 1) return make_float2(aa * cosf(pc) - 1.0f, aa * sinf(pc));
+  return make_float2(logf(aa), pc);
+}
+
+// conj(s)/|s| of the specular value, in double then rounded
+__device__ __forceinline__ float2 spec_norm_of(const float2* crop_a, int C,
+                                               int px, int py) {
+  const float2 s = crop_a[py * C + px];
+  const double re = s.x, im = s.y;
+  const double mod = sqrt(re * re + im * im);
+  return make_float2((float)(re / mod), (float)(-im / mod));
+}
+
+template <int C>
+__global__ void __launch_bounds__(256)
+k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
+             const int* __restrict__ posx, const int* __restrict__ posy,
+             const double* __restrict__ coef, int mode,
+             float2* __restrict__ fields) {
+  const int a = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= C * C) return;
+  const int X = i % C, Y = i / C;
+  const double* cph = coef + (size_t)a * 2 * NCOEF;
+  const float2 sn = (mode & 2) ? make_float2(0.f, 0.f)
+                               : spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
+  const float2 u = iu[((size_t)a * C + (Y + C / 2) % C) * C + (X + C / 2) % C];
+  fields[(size_t)a * C * C + i] = form_pixel(u, X, Y, cph, cph + NCOEF, sn, mode);
+}
+
+// ---------------------------------------------------------------------------
+// per-angle forward spectrum
+// G[y][x] = field_U[(y+NX)%C][(x+NX)%C] * w[(y+NX)%C] * w[(x+NX)%C]
+// ---------------------------------------------------------------------------
+template <int C>
+struct FwdRowOp {
+  const float2* src;      // fields (U frame) or IU
+  const float2* crop;
+  const double* coef;
+  const int* posx;
+  const int* posy;
+  const double* win;      // tukey1d(C, 0.1), double
+  float2* out;
+  const float2* tw;
+  int from_iu, mode;
+  float2 sn;
+  const double* cph;
+  __device__ void begin(int a) {
+    cph = coef + (size_t)a * 2 * NCOEF;
+    if (from_iu && !(mode & 2))
+      sn = spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
+  }
+  __device__ float2 load(int a, int y, int x) const {
+    const int X = (x + C / 2) % C, Y = (y + C / 2) % C;
+    const float w = (float)(win[X] * win[Y]);
+    float2 f;
+    if (from_iu) {
+      f = form_pixel(src[((size_t)a * C + y) * C + x], X, Y, cph, cph + NCOEF, sn, mode);
+    } else {
+      f = src[((size_t)a * C + Y) * C + X];
+    }
+    return make_float2(f.x * w, f.y * w);
+  }
+  __device__ void store(int a, int y, int x, float2 v) const {
+    out[((size_t)a * C + y) * C + x] = v;
+  }
+};
+
+template <int C>
+struct FwdColOp {
+  const float2* in;
+  float2* out;
+  const int* posx;
+  const int* posy;
+  const float2* tw;
+  float dx2;
+  int shift_pos;
+  int px, py;
+  __device__ void begin(int a) {
+    if (shift_pos) { px = posx[a]; py = posy[a]; } else { px = 0; py = 0; }
+  }
+  __device__ float2 load(int a, int col, int row) const {
+    return in[((size_t)a * C + row) * C + col];
+  }
+  __device__ void store(int a, int col, int row, float2 v) const {
+    out[((size_t)a * C + (row + py) % C) * C + (col + px) % C] =
+        make_float2(v.x * dx2, v.y * dx2);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int C>
+static hipError_t argmax_c(const float2* crop, int A, int* posx, int* posy,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((k_argmax<C>), dim3(A), dim3(256), 0, s, crop, posx, posy);
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
+                          int A, const float2* tw, float2* work, float2* iu,
+                          hipStream_t s) {
+  DemodRowOp<C> r{crop, posx, posy, work, tw, 0, 0};
+  hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A),
+                     rows_block<C>(), 0, s, r);
+  ColInPlaceOp<C> c{work, iu, tw};
+  constexpr int NC = cols_nc<C>();
+  hipLaunchKernelGGL((k_cols<C, true, NC, ColInPlaceOp<C>>), dim3(C / NC, A),
+                     dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t lsq_c(const float2* iu, int A, const LsqPlan& plan,
+                        double* coef, hipStream_t s) {
+  hipLaunchKernelGGL((k_lsq<C>), dim3(A), dim3(256), 0, s, iu, plan.d_samples,
+                     plan.M, plan.d_lu, plan.d_piv, coef);
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t form_c(const float2* iu, const float2* crop, const int* posx,
+                         const int* posy, int A, const double* coef, int mode,
+                         float2* fields, hipStream_t s) {
+  hipLaunchKernelGGL((k_form_field<C>), dim3((C * C + 255) / 256, A), dim3(256),
+                     0, s, iu, crop, posx, posy, coef, mode, fields);
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
+                        const double* coef, int mode, const int* posx,
+                        const int* posy, int A, const double* win,
+                        const float2* tw, float dx2, int shift_pos,
+                        float2* work, float2* out, hipStream_t s) {
+  FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode,
+                make_float2(0.f, 0.f), nullptr};
+  hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
+                     rows_block<C>(), 0, s, r);
+  FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, 0, 0};
+  constexpr int NC = cols_nc<C>();
+  hipLaunchKernelGGL((k_cols<C, false, NC, FwdColOp<C>>), dim3(C / NC, A),
+                     dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
+#define MTD_CROP_SIZES(X) \
+  X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(256) X(288) X(320) X(352) X(384) X(512)
+
+hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
+                         int* posy, hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return argmax_c<n>(crop, A, posx, posy, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_demod(int C, const float2* crop, const int* posx,
+                        const int* posy, int A, const float2* tw,
+                        float2* work, float2* iu, hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return demod_c<n>(crop, posx, posy, A, tw, work, iu, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_lsq(int C, const float2* iu, int A, const LsqPlan& plan,
+                      double* coef, hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return lsq_c<n>(iu, A, plan, coef, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
+                             const int* posx, const int* posy, int A,
+                             const double* coef, int mode, float2* fields,
+                             hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return form_c<n>(iu, crop, posx, posy, A, coef, mode, fields, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
+                               const float2* crop, const double* coef,
+                               int mode, const int* posx, const int* posy,
+                               int A, const double* win, const float2* tw,
+                               float dx2, int shift_pos, float2* work,
+                               float2* out, hipStream_t s) {
+  switch (C) {
+#define X(n)                                                                    \
+  case n:                                                                       \
+    return fwd_c<n>(src, from_iu, crop, coef, mode, posx, posy, A, win, tw, dx2, \
+                    shift_pos, work, out, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+bool crop_len_supported(int C) {
+  switch (C) {
+#define X(n) case n: return true;
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return false;
+}
+
+}  // namespace mtd

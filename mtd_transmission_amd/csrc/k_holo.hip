@@ -1,0 +1,148 @@
+// Hologram stage: windowed, centred 2-D FFT of each N x N hologram, keeping
+// only the C x C window around the +1 order (holo2TF_UBorn,
+// pretraitement_hors_axe_CPU/fonctions.cpp:279-304).
+//
+// The centred transform fftshift(FFT(fftshift(h)))/N^2 equals
+// (-1)^(kx+ky) * FFT(h * (-1)^(x+y))[ky][kx] / N^2 for even N, so both shifts
+// become sign flips applied on load and on store.
+//
+// Pass 1 (rows): two real rows are packed into one complex FFT of length N;
+// only the C bins kx in [x0, x0+C) are separated and written, to
+// t1[a][y][d] (N x C per hologram).
+// Pass 2 (columns): each of the C columns of t1 is transformed along y and
+// only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
+#include "fft_lds.hpp"
+#include "internal.hpp"
+
+namespace mtd {
+
+template <int N, typename TIn>
+__global__ void __launch_bounds__(256)
+k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
+            const float2* __restrict__ tw, int x0, int C,
+            float2* __restrict__ t1) {
+  constexpr int T = fft_threads(N);
+  constexpr int NP = divisor_le(N / 2, fft_per_block(N));  // row pairs per block
+  __shared__ float2 lds[NP][N];
+  const int a = blockIdx.y;
+  const int p = threadIdx.x / T, t = threadIdx.x % T;
+  const int y0 = 2 * (blockIdx.x * NP + p), y1 = y0 + 1;
+  const TIn* h0 = holos + ((size_t)a * N + y0) * N;
+  const TIn* h1 = h0 + N;
+  const double wy0 = win[y0], wy1 = win[y1];
+  float2* buf = lds[p];
+  for (int x = t; x < N; x += T) {
+    const double wx = win[x];
+    const float w0 = (float)(wx * wy0), w1 = (float)(wx * wy1);
+    float g0 = (float)h0[x] * w0;
+    float g1 = (float)h1[x] * w1;
+    if ((x + y0) & 1) g0 = -g0; else g1 = -g1;
+    buf[x] = make_float2(g0, g1);
+  }
+  __syncthreads();
+  fft_lds<N, T, false>(buf, t, tw);
+  float2* o0 = t1 + ((size_t)a * N + y0) * C;
+  float2* o1 = o0 + C;
+  for (int d = t; d < C; d += T) {
+    const int k = x0 + d;
+    const int m = (N - k) % N;
+    const float2 zk = buf[k], zm = buf[m];
+    // X0 = Z[k] + conj(Z[N-k]), X1 = -i (Z[k] - conj(Z[N-k])); the 1/2 is
+    // folded into the column pass scale.
+    o0[d] = make_float2(zk.x + zm.x, zk.y - zm.y);
+    o1[d] = make_float2(zk.y + zm.y, zm.x - zk.x);
+  }
+}
+
+template <int N, int NC>
+__global__ void __launch_bounds__(256)
+k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
+            int C, int x0, int y0, float scale, float2* __restrict__ crop) {
+  constexpr int T = 256 / NC;
+  __shared__ float2 lds[NC][N + 1];
+  const int a = blockIdx.y, d0 = blockIdx.x * NC;
+  const float2* src = t1 + (size_t)a * N * C;
+  for (int e = threadIdx.x; e < N * NC; e += 256) {
+    const int y = e / NC, c = e % NC;
+    lds[c][y] = src[(size_t)y * C + d0 + c];
+  }
+  __syncthreads();
+  const int c = threadIdx.x / T, t = threadIdx.x % T;
+  fft_lds<N, T, false>(lds[c], t, tw);
+  float2* dst = crop + (size_t)a * C * C;
+  for (int f = threadIdx.x; f < C * NC; f += 256) {
+    const int e = f / NC, cc = f % NC;
+    const int ky = y0 + e, kx = x0 + d0 + cc;
+    const float2 v = lds[cc][ky];
+    const float s = ((kx + ky) & 1) ? -scale : scale;
+    dst[(size_t)e * C + d0 + cc] = make_float2(v.x * s, v.y * s);
+  }
+}
+
+template <int N>
+static hipError_t rows_n(int dtype, const void* holos, int A, const double* win,
+                         const float2* tw, int x0, int C, float2* t1,
+                         hipStream_t s) {
+  constexpr int NP = divisor_le(N / 2, fft_per_block(N));
+  dim3 grid(N / (2 * NP), A), block(NP * fft_threads(N));
+  switch (dtype) {
+    case MTD_U8:
+      hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
+                         (const uint8_t*)holos, win, tw, x0, C, t1);
+      break;
+    case MTD_U16:
+      hipLaunchKernelGGL((k_holo_rows<N, uint16_t>), grid, block, 0, s,
+                         (const uint16_t*)holos, win, tw, x0, C, t1);
+      break;
+    default:
+      hipLaunchKernelGGL((k_holo_rows<N, float>), grid, block, 0, s,
+                         (const float*)holos, win, tw, x0, C, t1);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
+                         int x0, int y0, float2* crop, hipStream_t s) {
+  constexpr int NC = N >= 2048 ? 4 : 8;
+  const float scale = (float)(0.5 / ((double)N * (double)N));
+  hipLaunchKernelGGL((k_holo_cols<N, NC>), dim3(C / NC, A), dim3(256), 0, s,
+                     t1, tw, C, x0, y0, scale, crop);
+  return hipGetLastError();
+}
+
+#define MTD_HOLO_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
+
+hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
+                            const double* win, const float2* tw, int x0,
+                            int C, float2* t1, hipStream_t s) {
+  switch (N) {
+#define X(n) case n: return rows_n<n>(dtype, holos, A, win, tw, x0, C, t1, s);
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
+                            int C, int x0, int y0, float2* crop,
+                            hipStream_t s) {
+  switch (N) {
+#define X(n) case n: return cols_n<n>(t1, A, tw, C, x0, y0, crop, s);
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+bool holo_len_supported(int N) {
+  switch (N) {
+#define X(n) case n: return true;
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return false;
+}
+
+}  // namespace mtd

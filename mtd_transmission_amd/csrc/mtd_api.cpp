@@ -1,0 +1,591 @@
+// C-ABI entry points (include/mtd.h): validation, workspaces, host-side
+// tables and the stage pipelines over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace mtd {
+bool holo_len_supported(int N);
+bool crop_len_supported(int C);
+bool vol_len_supported(int V);
+
+int set_err(mtd_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_check(mtd_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return MTD_OK;
+  return set_err(ctx, MTD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* ws_get(mtd_ctx* ctx, int slot, size_t bytes) {
+  Buffer& b = ctx->ws[slot];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) {
+    hipStreamSynchronize(ctx->stream);
+    hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
+    b.ptr = nullptr;
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.ptr;
+}
+
+const float2* twiddle(mtd_ctx* ctx, int L) {
+  auto it = ctx->twiddles.find(L);
+  if (it != ctx->twiddles.end()) return it->second;
+  std::vector<float2> h(L);
+  for (int k = 0; k < L; ++k) {
+    // exp(-2 pi i k / L) from an exactly reduced angle
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+    h[k] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) return nullptr;
+  hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice);
+  ctx->twiddles[L] = d;
+  return d;
+}
+
+// tukey2D 1-D factor (pretraitement_hors_axe_CPU/FFT_fonctions.cpp:61-87):
+// alpha is a C float, 2*cpt/(alpha*(N-1)) a float division, pi = 3.1415.
+void tukey1d_host(int N, double alpha_d, double* out) {
+  const float alpha = (float)alpha_d;
+  const int borne1 = (int)std::round(alpha * (float)(N - 1) / 2.0f);
+  const int borne2 = (int)std::round((float)(N - 1) * (1.0f - alpha / 2.0f));
+  for (int cpt = 0; cpt < N; ++cpt) {
+    if (cpt < borne1 + 1) {
+      const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 1.0f;
+      out[cpt] = 0.5 * (1 + std::cos(3.1415 * (double)arg));
+    } else if (cpt < borne2 + 1) {
+      out[cpt] = 1.0;
+    } else {
+      const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 2.0f / alpha + 1.0f;
+      out[cpt] = 0.5 * (1 + std::cos(3.1415 * (double)arg));
+    }
+  }
+}
+
+const double* tukey_table(mtd_ctx* ctx, int n, double alpha) {
+  const auto key = std::make_pair(n, (long long)std::llround(alpha * 1e9));
+  auto it = ctx->tukey.find(key);
+  if (it != ctx->tukey.end()) return it->second;
+  std::vector<double> h(n);
+  tukey1d_host(n, alpha, h.data());
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * n) != hipSuccess) return nullptr;
+  hipMemcpy(d, h.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  ctx->tukey[key] = d;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// least-squares plan (Correction_aberration.cpp:86-149,186-217)
+// ---------------------------------------------------------------------------
+int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
+  std::vector<uint8_t> m(C * C, 255);
+  if (mask) std::memcpy(m.data(), mask, C * C);
+  LsqPlan& p = ctx->lsq;
+  if (p.d_samples && p.C == C && p.mask == m) return MTD_OK;
+  std::vector<int2> s;
+  for (int y = 0; y < C; y += 5)
+    for (int x = 0; x < C; x += 5)
+      if (m[y * C + x] > 45) s.push_back(make_int2(x, y));
+  const int M = (int)s.size();
+  if (M < 15)
+    return set_err(ctx, MTD_ERR_ARG,
+                   "aberration mask leaves " + std::to_string(M) +
+                       " sample points; the degree-4 fit needs at least 15");
+  // normal matrix A^T A with monomials x^i y^j (i outer, j inner)
+  double G[15][15] = {};
+  for (const int2& q : s) {
+    double xp[5] = {1, 0, 0, 0, 0}, yp[5] = {1, 0, 0, 0, 0}, mono[15];
+    for (int i = 1; i < 5; ++i) { xp[i] = xp[i - 1] * q.x; yp[i] = yp[i - 1] * q.y; }
+    int k = 0;
+    for (int i = 0; i <= 4; ++i)
+      for (int j = 0; j <= 4 - i; ++j) mono[k++] = xp[i] * yp[j];
+    for (int a = 0; a < 15; ++a)
+      for (int b = 0; b < 15; ++b) G[a][b] += mono[a] * mono[b];
+  }
+  // LU with partial pivoting, sequential row swaps
+  int piv[15];
+  for (int k = 0; k < 15; ++k) {
+    int pr = k;
+    for (int i = k + 1; i < 15; ++i)
+      if (std::fabs(G[i][k]) > std::fabs(G[pr][k])) pr = i;
+    piv[k] = pr;
+    if (G[pr][k] == 0.0)
+      return set_err(ctx, MTD_ERR_ARG, "aberration fit: singular normal matrix");
+    if (pr != k)
+      for (int j = 0; j < 15; ++j) std::swap(G[k][j], G[pr][j]);
+    for (int i = k + 1; i < 15; ++i) {
+      G[i][k] /= G[k][k];
+      for (int j = k + 1; j < 15; ++j) G[i][j] -= G[i][k] * G[k][j];
+    }
+  }
+  hipFree(p.d_samples);
+  hipFree(p.d_lu);
+  hipFree(p.d_piv);
+  p.d_samples = nullptr;
+  p.d_lu = nullptr;
+  p.d_piv = nullptr;
+  if (hipMalloc(&p.d_samples, sizeof(int2) * M) != hipSuccess ||
+      hipMalloc(&p.d_lu, sizeof(double) * 225) != hipSuccess ||
+      hipMalloc(&p.d_piv, sizeof(int) * 15) != hipSuccess)
+    return set_err(ctx, MTD_ERR_HIP, "out of device memory (fit plan)");
+  hipMemcpy(p.d_samples, s.data(), sizeof(int2) * M, hipMemcpyHostToDevice);
+  hipMemcpy(p.d_lu, &G[0][0], sizeof(double) * 225, hipMemcpyHostToDevice);
+  hipMemcpy(p.d_piv, piv, sizeof(int) * 15, hipMemcpyHostToDevice);
+  p.M = M;
+  p.C = C;
+  p.mask = m;
+  return MTD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ewald tables (Reconstruction/src/fonctions.cpp:875-935)
+// ---------------------------------------------------------------------------
+struct EwaldTables {
+  const double* htab;
+  const float* beta;
+  int r2tab;
+  int r2max;
+  double R2;
+};
+
+static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
+  const int nx = g->nxmax;
+  const double R = g->rayon;
+  const double R2 = R * R;
+  // pupil: (double)(fdx^2+fdy^2) < coef*NXMAX^2 (fonctions.cpp:915)
+  const double lim = g->coef_na * (double)(nx * nx);
+  int r2max = -1;
+  for (int r = 2 * nx * nx; r >= 0; --r)
+    if ((double)r < lim) { r2max = r; break; }
+  if (r2max < 0) return set_err(ctx, MTD_ERR_ARG, "empty pupil (COEF_NA_OBJ_LIMIT <= 0)");
+  if ((double)r2max > R2)
+    return set_err(ctx, MTD_ERR_ARG,
+                   "COEF_NA_OBJ_LIMIT*NXMAX^2 exceeds rayon^2: the reference would take "
+                   "the square root of a negative height");
+  const int r2tab = 2 * nx * nx + 1;
+  const auto key = std::make_pair((long long)std::llround(R), (long long)r2tab);
+  double* d_h;
+  auto it = ctx->htab.find(key);
+  std::vector<double> h(r2tab);
+  for (int r = 0; r < r2tab; ++r) {
+    const double v = R2 - (double)r;
+    h[r] = v >= 0 ? std::sqrt(v) : 0.0;
+  }
+  if (it == ctx->htab.end()) {
+    if (hipMalloc(&d_h, sizeof(double) * r2tab) != hipSuccess)
+      return set_err(ctx, MTD_ERR_HIP, "out of device memory (height table)");
+    hipMemcpy(d_h, h.data(), sizeof(double) * r2tab, hipMemcpyHostToDevice);
+    ctx->htab[key] = d_h;
+  } else {
+    d_h = it->second;
+  }
+  // beta = ((k0/PI) * (-2 PI)) * (h * (1/R)); PI = 3.14159, float n0/kv/k0
+  const double PI = 3.14159;
+  const float n0f = (float)g->n0;
+  const float kv = (float)(2 * PI / g->lambda);
+  const float k0 = kv * n0f;
+  const double gamma = ((double)k0 / PI) * (-2 * PI);
+  const double inv_r = 1.0 / R;
+  std::vector<float> b(r2max + 1);
+  for (int r = 0; r <= r2max; ++r) b[r] = (float)(gamma * (h[r] * inv_r));
+  long long bkey = std::llround(R) * 1000003LL + r2max;
+  bkey = bkey * 1000003LL + (long long)(*(const uint32_t*)&k0);
+  float* d_b;
+  auto bt = ctx->betatab.find(bkey);
+  if (bt == ctx->betatab.end()) {
+    if (hipMalloc(&d_b, sizeof(float) * (r2max + 1)) != hipSuccess)
+      return set_err(ctx, MTD_ERR_HIP, "out of device memory (beta table)");
+    hipMemcpy(d_b, b.data(), sizeof(float) * (r2max + 1), hipMemcpyHostToDevice);
+    ctx->betatab[bkey] = d_b;
+  } else {
+    d_b = bt->second;
+  }
+  t->htab = d_h;
+  t->beta = d_b;
+  t->r2tab = r2tab;
+  t->r2max = r2max;
+  t->R2 = R2;
+  return MTD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// validation
+// ---------------------------------------------------------------------------
+enum { NEED_HOLO = 1, NEED_VOL = 2 };
+
+static int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
+  if (!ctx) return MTD_ERR_ARG;
+  if (!g) return set_err(ctx, MTD_ERR_ARG, "null geometry");
+  if (A < 0) return set_err(ctx, MTD_ERR_ARG, "negative angle count");
+  const int C = 2 * g->nxmax;
+  if (g->nxmax <= 0) return set_err(ctx, MTD_ERR_ARG, "NXMAX must be positive");
+  if (!crop_len_supported(C))
+    return set_err(ctx, MTD_ERR_UNSUPPORTED, "crop side 2*NXMAX=" + std::to_string(C) + " not built");
+  if (g->deroul)
+    return set_err(ctx, MTD_ERR_UNSUPPORTED, "DEROUL 1 (phase unwrapping) is not implemented");
+  if (need & NEED_HOLO) {
+    if (!holo_len_supported(g->n))
+      return set_err(ctx, MTD_ERR_UNSUPPORTED, "hologram side " + std::to_string(g->n) + " not built");
+    const int x0 = g->circle_cx - g->nxmax, y0 = g->circle_cy - g->nxmax;
+    if (x0 < 0 || y0 < 0 || x0 + C > g->n || y0 + C > g->n)
+      return set_err(ctx, MTD_ERR_ARG,
+                     "crop window (CIRCLE_CX/CY +- NXMAX) leaves the hologram");
+  }
+  if (need & NEED_VOL) {
+    const int V = g->dim_final;
+    if (!vol_len_supported(V))
+      return set_err(ctx, MTD_ERR_UNSUPPORTED, "DIM_FINAL " + std::to_string(V) + " not built");
+    if (!(g->rayon > 0) || g->rayon != std::floor(g->rayon))
+      return set_err(ctx, MTD_ERR_ARG, "rayon must be a positive integer value");
+    // every voxel index must stay inside V^3 (the reference does not check,
+    // Reconstruction/src/fonctions.cpp:926-928)
+    if (V < 4 * g->nxmax - 2)
+      return set_err(ctx, MTD_ERR_ARG, "DIM_FINAL < 4*NXMAX-2: lateral voxel index out of range");
+    if (g->rayon > V / 2 - 1)
+      return set_err(ctx, MTD_ERR_ARG, "rayon > DIM_FINAL/2-1: axial voxel index out of range");
+    if (!(g->delta_f > 0) || !(g->pix_uborn > 0) || !(g->lambda > 0) || !(g->n0 > 0))
+      return set_err(ctx, MTD_ERR_ARG, "non-positive optical constants");
+  }
+  return MTD_OK;
+}
+
+#define MTD_TRY(expr)              \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != MTD_OK) return _rc; \
+  } while (0)
+
+static int chunk_of(size_t bytes_per_angle, size_t budget, int A) {
+  size_t c = budget / std::max<size_t>(bytes_per_angle, 1);
+  if (c < 1) c = 1;
+  if ((int)c > A) c = A;
+  return (int)c;
+}
+
+static size_t dtype_bytes(int dtype) {
+  return dtype == MTD_U8 ? 1 : dtype == MTD_U16 ? 2 : 4;
+}
+
+// holograms [a0, a0+n) -> crops
+static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                      int dtype, int a0, int n, float2* crop_out) {
+  const int N = g->n, C = 2 * g->nxmax;
+  const double* win = tukey_table(ctx, N, g->alpha_holo);
+  const float2* twN = twiddle(ctx, N);
+  float2* t1 = (float2*)ws_get(ctx, WS_T1, (size_t)n * N * C * sizeof(float2));
+  if (!win || !twN || !t1) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram stage)");
+  const char* src = (const char*)holos + (size_t)a0 * N * N * dtype_bytes(dtype);
+  MTD_TRY(hip_check(ctx, launch_holo_rows(N, dtype, src, n, win, twN, g->circle_cx - g->nxmax,
+                                          C, t1, ctx->stream), "hologram row pass"));
+  MTD_TRY(hip_check(ctx, launch_holo_cols(N, t1, n, twN, C, g->circle_cx - g->nxmax,
+                                          g->circle_cy - g->nxmax, crop_out, ctx->stream),
+                    "hologram column pass"));
+  return MTD_OK;
+}
+
+static int field_mode(const mtd_geom* g) { return (g->born ? 1 : 0) | (g->c_aber ? 2 : 0); }
+
+// crops [n] (angles a0..) -> fields (out_fields) or unshifted spectra F (out_F)
+static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int n,
+                       int* posx, int* posy, float2* out_fields, float2* out_F) {
+  const int C = 2 * g->nxmax;
+  const float2* twC = twiddle(ctx, C);
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)n * C * C * sizeof(float2));
+  float2* iu = (float2*)ws_get(ctx, WS_IU, (size_t)n * C * C * sizeof(float2));
+  double* coef = (double*)ws_get(ctx, WS_COEF, (size_t)n * 30 * sizeof(double));
+  if (!twC || !work || !iu || !coef) return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
+  MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, ctx->stream), "argmax"));
+  MTD_TRY(hip_check(ctx, launch_demod(C, crop, posx, posy, n, twC, work, iu, ctx->stream), "demodulation"));
+  const int mode = field_mode(g);
+  if (g->c_aber)
+    MTD_TRY(hip_check(ctx, launch_lsq(C, iu, n, ctx->lsq, coef, ctx->stream), "aberration fit"));
+  if (out_fields)
+    MTD_TRY(hip_check(ctx, launch_form_field(C, iu, crop, posx, posy, n, coef, mode, out_fields,
+                                             ctx->stream), "field formation"));
+  if (out_F) {
+    const double* winC = tukey_table(ctx, C, g->alpha_field);
+    if (!winC) return set_err(ctx, MTD_ERR_HIP, "out of device memory (window)");
+    const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+    MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, iu, 1, crop, coef, mode, posx, posy, n, winC,
+                                               twC, dx2, 0, work, out_F, ctx->stream),
+                      "forward spectrum"));
+  }
+  return MTD_OK;
+}
+
+static const size_t kChunkBudget = 128ull << 20;  // per-chunk hologram-stage scratch
+
+}  // namespace mtd
+
+using namespace mtd;
+
+extern "C" {
+
+const char* mtd_version(void) { return "mtd-mi355x 0.1 (gfx950)"; }
+
+int mtd_ctx_create(int device, mtd_ctx** out) {
+  if (!out) return MTD_ERR_ARG;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return MTD_ERR_HIP;
+  mtd_ctx* c = new mtd_ctx();
+  c->device = device;
+  c->stream = nullptr;
+  *out = c;
+  return MTD_OK;
+}
+
+int mtd_ctx_destroy(mtd_ctx* ctx) {
+  if (!ctx) return MTD_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->twiddles) hipFree(kv.second);
+  for (auto& kv : ctx->tukey) hipFree(kv.second);
+  for (auto& kv : ctx->htab) hipFree(kv.second);
+  for (auto& kv : ctx->betatab) hipFree(kv.second);
+  for (auto& kv : ctx->ws) hipFree(kv.second.ptr);
+  hipFree(ctx->lsq.d_samples);
+  hipFree(ctx->lsq.d_lu);
+  hipFree(ctx->lsq.d_piv);
+  delete ctx;
+  return MTD_OK;
+}
+
+int mtd_ctx_set_stream(mtd_ctx* ctx, void* hip_stream) {
+  if (!ctx) return MTD_ERR_ARG;
+  ctx->stream = (hipStream_t)hip_stream;
+  return MTD_OK;
+}
+
+int mtd_ctx_sync(mtd_ctx* ctx) {
+  if (!ctx) return MTD_ERR_ARG;
+  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "stream synchronize");
+}
+
+const char* mtd_last_error(const mtd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mtd_tukey1d(int n, double alpha, double* out) {
+  if (n < 2 || !out) return MTD_ERR_ARG;
+  tukey1d_host(n, alpha, out);
+  return MTD_OK;
+}
+
+int mtd_geom_derive(double ccd_roix, double nxmax, double dim_final, double circle_cx,
+                    double circle_cy, double na, double n0, double lambda, double tpcam,
+                    double f_tube, double f_obj, double rf, double coef_na, int born,
+                    int c_aber, int deroul, mtd_geom* out) {
+  if (!out) return MTD_ERR_ARG;
+  // every value passes through a C float (extract_val), then the double
+  // derivations of manip.cpp:37-57,104 / Reconstruction/src/manip.cpp:52-94
+  const double n0d = (float)n0, nad = (float)na, lamd = (float)lambda;
+  const double G = (double)(float)f_tube / (double)(float)f_obj;
+  const double Gt = G / (double)(float)rf;
+  const double pix = (double)(float)tpcam / Gt;
+  const size_t NX = (size_t)(float)nxmax;
+  const int N = (int)(size_t)(float)ccd_roix;
+  mtd_geom g{};
+  g.n = N;
+  g.nxmax = (int)NX;
+  g.dim_final = (int)(size_t)(float)dim_final;
+  g.circle_cx = (int)(size_t)(float)circle_cx;
+  g.circle_cy = (int)(size_t)(float)circle_cy;
+  g.born = born;
+  g.c_aber = c_aber;
+  g.deroul = deroul;
+  g.n0 = n0d;
+  g.na = nad;
+  g.lambda = lamd;
+  g.pix_holo = pix;
+  g.rayon = std::round((double)NX * n0d / nad);
+  g.pix_uborn = NX ? pix * N / (double)(2 * NX) : 0.0;
+  g.delta_f = NX ? 1.0 / (g.pix_uborn * 2 * (double)NX) : 0.0;
+  g.coef_na = (float)coef_na;
+  g.alpha_holo = 0.05;
+  g.alpha_field = 0.1;
+  *out = g;
+  return NX ? MTD_OK : MTD_ERR_ARG;
+}
+
+int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype,
+                     int A, float* crops) {
+  MTD_TRY(validate(ctx, g, A, NEED_HOLO));
+  if (A == 0) return MTD_OK;
+  if (!holos || !crops) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int N = g->n, C = 2 * g->nxmax;
+  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, A);
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, (float2*)crops + (size_t)a0 * C * C));
+  }
+  return MTD_OK;
+}
+
+int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A,
+                      const uint8_t* mask, int32_t* posspec, float* fields) {
+  MTD_TRY(validate(ctx, g, A, 0));
+  if (A == 0) return MTD_OK;
+  if (!crops || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int C = 2 * g->nxmax;
+  if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2) * 2, kChunkBudget, A);
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(front_chunk(ctx, g, (const float2*)crops + (size_t)a0 * C * C, n, posspec + a0,
+                        posspec + A + a0, (float2*)fields + (size_t)a0 * C * C, nullptr));
+  }
+  return MTD_OK;
+}
+
+int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
+                   const uint8_t* mask, int32_t* posspec, float* fields) {
+  MTD_TRY(validate(ctx, g, A, NEED_HOLO));
+  if (A == 0) return MTD_OK;
+  if (!holos || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int N = g->n, C = 2 * g->nxmax;
+  if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
+  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, A);
+  float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
+  if (!crop) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(front_chunk(ctx, g, crop, n, posspec + a0, posspec + A + a0,
+                        (float2*)fields + (size_t)a0 * C * C, nullptr));
+  }
+  return MTD_OK;
+}
+
+int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
+                          const int32_t* posspec, int A, float* spectra) {
+  MTD_TRY(validate(ctx, g, A, 0));
+  if (A == 0) return MTD_OK;
+  if (!fields || !posspec || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int C = 2 * g->nxmax;
+  const float2* twC = twiddle(ctx, C);
+  const double* winC = tukey_table(ctx, C, g->alpha_field);
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kChunkBudget, A);
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
+  if (!twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
+                                               nullptr, nullptr, field_mode(g), posspec + a0,
+                                               posspec + A + a0, n, winC, twC, dx2, 1, work,
+                                               (float2*)spectra + (size_t)a0 * C * C,
+                                               ctx->stream),
+                      "forward spectrum"));
+  }
+  return MTD_OK;
+}
+
+static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
+                        const int32_t* posspec, int A, float* vol, int32_t* counts) {
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
+  if (!ag) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
+  if (A > 0)
+    MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
+                                            t.r2tab, ag, ctx->stream), "angle geometry"));
+  MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
+                                       2 * g->nxmax, g->nxmax, g->dim_final, t.htab, t.beta,
+                                       t.r2max, (float2*)vol, counts, ctx->stream),
+                    "Ewald gather"));
+  return MTD_OK;
+}
+
+int mtd_ewald_scatter(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                      const int32_t* posspec, int A, float* vol, int32_t* counts) {
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (!vol || !counts || (A > 0 && (!spectra || !posspec)))
+    return set_err(ctx, MTD_ERR_ARG, "null array");
+  return scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, vol, counts);
+}
+
+int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
+                         const int32_t* posspec, int A, float* vol, int32_t* counts) {
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (!vol || !counts || (A > 0 && (!fields || !posspec)))
+    return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int C = 2 * g->nxmax;
+  float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
+  const float2* twC = twiddle(ctx, C);
+  const double* winC = tukey_table(ctx, C, g->alpha_field);
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kChunkBudget, std::max(A, 1));
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
+  if (!F || !twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
+                                               nullptr, nullptr, field_mode(g), posspec + a0,
+                                               posspec + A + a0, n, winC, twC, dx2, 0, work,
+                                               F + (size_t)a0 * C * C, ctx->stream),
+                      "forward spectrum"));
+  }
+  return scatter_impl(ctx, g, F, 0, posspec, A, vol, counts);
+}
+
+int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_t* counts,
+                 float* index_re, float* index_im, uint8_t* otf) {
+  MTD_TRY(validate(ctx, g, 0, NEED_VOL));
+  if (!vol || !counts || !index_re || !index_im) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int V = g->dim_final;
+  float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
+  const float2* twV = twiddle(ctx, V);
+  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory (volume work)");
+  const double kv = 2 * M_PI / g->lambda;
+  const double cte = -1.0 / (2 * g->n0 * kv * kv);
+  const float df3 = (float)std::pow(g->delta_f, 3);
+  const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
+  return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, work, df3, (float)cte,
+                                        gsign, index_re, index_im, otf, twV, ctx->stream),
+                   "finalize");
+}
+
+int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
+                          const uint8_t* mask, int32_t* posspec, float* index_re,
+                          float* index_im) {
+  MTD_TRY(validate(ctx, g, A, NEED_HOLO | NEED_VOL));
+  if (!index_re || !index_im || (A > 0 && !holos)) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int N = g->n, C = 2 * g->nxmax, V = g->dim_final;
+  if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
+  int32_t* pos = posspec;
+  if (!pos) {
+    pos = (int32_t*)ws_get(ctx, WS_POS, (size_t)std::max(2 * A, 1) * sizeof(int32_t));
+    if (!pos) return set_err(ctx, MTD_ERR_HIP, "out of device memory (posspec)");
+  }
+  float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
+  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, std::max(A, 1));
+  float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
+  float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
+  int32_t* cnt = (int32_t*)ws_get(ctx, WS_CNT, (size_t)V * V * V * sizeof(int32_t));
+  if (!F || !crop || !vol || !cnt) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
+                        F + (size_t)a0 * C * C));
+  }
+  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, cnt));
+  return mtd_finalize(ctx, g, (const float*)vol, cnt, index_re, index_im, nullptr);
+}
+
+}  // extern "C"

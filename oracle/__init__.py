@@ -1,0 +1,5 @@
+"""Oracle package: CPU restatement of the reference path.
+
+TEST INFRASTRUCTURE. Importable only from tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. The product never imports it.
+"""

@@ -1,0 +1,580 @@
+"""Oracle: a CPU restatement of the reference's off-axis preprocessing
+(``Tomo_prog/pretraitement_hors_axe_CPU``) and Fourier-space reconstruction
+(``Tomo_prog/Reconstruction``) for the plain method (BORN 1 / C_ABER 1 /
+DEROUL 0, plus the Rytov and C_ABER 0 branches).
+
+TEST INFRASTRUCTURE. Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker (or the timed CPU baseline). The product path never calls it.
+
+Two precisions are restated:
+
+* ``prec="f64"`` -- the FP64 oracle: every step in float64, as the reference
+  computes.
+* ``prec="f32"`` -- the reference FP32 path: the same steps with FFTs, field
+  and volume arithmetic in float32; the least-squares polynomial fit and the
+  background evaluation stay in float64 (the only FP64 stage of the GPU path).
+
+Every function cites the reference lines it follows. Pins (checks that do not
+retype these formulas) live in ``tests/test_oracle_*.py``:
+
+* config parsing        -- hand-computed known answers (float rounding, last key
+                           wins, comments, defaults);
+* tukey windows         -- borne1/borne2 known answers, plateau, end values;
+* centred FFT + crop    -- brute-force DFT sum on tiny inputs, pure-carrier
+                           closed form, Parseval;
+* argmax                -- brute-force scan on tiny inputs with ties and the
+                           slot-0 squared quirk;
+* demodulation          -- plane wave at a known sideband offset gives a
+                           constant field of known amplitude;
+* least squares         -- exact recovery of a degree-4 background, residual
+                           orthogonality, agreement with numpy.linalg.lstsq;
+* Born/Rytov formation  -- empty sample gives U_B == 0, known phase object;
+* Ewald scatter         -- closed-form total count, spherical-cap z for a
+                           centred specular, single-pixel spectrum gives one
+                           voxel with the closed-form value;
+* finalize              -- single normalised delta gives the closed-form plane
+                           wave; Parseval.
+
+Known, documented deviations from the reference (both unpinnable without
+running OpenCV):
+
+* ``compuBackgr2`` reads ``coef[15]`` past the end of the 15 coefficients
+  (Correction_aberration.cpp:157). The restatement evaluates the intended
+  15-term polynomial.
+* ``cv::solve(..., DECOMP_NORMAL)`` rounding is restated as LAPACK LU with
+  partial pivoting on the same normal equations (numpy.linalg.solve).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ``#define PI 3.14159`` at Reconstruction/src/fonctions.cpp:2 (used by
+# retroPropag_Born); ``3.1415`` is written inline in tukey2D
+# (pretraitement_hors_axe_CPU/FFT_fonctions.cpp:75,81); Reconstruction/main.cpp:28
+# uses M_PI for the potential-to-index constant.
+PI_RETRO = 3.14159
+PI_TUKEY = 3.1415
+
+
+def _real(prec):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def _cplx(prec):
+    return np.complex64 if prec == "f32" else np.complex128
+
+
+def _mk(re, im, ct):
+    """Assemble a complex array from separately computed parts (no extra
+    rounding: the parts are stored as computed)."""
+    out = np.empty(np.shape(re), dtype=ct)
+    out.real = re
+    out.imag = im
+    return out
+
+
+# --------------------------------------------------------------------------
+# A1: config files
+# --------------------------------------------------------------------------
+def _config_lines(path):
+    """Lines of a config file as the reference tokenises them
+    (pretraitement_hors_axe_CPU/src/IO_fonctions.cpp:85-95): split on '\\n',
+    drop lines whose first character is '#'."""
+    with open(path, "rb") as fh:
+        text = fh.read().decode("latin-1")
+    return [ln for ln in text.split("\n") if not ln.startswith("#")]
+
+
+def _lookup(path, token):
+    """Last value string for ``token`` or None (IO_fonctions.cpp:97-112).
+
+    The key is the text before the first space; the value is the text after
+    it. A line without a space is its own key with the whole line as value.
+    """
+    value = None
+    for ln in _config_lines(path):
+        if ln == "":
+            continue
+        pos = ln.find(" ")
+        key = ln if pos < 0 else ln[:pos]
+        if key == token:
+            value = ln if pos < 0 else ln[pos + 1:]
+    return value
+
+
+def _atof(s):
+    """C ``atof``: longest leading decimal-float prefix, 0 when none."""
+    import re
+    m = re.match(r"\s*([+-]?(\d+\.?\d*([eE][+-]?\d+)?|\.\d+([eE][+-]?\d+)?))", s)
+    return float(m.group(1)) if m else 0.0
+
+
+def extract_val(path, token, default=None):
+    """``extract_val``: value parsed with atof and stored in a C ``float``
+    (IO_fonctions.cpp:81-118; the 3-argument default form is
+    Reconstruction/src/fonctions.cpp:137-180). Returns the float value
+    promoted to a Python float (= C double)."""
+    v = _lookup(path, token)
+    if v is None or v == "":
+        return float(np.float32(default)) if default is not None else 0.0
+    return float(np.float32(_atof(v)))
+
+
+def extract_string(path, token):
+    """``extract_string`` (IO_fonctions.cpp:44-79): last value, '' if absent."""
+    v = _lookup(path, token)
+    return "" if v is None else v
+
+
+# --------------------------------------------------------------------------
+# small helpers
+# --------------------------------------------------------------------------
+def round_half_away(x):
+    """C ``round``: halves away from zero, exact for every double."""
+    x = np.asarray(x, dtype=np.float64)
+    t = np.trunc(x)
+    frac = x - t
+    return t + np.where(np.abs(frac) >= 0.5, np.sign(x), 0.0)
+
+
+def fftshift2(a):
+    """fftshift2D for even sides (FFT_fonctions.cpp:89-114,237-264):
+    quadrant swap == np.roll by half a side."""
+    n0, n1 = a.shape[-2:]
+    return np.roll(a, (n0 // 2, n1 // 2), axis=(-2, -1))
+
+
+def fftshift3(a):
+    """fftshift3D for even sides (Reconstruction/src/FFT_fonctions.cpp:78-151)."""
+    return np.roll(a, tuple(s // 2 for s in a.shape), axis=(0, 1, 2))
+
+
+def decal2d(a, dx, dy):
+    """decal2DCplxGen / decal2DCplxGen2 (fonctions.cpp:64-85,
+    Reconstruction/src/fonctions.cpp:1268-1309): out[(y+dy)%C][(x+dx)%C] =
+    in[y][x]."""
+    return np.roll(a, (dy, dx), axis=(-2, -1))
+
+
+# --------------------------------------------------------------------------
+# A3: Tukey window
+# --------------------------------------------------------------------------
+def tukey1d(n, alpha):
+    """1-D factor of tukey2D (FFT_fonctions.cpp:61-87; identical copy in
+    Reconstruction/src/FFT_fonctions.cpp:224-248). ``alpha`` is a C float and
+    ``2*cpt/(alpha*(N-1))`` is a float division; pi is written 3.1415."""
+    f = np.float32
+    a = f(alpha)
+    nm1 = f(n - 1)
+    borne1 = int(round_half_away(float(f(a * nm1) / f(2))))
+    borne2 = int(round_half_away(float(f(nm1 * f(f(1) - f(a / f(2)))))))
+    den = f(a * nm1)
+    two_over_a = f(f(2) / a)
+    t = np.empty(n, dtype=np.float64)
+    for cpt in range(n):
+        if cpt < borne1 + 1:
+            arg = f(f(f(2 * cpt) / den) - f(1))
+            t[cpt] = 0.5 * (1 + math.cos(PI_TUKEY * float(arg)))
+        elif cpt < borne2 + 1:
+            t[cpt] = 1.0
+        else:
+            arg = f(f(f(f(2 * cpt) / den) - two_over_a) + f(1))
+            t[cpt] = 0.5 * (1 + math.cos(PI_TUKEY * float(arg)))
+    return t, borne1, borne2
+
+
+def tukey2d(n, alpha):
+    """tukey2D: tuk2D[y*N+x] = tuk1D[x]*tuk1D[y] in double."""
+    t, _, _ = tukey1d(n, alpha)
+    return t[None, :] * t[:, None]
+
+
+# --------------------------------------------------------------------------
+# geometry (A1 derivations)
+# --------------------------------------------------------------------------
+@dataclass
+class Geom:
+    """Derived acquisition geometry, as the two ``manip`` constructors compute
+    it (pretraitement_hors_axe_CPU/manip.cpp:9-157,
+    Reconstruction/src/manip.cpp:10-163)."""
+    n: int                 # CCD_ROIX (hologram side N)
+    nxmax: int             # NXMAX (crop side C = 2*NXMAX)
+    dim_final: int         # DIM_FINAL (volume side V)
+    circle_cx: int
+    circle_cy: int
+    n0: float
+    na: float
+    lam: float
+    pix_holo: float        # tailleTheoPixelHolo = TPCAM/Gt
+    rayon: float           # round(NXMAX*n0/NA)
+    pix_uborn: float       # tailleTheoPixelHolo*N/(2*NXMAX)
+    delta_f: float         # 1/(pix_uborn*2*NXMAX)
+    coef_na: float = 1.0   # COEF_NA_OBJ_LIMIT (default 1)
+    born: int = 1
+    c_aber: int = 1
+    deroul: int = 0
+    alpha_holo: float = 0.05   # main.cpp:105
+    alpha_field: float = 0.1   # main.cpp:180, Reconstruction/main.cpp:185
+
+    @property
+    def C(self):
+        return 2 * self.nxmax
+
+    @property
+    def coin(self):
+        """coinHA = (CIRCLE_CX-NXMAX, CIRCLE_CY-NXMAX) (main.cpp:116)."""
+        return self.circle_cx - self.nxmax, self.circle_cy - self.nxmax
+
+
+def derive_geom(n, nxmax, dim_final, circle_cx, circle_cy, na, n0, lam,
+                tpcam, f_tube, f_obj, rf, coef_na=1.0, born=1, c_aber=1,
+                deroul=0):
+    """Apply the reference's config arithmetic: every config value goes through
+    a C float (IO_fonctions.cpp:109) before the double derivations of
+    manip.cpp:37-57,104 and Reconstruction/src/manip.cpp:52-94."""
+    f = lambda v: float(np.float32(v))
+    n0, na, lam = f(n0), f(na), f(lam)
+    g_ = f(f_tube) / f(f_obj)
+    gt = g_ / f(rf)
+    pix_holo = f(tpcam) / gt
+    nxmax = int(np.float32(nxmax))
+    n = int(np.float32(n))
+    dim_final = int(np.float32(dim_final))
+    rayon = float(round_half_away(float(nxmax) * n0 / na))
+    pix_uborn = pix_holo * n / (2 * nxmax)
+    delta_f = 1.0 / (pix_uborn * 2 * nxmax)
+    return Geom(n=n, nxmax=nxmax, dim_final=dim_final,
+                circle_cx=int(np.float32(circle_cx)),
+                circle_cy=int(np.float32(circle_cy)), n0=n0, na=na, lam=lam,
+                pix_holo=pix_holo, rayon=rayon, pix_uborn=pix_uborn,
+                delta_f=delta_f, coef_na=f(coef_na), born=born, c_aber=c_aber,
+                deroul=deroul)
+
+
+# --------------------------------------------------------------------------
+# A4: hologram -> centred spectrum -> crop
+# --------------------------------------------------------------------------
+def holo_to_crop(holo, g: Geom, prec="f64", window=None):
+    """holo2TF_UBorn (fonctions.cpp:279-304): H*tukey(0.05); fftshift2D;
+    FFTW c2c forward; /N^2 (FFT_fonctions.cpp:316-330); fftshift2D;
+    coupeCplx at coinHA (fonctions.cpp:104-124)."""
+    rt, ct = _real(prec), _cplx(prec)
+    N, C = g.n, g.C
+    if window is None:
+        window = tukey2d(N, g.alpha_holo)
+    h = np.asarray(holo).astype(rt) * window.astype(rt)
+    spec = np.fft.fft2(fftshift2(h).astype(ct))
+    spec = (spec / rt(N * N)).astype(ct)
+    spec = fftshift2(spec)
+    x0, y0 = g.coin
+    return spec[y0:y0 + C, x0:x0 + C].copy()
+
+
+# --------------------------------------------------------------------------
+# A5: specular position
+# --------------------------------------------------------------------------
+def coord_spec(crop, prec="f64"):
+    """coordSpec (fonctions.cpp:88-101): slot 0 holds |TF[0]|^2, every other
+    slot |TF|; a strictly-greater scan keeps the first maximum, i.e. the first
+    index of the overall maximum."""
+    rt = _real(prec)
+    flat = crop.ravel()
+    mod = np.abs(flat).astype(rt)
+    mod[0] = rt(flat[0].real) * rt(flat[0].real) + rt(flat[0].imag) * rt(flat[0].imag)
+    return int(np.argmax(mod))
+
+
+# --------------------------------------------------------------------------
+# A6: demodulation
+# --------------------------------------------------------------------------
+def demod(crop, px, py, prec="f64"):
+    """calc_Uborn2 (fonctions.cpp:348-374): circular shift by -posSpec
+    (decal2DCplxGen2), unnormalised backward FFT (TF2Dcplx_INV,
+    FFT_fonctions.cpp:420-438), circular shift by (NXMAX, NXMAX)."""
+    ct = _cplx(prec)
+    C = crop.shape[0]
+    t = decal2d(crop.astype(ct), -px, -py)
+    iu = np.fft.ifft2(t, norm="forward").astype(ct)
+    return decal2d(iu, C // 2, C // 2)
+
+
+# --------------------------------------------------------------------------
+# A9-A11: aberration (polynomial background) least squares, always FP64
+# --------------------------------------------------------------------------
+POLY_DEG = 4
+
+
+def poly_terms(deg=POLY_DEG):
+    """Monomial order of CalcPolyUs_xy / CalcPoly_xy
+    (Correction_aberration.cpp:108-149): powX outer, powY inner."""
+    return [(i, j) for i in range(deg + 1) for j in range(deg + 1 - i)]
+
+
+def sample_points(mask):
+    """countM + the ==220 selection (Correction_aberration.cpp:86-106,
+    116-127, 196-203): pixels with row%5==0 and col%5==0 and mask>45, in
+    row-major (y outer, x inner) order. Returns (ys, xs) int arrays."""
+    C = mask.shape[0]
+    ys, xs = np.mgrid[0:C, 0:C]
+    sel = (ys % 5 == 0) & (xs % 5 == 0) & (np.asarray(mask) > 45)
+    return ys[sel].astype(np.int64), xs[sel].astype(np.int64)
+
+
+def poly_matrix(xs, ys, deg=POLY_DEG):
+    """Rows x^i*y^j in raw pixel units (exact in double)."""
+    xs = np.asarray(xs, dtype=np.float64)
+    ys = np.asarray(ys, dtype=np.float64)
+    return np.stack([xs ** i * ys ** j for i, j in poly_terms(deg)], axis=1)
+
+
+@dataclass
+class LsqPlan:
+    """Angle-independent part of the fit: sample set, normal matrix A^T A and
+    the full-image basis."""
+    ys: np.ndarray
+    xs: np.ndarray
+    A: np.ndarray
+    G: np.ndarray
+    full: np.ndarray
+
+    @staticmethod
+    def build(mask):
+        C = mask.shape[0]
+        ys, xs = sample_points(mask)
+        if len(ys) < 15:
+            raise ValueError("aberration mask leaves %d sample points; the degree-4 "
+                             "fit needs at least 15" % len(ys))
+        A = poly_matrix(xs, ys)
+        gy, gx = np.mgrid[0:C, 0:C]
+        full = poly_matrix(gx.ravel(), gy.ravel())
+        return LsqPlan(ys, xs, A, A.T @ A, full)
+
+
+def lsq_background(values, plan: LsqPlan):
+    """compuCoefPoly2 (Correction_aberration.cpp:186-217) with
+    cv::solve(DECOMP_NORMAL): LU on A^T A c = A^T b; then compuBackgr2
+    (:151-164) evaluating the 15-term polynomial at every pixel (the
+    reference's 16th, out-of-bounds term is not restated)."""
+    C = values.shape[0]
+    b = np.asarray(values, dtype=np.float64)[plan.ys, plan.xs]
+    coef = np.linalg.solve(plan.G, plan.A.T @ b)
+    return (plan.full @ coef).reshape(C, C), coef
+
+
+# --------------------------------------------------------------------------
+# A7, A10-A13: field formation
+# --------------------------------------------------------------------------
+def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
+    """main.cpp:254-341 for one angle, from the demodulated field U.
+
+    C_ABER 1: phase = atan2 (fonctions.cpp:39-43); DEROUL 0 keeps it wrapped
+    (main.cpp:277); phase and amplitude backgrounds by aberCorr2 / ampliCorr2
+    (Correction_aberration.cpp:166-184); amplitude clamp (main.cpp:301-302);
+    Born (main.cpp:307,309) or Rytov (:312-313).
+    C_ABER 0: the spectrum is multiplied by conj(s)/|s| before the demodulation
+    (main.cpp:332-337); by linearity this equals scaling U."""
+    rt, ct = _real(prec), _cplx(prec)
+    if not g.c_aber:
+        s = complex(spec_value)
+        mod = math.sqrt(s.real * s.real + s.imag * s.imag)
+        f = ct(complex(s.real / mod, -s.imag / mod))
+        return (U.astype(ct) * f).astype(ct)
+    phi = np.arctan2(U.imag, U.real).astype(rt)
+    amp = np.abs(U).astype(rt)
+    bg_phi, _ = lsq_background(phi, plan)
+    bg_amp, _ = lsq_background(amp, plan)
+    phic = phi.astype(np.float64) - bg_phi
+    a = amp.astype(np.float64) / (bg_amp + 0.00000001)
+    a = np.where(a > 10, 1.0, a)
+    a = np.where(a < -10, 1.0, a)
+    if prec == "f32":
+        phic = phic.astype(np.float32)
+        aa = np.abs(a).astype(np.float32)
+        if g.born:
+            re = aa * np.cos(phic) - np.float32(1)
+            im = aa * np.sin(phic)
+        else:
+            re = np.log(aa)
+            im = phic
+        return _mk(re, im, np.complex64)
+    aa = np.sqrt(a * a)
+    if g.born:
+        return _mk(aa * np.cos(phic) - 1, aa * np.sin(phic), np.complex128)
+    return _mk(np.log(aa), phic, np.complex128)
+
+
+def load_mask(C, mask=None):
+    """init_mask_aber (Correction_aberration.cpp:43-70): absent mask => 255."""
+    if mask is None:
+        return np.full((C, C), 255, dtype=np.uint8)
+    return np.asarray(mask, dtype=np.uint8)
+
+
+def preprocess(holos, g: Geom, mask=None, prec="f64", return_crops=False):
+    """The two hot loops of pretraitement_hors_axe_CPU/main.cpp:130-342 over
+    holograms that all exist. Returns (fields[A,C,C], posspec[2A] as ints:
+    all x then all y (main.cpp:250-251))."""
+    if g.deroul:
+        raise NotImplementedError("DEROUL 1 (phase unwrapping) is not restated")
+    C = g.C
+    A = len(holos)
+    window = tukey2d(g.n, g.alpha_holo)
+    plan = LsqPlan.build(load_mask(C, mask))
+    fields = np.empty((A, C, C), dtype=_cplx(prec))
+    crops = np.empty((A, C, C), dtype=_cplx(prec)) if return_crops else None
+    pos = np.empty(2 * A, dtype=np.int64)
+    for a in range(A):
+        crop = holo_to_crop(holos[a], g, prec, window)
+        if return_crops:
+            crops[a] = crop
+        i = coord_spec(crop, prec)
+        px, py = i % C, i // C
+        pos[a], pos[a + A] = px, py
+        U = demod(crop, px, py, prec)
+        fields[a] = form_field(U, plan, g, prec, spec_value=crop[py, px])
+    if return_crops:
+        return fields, pos, crops
+    return fields, pos
+
+
+# --------------------------------------------------------------------------
+# A16: per-angle spectrum in the reconstruction
+# --------------------------------------------------------------------------
+def field_to_spectrum(field_, px, py, g: Geom, prec="f64", window=None):
+    """Reconstruction/main.cpp:207-217: U_B * tukey(C, 0.1); decal2DCplxGen by
+    (NXMAX, NXMAX); FFTW forward * pixUBorn^2 (src/FFT_fonctions.cpp:333-349);
+    decal2DCplxGen by +posSpec. Returns S[C][C] indexed like TF_UBorn_normC."""
+    rt, ct = _real(prec), _cplx(prec)
+    C = g.C
+    if window is None:
+        window = tukey2d(C, g.alpha_field)
+    w = window.astype(rt)
+    ub = _mk(field_.real.astype(rt) * w, field_.imag.astype(rt) * w, ct)
+    ub = decal2d(ub, g.nxmax, g.nxmax)
+    F = np.fft.fft2(ub).astype(ct)
+    F = (F * rt(g.pix_uborn ** 2)).astype(ct)
+    return decal2d(F, px, py)
+
+
+# --------------------------------------------------------------------------
+# A17: Ewald-cap scatter with hit counts
+# --------------------------------------------------------------------------
+def ewald_constants(g: Geom):
+    """Constants of retroPropag_Born (Reconstruction/src/fonctions.cpp:875-905):
+    float n0, kv = 2*PI/lambda0 (double product stored in a float),
+    k0 = kv*n0 in float; cteUb2Pot = i*k0/PI; cteNorm = -2*PI; the per-pixel
+    factor beta = ((k0/PI)*cteNorm)*sdz."""
+    n0f = np.float32(g.n0)
+    kv = np.float32(2 * PI_RETRO / g.lam)
+    k0 = np.float32(kv * n0f)
+    gamma = (float(k0) / PI_RETRO) * (-2 * PI_RETRO)
+    return gamma
+
+
+def pupil_tables(g: Geom):
+    """fd grid, pupil mask (|fd|^2 < coef*NXMAX^2, fonctions.cpp:915), the
+    height h = sqrt(R^2 - fdx^2 - fdy^2) and beta per pixel."""
+    nx = g.nxmax
+    fdy, fdx = np.mgrid[-nx:nx, -nx:nx]
+    r2i = fdx * fdx + fdy * fdy
+    pupil = r2i.astype(np.float64) < g.coef_na * float(nx * nx)
+    R = g.rayon
+    h = np.sqrt(np.maximum((R * R - fdx * fdx) - fdy * fdy, 0.0))
+    sdz = h * (1.0 / R)
+    beta = ewald_constants(g) * sdz
+    return fdx, fdy, pupil, h, beta
+
+
+def ewald_scatter(S, pos, g: Geom, prec="f64"):
+    """retroPropag_Born (Reconstruction/src/fonctions.cpp:875-935) over all
+    angles in order (main.cpp:203-225). S: [A][C][C] (TF_UBorn_normC frame);
+    pos: ints [2A]. Returns (vol[V^3] complex, counts[V^3] int32).
+
+    koz = round(sqrt(R^2-|fd|^2) - fzm0); vox = (fdx-fxm0+V/2) +
+    (fdy-fym0+V/2)*V + (koz+V/2)*V^2; the angle is skipped when
+    round(R^2-|fm0|^2) < 0. Values (i*beta)*S = (-beta*Si, beta*Sr) summed in
+    angle order."""
+    ct = _cplx(prec)
+    rt = _real(prec)
+    A = S.shape[0]
+    nx, V, R = g.nxmax, g.dim_final, g.rayon
+    half = V // 2
+    fdx, fdy, pupil, h, beta = pupil_tables(g)
+    vol = np.zeros(V ** 3, dtype=ct)
+    cnt = np.zeros(V ** 3, dtype=np.int32)
+    fdx_p, fdy_p, h_p = fdx[pupil], fdy[pupil], h[pupil]
+    beta_p = beta[pupil].astype(rt)
+    for a in range(A):
+        px, py = int(pos[a]), int(pos[a + A])
+        fxm0, fym0 = px - nx, py - nx
+        fz2 = R * R - fxm0 * fxm0 - fym0 * fym0
+        if round_half_away(fz2) < 0:
+            continue
+        fzm0 = math.sqrt(fz2)
+        koz = round_half_away(h_p - fzm0).astype(np.int64)
+        vox = (fdx_p - fxm0 + half) + (fdy_p - fym0 + half) * V + (koz + half) * V * V
+        s = S[a][pupil]
+        val = _mk(-(beta_p * s.imag.astype(rt)), beta_p * s.real.astype(rt), ct)
+        vol[vox] += val
+        cnt[vox] += 1
+    return vol, cnt
+
+
+def ewald_bounds_ok(pos, g: Geom):
+    """True when every voxel index of every non-skipped angle lies inside the
+    V^3 volume (the reference has no check, fonctions.cpp:926-928)."""
+    A = len(pos) // 2
+    nx, V, R = g.nxmax, g.dim_final, g.rayon
+    half = V // 2
+    fdx, fdy, pupil, h, _ = pupil_tables(g)
+    for a in range(A):
+        fxm0, fym0 = int(pos[a]) - nx, int(pos[a + A]) - nx
+        fz2 = R * R - fxm0 * fxm0 - fym0 * fym0
+        if round_half_away(fz2) < 0:
+            continue
+        koz = round_half_away(h[pupil] - math.sqrt(fz2))
+        for arr in (fdx[pupil] - fxm0 + half, fdy[pupil] - fym0 + half, koz + half):
+            if arr.min() < 0 or arr.max() >= V:
+                return False
+    return True
+
+
+# --------------------------------------------------------------------------
+# A18-A22: normalisation, 3-D inverse FFT, potential -> index
+# --------------------------------------------------------------------------
+def finalize(vol, cnt, g: Geom, prec="f64"):
+    """Reconstruction/main.cpp:243-352: counts 0 -> 1 and divide; OTF mask
+    Re != 0 (:262-267); fftshift3D; FFTW backward 3-D * delta_f^3
+    (src/FFT_fonctions.cpp:394-412); fftshift3D; * -1/(2 n0 kv^2) with
+    kv = 2*M_PI/lambda0 (:346-352). Returns (index complex V^3, otf bool V^3)."""
+    rt, ct = _real(prec), _cplx(prec)
+    V = g.dim_final
+    c = np.where(cnt == 0, 1, cnt).astype(rt)
+    t = _mk(vol.real.astype(rt) / c, vol.imag.astype(rt) / c, ct)
+    otf = t.real != 0
+    t = fftshift3(t.reshape(V, V, V))
+    p = np.fft.ifftn(t, norm="forward").astype(ct)
+    p = (p * rt(g.delta_f ** 3)).astype(ct)
+    p = fftshift3(p)
+    kv = 2 * math.pi / g.lam
+    cte = -1.0 / (2 * g.n0 * kv * kv)
+    idx = (p * rt(cte)).astype(ct)
+    return idx, otf.reshape(V, V, V)
+
+
+def reconstruct(fields, pos, g: Geom, prec="f64"):
+    """tomo_reconstruction's compute (Reconstruction/main.cpp:203-352) from
+    the preprocessing outputs. Returns (index, vol, counts, otf)."""
+    A = fields.shape[0]
+    window = tukey2d(g.C, g.alpha_field)
+    S = np.empty_like(fields, dtype=_cplx(prec))
+    for a in range(A):
+        S[a] = field_to_spectrum(fields[a], int(pos[a]), int(pos[a + A]), g,
+                                 prec, window)
+    vol, cnt = ewald_scatter(S, pos, g, prec)
+    idx, otf = finalize(vol, cnt, g, prec)
+    return idx, vol, cnt, otf

@@ -1,0 +1,300 @@
+"""Parity of the gfx950 path (through the C ABI) against the oracle.
+
+Bars: posspec and hit counts bit-exact; the Ewald accumulation bit-exact
+against the FP32 oracle on identical spectra; complex crops, fields, spectra
+and the index volume within rel-L2 1e-5 of the FP32 oracle (the reference FP32
+path). Each test also prints its error against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from cases import make_case, rel_l2
+from oracle import ref
+
+TOL = 1e-5
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _empty(shape, dtype):
+    import torch
+    return torch.empty(shape, dtype=dtype, device="cuda")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def oracle_front(case, prec):
+    """Oracle crops, posspec, fields."""
+    fields, pos, crops = ref.preprocess(case.acq.holos, case.g, case.acq.mask, prec,
+                                        return_crops=True)
+    return crops, pos, fields
+
+
+def oracle_spectra(case, fields, pos, prec):
+    A = fields.shape[0]
+    return np.stack([ref.field_to_spectrum(fields[a], int(pos[a]), int(pos[a + A]),
+                                           case.g, prec) for a in range(A)])
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "u16", "cfg1"])
+def test_holo_to_crop(gpu_ctx, name):
+    import torch
+    case = make_case(name)
+    _say(f"holo_to_crop case={name} holos={case.acq.holos.shape} {case.acq.holos.dtype}")
+    g = case.lib_geom()
+    A, C = case.acq.holos.shape[0], g.C
+    crops = _empty((A, C, C), torch.complex64)
+    gpu_ctx.holo_to_crop(g, _t(case.acq.holos), crops)
+    torch.cuda.synchronize()
+    got = crops.cpu().numpy()
+    exp32 = np.stack([ref.holo_to_crop(h, case.g, "f32") for h in case.acq.holos])
+    exp64 = np.stack([ref.holo_to_crop(h, case.g, "f64") for h in case.acq.holos])
+    e32, e64 = rel_l2(got, exp32), rel_l2(got, exp64)
+    _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}  (f32 oracle vs f64 {rel_l2(exp32, exp64):.3e})")
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "u16", "cfg1"])
+def test_crop_to_field(gpu_ctx, name):
+    import torch
+    case = make_case(name)
+    _say(f"crop_to_field case={name}")
+    g = case.lib_geom()
+    crops32, pos32, fields32 = oracle_front(case, "f32")
+    _, pos64, fields64 = oracle_front(case, "f64")
+    A, C = crops32.shape[0], g.C
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.crop_to_field(g, _t(crops32), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    gp = pos.cpu().numpy()
+    assert np.array_equal(gp, pos32), (gp, pos32)
+    assert np.array_equal(gp, pos64)
+    got = fields.cpu().numpy()
+    e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
+    _say(f"  fields rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["small", "u16"])
+@pytest.mark.parametrize("born,c_aber", [(0, 1), (1, 0)])
+def test_crop_to_field_modes(gpu_ctx, name, born, c_aber):
+    """Rytov (BORN 0) and the C_ABER 0 branch (main.cpp:311-341)."""
+    import torch
+    case = make_case(name, born=born, c_aber=c_aber)
+    _say(f"crop_to_field modes case={name} born={born} c_aber={c_aber}")
+    g = case.lib_geom()
+    crops32, pos32, fields32 = oracle_front(case, "f32")
+    A, C = crops32.shape[0], g.C
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.crop_to_field(g, _t(crops32), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), pos32)
+    e32 = rel_l2(fields.cpu().numpy(), fields32)
+    _say(f"  rel-L2 vs f32 {e32:.3e}")
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
+def test_preprocess(gpu_ctx, name):
+    import torch
+    case = make_case(name)
+    _say(f"preprocess case={name}")
+    g = case.lib_geom()
+    _, pos32, fields32 = oracle_front(case, "f32")
+    _, _, fields64 = oracle_front(case, "f64")
+    A, C = case.acq.holos.shape[0], g.C
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.preprocess(g, _t(case.acq.holos), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), pos32)
+    got = fields.cpu().numpy()
+    e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
+    _say(f"  fields rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
+def test_field_to_spectrum(gpu_ctx, name):
+    import torch
+    case = make_case(name)
+    _say(f"field_to_spectrum case={name}")
+    g = case.lib_geom()
+    _, pos, fields32 = oracle_front(case, "f32")
+    S32 = oracle_spectra(case, fields32, pos, "f32")
+    S64 = oracle_spectra(case, fields32.astype(np.complex128), pos, "f64")
+    A, C = fields32.shape[0], g.C
+    out = _empty((A, C, C), torch.complex64)
+    gpu_ctx.field_to_spectrum(g, _t(fields32), _t(pos.astype(np.int32)), out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    e32, e64 = rel_l2(got, S32), rel_l2(got, S64)
+    _say(f"  spectra rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "u16", "cfg1"])
+def test_ewald_scatter_bit_exact(gpu_ctx, name):
+    """Identical FP32 spectra in -> bit-identical counts AND values out."""
+    import torch
+    case = make_case(name)
+    _say(f"ewald_scatter case={name}")
+    g = case.lib_geom()
+    _, pos, fields32 = oracle_front(case, "f32")
+    S32 = oracle_spectra(case, fields32, pos, "f32")
+    vol32, cnt32 = ref.ewald_scatter(S32, pos, case.g, "f32")
+    V = g.dim_final
+    vol = _empty((V, V, V), torch.complex64)
+    cnt = _empty((V, V, V), torch.int32)
+    gpu_ctx.ewald_scatter(g, _t(S32), _t(pos.astype(np.int32)), vol, cnt)
+    torch.cuda.synchronize()
+    gc = cnt.cpu().numpy().ravel()
+    gv = vol.cpu().numpy().ravel()
+    _say(f"  hits {int(cnt32.sum())} gpu {int(gc.sum())} max {int(gc.max())}")
+    assert np.array_equal(gc, cnt32)
+    assert np.array_equal(gv.view(np.uint32), vol32.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
+def test_finalize(gpu_ctx, name):
+    import torch
+    case = make_case(name)
+    _say(f"finalize case={name}")
+    g = case.lib_geom()
+    _, pos, fields32 = oracle_front(case, "f32")
+    S32 = oracle_spectra(case, fields32, pos, "f32")
+    vol32, cnt32 = ref.ewald_scatter(S32, pos, case.g, "f32")
+    idx32, otf32 = ref.finalize(vol32, cnt32, case.g, "f32")
+    idx64, _ = ref.finalize(vol32.astype(np.complex128), cnt32, case.g, "f64")
+    V = g.dim_final
+    re = _empty((V, V, V), torch.float32)
+    im = _empty((V, V, V), torch.float32)
+    otf = _empty((V, V, V), torch.uint8)
+    gpu_ctx.finalize(g, _t(vol32.reshape(V, V, V)), _t(cnt32.reshape(V, V, V)), re, im, otf)
+    torch.cuda.synchronize()
+    got = re.cpu().numpy() + 1j * im.cpu().numpy()
+    e32, e64 = rel_l2(got, idx32), rel_l2(got, idx64)
+    _say(f"  index rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert np.array_equal(otf.cpu().numpy().astype(bool), otf32)
+    assert e32 < TOL
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
+def test_reconstruct_plain(gpu_ctx, name):
+    """Holograms to index in one call vs the oracle's two programs."""
+    import torch
+    case = make_case(name)
+    _say(f"reconstruct_plain case={name}")
+    g = case.lib_geom()
+    fields32, pos32 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f32")
+    idx32, vol32, cnt32, _ = ref.reconstruct(fields32, pos32, case.g, "f32")
+    fields64, pos64 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f64")
+    idx64, _, cnt64, _ = ref.reconstruct(fields64, pos64, case.g, "f64")
+    assert np.array_equal(cnt32, cnt64)
+    V = g.dim_final
+    A = case.acq.holos.shape[0]
+    re = _empty((V, V, V), torch.float32)
+    im = _empty((V, V, V), torch.float32)
+    pos = _empty((2 * A,), torch.int32)
+    gpu_ctx.reconstruct_plain(g, _t(case.acq.holos), case.acq.mask, re, im, pos)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), pos32)
+    got = re.cpu().numpy() + 1j * im.cpu().numpy()
+    e32, e64 = rel_l2(got, idx32), rel_l2(got, idx64)
+    _say(f"  index rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e} (oracle f32 vs f64 {rel_l2(idx32, idx64):.3e})")
+    assert e32 < TOL
+
+
+def test_accumulate_counts_match_scatter(gpu_ctx):
+    """mtd_ewald_accumulate (fields in) == field_to_spectrum + ewald_scatter."""
+    import torch
+    case = make_case("small")
+    _say("accumulate vs scatter case=small")
+    g = case.lib_geom()
+    _, pos, fields32 = oracle_front(case, "f32")
+    A, C, V = fields32.shape[0], g.C, g.dim_final
+    f = _t(fields32)
+    p = _t(pos.astype(np.int32))
+    S = _empty((A, C, C), torch.complex64)
+    gpu_ctx.field_to_spectrum(g, f, p, S)
+    v1 = _empty((V, V, V), torch.complex64)
+    c1 = _empty((V, V, V), torch.int32)
+    gpu_ctx.ewald_scatter(g, S, p, v1, c1)
+    v2 = _empty((V, V, V), torch.complex64)
+    c2 = _empty((V, V, V), torch.int32)
+    gpu_ctx.ewald_accumulate(g, f, p, v2, c2)
+    torch.cuda.synchronize()
+    assert torch.equal(c1, c2)
+    assert torch.equal(v1, v2)
+
+
+def test_empty_stack_gives_zero_volume(gpu_ctx):
+    """A = 0: counts and spectrum all zero (every voxel written)."""
+    import torch
+    case = make_case("tiny")
+    _say("empty stack case=tiny")
+    g = case.lib_geom()
+    V = g.dim_final
+    vol = torch.full((V, V, V), 7.0, dtype=torch.complex64, device="cuda")
+    cnt = torch.full((V, V, V), 7, dtype=torch.int32, device="cuda")
+    S = _empty((0, g.C, g.C), torch.complex64)
+    gpu_ctx.ewald_scatter(g, S, _empty((0,), torch.int32), vol, cnt)
+    torch.cuda.synchronize()
+    assert int(cnt.abs().sum()) == 0
+    assert float(vol.abs().sum()) == 0.0
+
+
+def test_skipped_angle(gpu_ctx):
+    """An angle whose specular lies outside the Ewald sphere contributes
+    nothing (Reconstruction/src/fonctions.cpp:897)."""
+    import torch
+    case = make_case("tiny")
+    _say("skipped angle case=tiny")
+    g = case.lib_geom()
+    C, V, nx = g.C, g.dim_final, g.nxmax
+    rng = np.random.default_rng(3)
+    S = (rng.normal(size=(2, C, C)) + 1j * rng.normal(size=(2, C, C))).astype(np.complex64)
+    # corner specular: |fm0|^2 = 2*nx^2 > R^2 -> skipped
+    pos = np.array([0, nx, 0, nx], dtype=np.int64)
+    R = case.g.rayon
+    assert 2 * nx * nx > R * R
+    vol32, cnt32 = ref.ewald_scatter(S, pos, case.g, "f32")
+    vol = _empty((V, V, V), torch.complex64)
+    cnt = _empty((V, V, V), torch.int32)
+    gpu_ctx.ewald_scatter(g, _t(S), _t(pos.astype(np.int32)), vol, cnt)
+    torch.cuda.synchronize()
+    P = int((ref.pupil_tables(case.g)[2]).sum())
+    assert int(cnt.sum()) == P  # only the second angle
+    assert np.array_equal(cnt.cpu().numpy().ravel(), cnt32)
+    assert np.array_equal(vol.cpu().numpy().ravel().view(np.uint32), vol32.view(np.uint32))
+
+
+def test_invalid_geometry_errors(gpu_ctx):
+    import mtd_transmission_amd as m
+    import torch
+    case = make_case("tiny")
+    g = case.lib_geom()
+    g.dim_final = 16  # < 4*NXMAX: the reference would write out of bounds
+    V = 16
+    vol = _empty((V, V, V), torch.complex64)
+    cnt = _empty((V, V, V), torch.int32)
+    with pytest.raises(m.MTDError, match="DIM_FINAL"):
+        gpu_ctx.ewald_scatter(g, _empty((1, g.C, g.C), torch.complex64),
+                              _empty((2,), torch.int32), vol, cnt)
+    g = case.lib_geom()
+    g.deroul = 1
+    with pytest.raises(m.MTDError, match="DEROUL"):
+        gpu_ctx.holo_to_crop(g, _t(case.acq.holos), _empty((6, g.C, g.C), torch.complex64))
+    g = case.lib_geom()
+    g.circle_cx = 2  # crop window leaves the hologram
+    with pytest.raises(m.MTDError, match="crop window"):
+        gpu_ctx.holo_to_crop(g, _t(case.acq.holos), _empty((6, g.C, g.C), torch.complex64))
