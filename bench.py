@@ -1,0 +1,222 @@
+"""Benchmark of the plain method on one MI355X.
+
+Workload (the full-size single-GPU configuration): 600 off-axis holograms
+1024x1024 uint16 of a seeded 5-12 ellipsoid phantom, Fermat-spiral
+illumination filling 0.95 NA, Poisson shot noise at 20k photons peak,
+sideband crop 256x256, 512^3 complex64 spectrum + int32 hit counts, 3-D
+inverse FFT to a float32 index volume. One step = holograms already in HBM ->
+index volume (mtd_reconstruct_plain: both programs' compute).
+
+Prints one JSON line. value = holograms/s of the end-to-end step
+(A / step time); ms_per_step = ms per 512^3 volume.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("holograms/s and ms per 512³ volume; plain vs adjoint-TV at equal "
+          "index RMSE")
+PEAK_HBM_GBS = 8000.0
+
+# cfg: N, NXMAX, V, A
+WORKLOADS = {
+    "cfg2": dict(n=1024, nxmax=128, v=512, A=600, seed=2024),
+}
+LAM, N0, NA = 6.33e-7, 1.333, 1.2
+
+
+def geometry(w):
+    n, nx = w["n"], w["nxmax"]
+    pix = nx * LAM / (NA * n)
+    cx, cy = n // 2 - round(0.3 * n), n // 2 + round(0.3 * n)
+    return n, nx, w["v"], w["A"], pix, cx, cy
+
+
+def kernel_bytes(name, n, C, V, A, P, M, b_in=2):
+    """Algorithmic (compulsory) bytes of one step, per kernel: each stage
+    interface read once and written once."""
+    c2 = C * C * 8
+    table = {
+        "holo_rows": A * (n * n * b_in + n * C * 8),
+        "holo_cols": A * (n * C * 8 + c2),
+        "argmax": A * c2,
+        "demod_rows": A * 2 * c2,
+        "demod_cols": A * 2 * c2,
+        "lsq": A * M * 8,
+        "fwd_rows_field": A * 2 * c2,
+        "fwd_cols": A * 2 * c2,
+        "angle_geo": A * 32,
+        "ewald_gather": A * P * 8 + V ** 3 * 12,
+        "vol_x": V ** 3 * (12 + 8),
+        "vol_y": V ** 3 * 16,
+        "vol_z": V ** 3 * 16,
+    }
+    return table.get(name)
+
+
+def cpu_baseline_child(path_holos, path_out):
+    """Runs in a child with one thread: the oracle (FP64, the reference's
+    arithmetic) on a sample of holograms, plus one full-volume finalize."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import ref
+    d = np.load(path_holos)
+    holos, mask = d["holos"], d["mask"]
+    n, nx, v, A, pix, cx, cy = geometry(WORKLOADS["cfg2"])
+    g = ref.derive_geom(n, nx, v, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    S = holos.shape[0]
+    t0 = time.perf_counter()
+    fields, pos = ref.preprocess(holos, g, mask, "f64")
+    t1 = time.perf_counter()
+    spectra = np.stack([ref.field_to_spectrum(fields[a], int(pos[a]), int(pos[a + S]), g, "f64")
+                        for a in range(S)])
+    vol, cnt = ref.ewald_scatter(spectra, pos, g, "f64")
+    t2 = time.perf_counter()
+    ref.finalize(vol, cnt, g, "f64")
+    t3 = time.perf_counter()
+    json.dump({"S": S, "pre": t1 - t0, "rec": t2 - t1, "fin": t3 - t2}, open(path_out, "w"))
+
+
+def cpu_baseline(holos_dev, mask, sample):
+    import numpy as np
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="mtd_cpu_")
+    ph, po = os.path.join(tmp, "in.npz"), os.path.join(tmp, "out.json")
+    np.savez(ph, holos=holos_dev[:sample].cpu().numpy(), mask=mask)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1",
+               MKL_NUM_THREADS="1", HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
+    subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", ph, po],
+                   check=True, env=env)
+    r = json.load(open(po))
+    os.remove(ph)
+    os.remove(po)
+    os.rmdir(tmp)
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--cpu-sample", type=int, default=16,
+                    help="holograms timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-child", nargs=2, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_child:
+        cpu_baseline_child(*args.cpu_child)
+        return
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or ws != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import mtd_transmission_amd as m
+    from mtd_transmission_amd import synth
+
+    w = WORKLOADS[args.workload]
+    n, nx, V, A, pix, cx, cy = geometry(w)
+    C = 2 * nx
+    rng = np.random.default_rng(w["seed"])
+    n_ell = int(rng.integers(5, 13))
+    phantom = synth.make_phantom(w["seed"], n_ell)
+    kx, ky = synth.scan_fermat(A, nx, 0.95)
+    t0 = time.perf_counter()
+    holos, mask = synth.simulate_torch(n, nx, cx, cy, pix, LAM, N0, kx, ky, phantom,
+                                       seed=w["seed"] + 1)
+    torch.cuda.synchronize()
+    print(f"[bench] synthesized {A} holograms {n}^2 uint16 in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    g = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix)
+    ctx = m.Context(0)
+    re = torch.empty((V, V, V), dtype=torch.float32, device="cuda")
+    im = torch.empty_like(re)
+    pos = torch.empty((2 * A,), dtype=torch.int32, device="cuda")
+    for _ in range(args.warmup):
+        ctx.reconstruct_plain(g, holos, mask, re, im, pos)
+    torch.cuda.synchronize()
+
+    ctx.timing_reset()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reconstruct_plain(g, holos, mask, re, im, pos)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    kt = ctx.timing()
+    ms = (t1 - t0) * 1e3 / args.steps
+    print(f"[bench] {args.steps} steps, {ms:.3f} ms/volume", file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel
+    P = int(sum(1 for y in range(-nx, nx) for x in range(-nx, nx) if x * x + y * y < nx * nx))
+    M = int(((mask[::5, ::5]) > 45).sum())
+    per_kernel = {}
+    for name, (launches, tot) in kt.items():
+        b = kernel_bytes(name, n, C, V, A, P, M)
+        lps = launches / args.steps
+        avg = tot / launches
+        per_kernel[name] = {"ms_per_launch": round(avg, 5), "launches_per_step": lps,
+                            "ms_per_step": round(tot / args.steps, 4)}
+        if b is not None:
+            per_kernel[name]["GBps"] = round(b / lps / (avg * 1e-3) / 1e9, 1)
+    dom = max(kt, key=lambda k: kt[k][1])
+    launches, tot = kt[dom]
+    lps = launches / args.steps
+    bpl = kernel_bytes(dom, n, C, V, A, P, M) / lps
+    avg_s = tot / launches * 1e-3
+    achieved = bpl / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes_per_launch": int(bpl),
+                "ms_per_launch": round(avg_s * 1e3, 5)}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        r = cpu_baseline(holos, mask, args.cpu_sample)
+        per_holo = (r["pre"] + r["rec"]) / r["S"]
+        cpu_ms_volume = (per_holo * A + r["fin"]) * 1e3
+        cpu = {"value": round(A / (cpu_ms_volume / 1e3), 3), "unit": "holograms/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"FP64 oracle on {r['S']} of the {A} holograms (preprocess "
+                          f"{r['pre']:.2f}s + spectra/scatter {r['rec']:.2f}s) extrapolated "
+                          f"to {A}, plus one full {V}^3 finalize ({r['fin']:.2f}s); numpy "
+                          f"pocketfft, 1 thread"),
+               "ms_per_volume": round(cpu_ms_volume, 1)}
+
+    out = {
+        "metric": METRIC, "value": round(A / (ms / 1e3), 2), "unit": "holograms/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain method: {A} x {n}^2 uint16 holograms -> crop {C}^2 -> "
+                               f"{V}^3 complex64 spectrum + int32 counts -> {V}^3 float32 index",
+                   "holograms": A, "hologram_side": n, "crop": C, "volume": V},
+        "ms_per_volume": round(ms, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": per_kernel,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,15 @@ int mtd_ctx_set_stream(mtd_ctx* ctx, void* hip_stream);
 int mtd_ctx_sync(mtd_ctx* ctx);
 const char* mtd_last_error(const mtd_ctx* ctx);
 const char* mtd_version(void);
+/* Per-kernel timing with HIP events on the context's stream; replaces the
+ * std::chrono prints at the reference's stage boundaries
+ * (pretraitement_hors_axe_CPU/main.cpp:122-169,226-346;
+ * Reconstruction/main.cpp:200-229,298-330). on=1 starts recording. */
+int mtd_ctx_set_timing(mtd_ctx* ctx, int on);
+/* Synchronises, then writes "name launches total_ms\n" lines (one per kernel,
+ * totals since the last reset) into buf. Returns the number of lines. */
+int mtd_ctx_timing_report(mtd_ctx* ctx, char* buf, size_t cap);
+int mtd_ctx_timing_reset(mtd_ctx* ctx);
 
 /* ---- host-only helpers (no GPU needed) -------------------------------- */
 /* extract_val: last "KEY value" line, value through atof into a C float

@@ -58,6 +58,9 @@ SIGNATURES = {
     "mtd_ctx_set_stream": (_I, [_P, _P]),
     "mtd_ctx_sync": (_I, [_P]),
     "mtd_last_error": (ctypes.c_char_p, [_P]),
+    "mtd_ctx_set_timing": (_I, [_P, _I]),
+    "mtd_ctx_timing_report": (_I, [_P, ctypes.c_char_p, ctypes.c_size_t]),
+    "mtd_ctx_timing_reset": (_I, [_P]),
     "mtd_config_extract_val": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _D,
                                     ctypes.POINTER(ctypes.c_float)]),
     "mtd_config_extract_string": (_I, [ctypes.c_char_p, ctypes.c_char_p,
@@ -204,6 +207,22 @@ class Context:
 
     def sync(self):
         self._check(lib().mtd_ctx_sync(self.h), "sync")
+
+    def set_timing(self, on=True):
+        lib().mtd_ctx_set_timing(self.h, int(bool(on)))
+
+    def timing_reset(self):
+        lib().mtd_ctx_timing_reset(self.h)
+
+    def timing(self):
+        """{kernel: (launches, total_ms)} recorded since the last reset."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib().mtd_ctx_timing_report(self.h, buf, 1 << 16)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
 
     # ---- stages ---------------------------------------------------------
     def holo_to_crop(self, g: Geom, holos, crops):

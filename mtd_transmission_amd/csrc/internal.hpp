@@ -38,6 +38,16 @@ struct Buffer {
   size_t bytes = 0;
 };
 
+struct KTime {
+  double ms = 0;
+  long n = 0;
+};
+
+struct EvPair {
+  hipEvent_t a, b;
+  const char* name;
+};
+
 }  // namespace mtd
 
 struct mtd_ctx {
@@ -50,9 +60,40 @@ struct mtd_ctx {
   std::map<long long, float*> betatab;                    // per geometry hash
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   mtd::LsqPlan lsq;
+  // per-kernel event timing (mtd_ctx_set_timing)
+  bool timing = false;
+  std::vector<mtd::EvPair> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, mtd::KTime> ktimes;
 };
 
 namespace mtd {
+
+// Context of the entry point running on this thread (kernel timing hook).
+extern thread_local mtd_ctx* tl_ctx;
+hipEvent_t ev_take(mtd_ctx* c);
+
+// Brackets one kernel launch with events when timing is on.
+struct Timed {
+  mtd_ctx* c = nullptr;
+  const char* name;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  Timed(const char* n, hipStream_t st) : name(n), s(st) {
+    if (tl_ctx && tl_ctx->timing) {
+      c = tl_ctx;
+      a = ev_take(c);
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~Timed() {
+    if (!a) return;
+    hipEvent_t b = ev_take(c);
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    c->pending.push_back(EvPair{a, b, name});
+  }
+};
 
 // scratch slots
 enum WsSlot {

@@ -303,6 +303,7 @@ struct FwdColOp {
 template <int C>
 static hipError_t argmax_c(const float2* crop, int A, int* posx, int* posy,
                            hipStream_t s) {
+  Timed tm("argmax", s);
   hipLaunchKernelGGL((k_argmax<C>), dim3(A), dim3(256), 0, s, crop, posx, posy);
   return hipGetLastError();
 }
@@ -312,10 +313,14 @@ static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
                           int A, const float2* tw, float2* work, float2* iu,
                           hipStream_t s) {
   DemodRowOp<C> r{crop, posx, posy, work, tw, 0, 0};
-  hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A),
-                     rows_block<C>(), 0, s, r);
+  {
+    Timed tm("demod_rows", s);
+    hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A),
+                       rows_block<C>(), 0, s, r);
+  }
   ColInPlaceOp<C> c{work, iu, tw};
   constexpr int NC = cols_nc<C>();
+  Timed tm("demod_cols", s);
   hipLaunchKernelGGL((k_cols<C, true, NC, ColInPlaceOp<C>>), dim3(C / NC, A),
                      dim3(256), 0, s, c);
   return hipGetLastError();
@@ -324,6 +329,7 @@ static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
 template <int C>
 static hipError_t lsq_c(const float2* iu, int A, const LsqPlan& plan,
                         double* coef, hipStream_t s) {
+  Timed tm("lsq", s);
   hipLaunchKernelGGL((k_lsq<C>), dim3(A), dim3(256), 0, s, iu, plan.d_samples,
                      plan.M, plan.d_lu, plan.d_piv, coef);
   return hipGetLastError();
@@ -333,6 +339,7 @@ template <int C>
 static hipError_t form_c(const float2* iu, const float2* crop, const int* posx,
                          const int* posy, int A, const double* coef, int mode,
                          float2* fields, hipStream_t s) {
+  Timed tm("form_field", s);
   hipLaunchKernelGGL((k_form_field<C>), dim3((C * C + 255) / 256, A), dim3(256),
                      0, s, iu, crop, posx, posy, coef, mode, fields);
   return hipGetLastError();
@@ -346,10 +353,14 @@ static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
                         float2* work, float2* out, hipStream_t s) {
   FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode,
                 make_float2(0.f, 0.f), nullptr};
-  hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
-                     rows_block<C>(), 0, s, r);
+  {
+    Timed tm(from_iu ? "fwd_rows_field" : "fwd_rows", s);
+    hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
+                       rows_block<C>(), 0, s, r);
+  }
   FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, 0, 0};
   constexpr int NC = cols_nc<C>();
+  Timed tm("fwd_cols", s);
   hipLaunchKernelGGL((k_cols<C, false, NC, FwdColOp<C>>), dim3(C / NC, A),
                      dim3(256), 0, s, c);
   return hipGetLastError();

@@ -85,6 +85,7 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const double* win,
                          hipStream_t s) {
   constexpr int NP = divisor_le(N / 2, fft_per_block(N));
   dim3 grid(N / (2 * NP), A), block(NP * fft_threads(N));
+  Timed tm("holo_rows", s);
   switch (dtype) {
     case MTD_U8:
       hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
@@ -107,6 +108,7 @@ static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
                          int x0, int y0, float2* crop, hipStream_t s) {
   constexpr int NC = N >= 2048 ? 4 : 8;
   const float scale = (float)(0.5 / ((double)N * (double)N));
+  Timed tm("holo_cols", s);
   hipLaunchKernelGGL((k_holo_cols<N, NC>), dim3(C / NC, A), dim3(256), 0, s,
                      t1, tw, C, x0, y0, scale, crop);
   return hipGetLastError();

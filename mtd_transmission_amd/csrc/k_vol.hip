@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g) {
 hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
                             double R2, const double* htab, int r2tab,
                             AngleGeo* ageo, hipStream_t s) {
+  Timed tm("angle_geo", s);
   hipLaunchKernelGGL(k_angle_geo, dim3((A + 127) / 128), dim3(128), 0, s, posx,
                      posy, A, nxmax, R2, htab, r2tab, ageo);
   return hipGetLastError();
@@ -182,6 +183,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
+  Timed tm("ewald_gather", s);
   hipLaunchKernelGGL((k_gather<16>), dim3(V / GT, V / GT), dim3(256), 0, s, g);
   return hipGetLastError();
 }
@@ -244,13 +246,20 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, float2* work
                              float* im, uint8_t* otf, const float2* tw,
                              hipStream_t s) {
   NormXOp<V> x{vol, cnt, work, otf, tw};
-  hipLaunchKernelGGL((k_rows<V, true, NormXOp<V>>), rows_grid<V>(V), rows_block<V>(),
-                     0, s, x);
+  {
+    Timed tm("vol_x", s);
+    hipLaunchKernelGGL((k_rows<V, true, NormXOp<V>>), rows_grid<V>(V), rows_block<V>(),
+                       0, s, x);
+  }
   constexpr int NC = cols_nc<V>();
   PlaneYOp<V> y{work, tw};
-  hipLaunchKernelGGL((k_cols<V, true, NC, PlaneYOp<V>>), dim3(V / NC, V), dim3(256),
-                     0, s, y);
+  {
+    Timed tm("vol_y", s);
+    hipLaunchKernelGGL((k_cols<V, true, NC, PlaneYOp<V>>), dim3(V / NC, V), dim3(256),
+                       0, s, y);
+  }
   ZOutOp<V> z{work, re, im, tw, df3, cte, gsign};
+  Timed tm("vol_z", s);
   hipLaunchKernelGGL((k_cols<V, true, NC, ZOutOp<V>>), dim3(V / NC, V), dim3(256),
                      0, s, z);
   return hipGetLastError();

@@ -16,6 +16,42 @@ bool holo_len_supported(int N);
 bool crop_len_supported(int C);
 bool vol_len_supported(int V);
 
+thread_local mtd_ctx* tl_ctx = nullptr;
+
+hipEvent_t ev_take(mtd_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Sets the thread's current context for the kernel-timing hook.
+struct CtxScope {
+  mtd_ctx* prev;
+  explicit CtxScope(mtd_ctx* c) : prev(tl_ctx) { tl_ctx = c; }
+  ~CtxScope() { tl_ctx = prev; }
+};
+
+static void timing_collect(mtd_ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (EvPair& p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      KTime& k = c->ktimes[p.name];
+      k.ms += ms;
+      k.n += 1;
+    }
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
 int set_err(mtd_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -360,6 +396,8 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (auto& kv : ctx->htab) hipFree(kv.second);
   for (auto& kv : ctx->betatab) hipFree(kv.second);
   for (auto& kv : ctx->ws) hipFree(kv.second.ptr);
+  timing_collect(ctx);
+  for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   hipFree(ctx->lsq.d_samples);
   hipFree(ctx->lsq.d_lu);
   hipFree(ctx->lsq.d_piv);
@@ -379,6 +417,33 @@ int mtd_ctx_sync(mtd_ctx* ctx) {
 }
 
 const char* mtd_last_error(const mtd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mtd_ctx_set_timing(mtd_ctx* ctx, int on) {
+  if (!ctx) return MTD_ERR_ARG;
+  ctx->timing = on != 0;
+  return MTD_OK;
+}
+
+int mtd_ctx_timing_report(mtd_ctx* ctx, char* buf, size_t cap) {
+  if (!ctx || !buf || cap == 0) return -1;
+  timing_collect(ctx);
+  std::string out;
+  for (const auto& kv : ctx->ktimes) {
+    char line[256];
+    snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.n, kv.second.ms);
+    out += line;
+  }
+  std::strncpy(buf, out.c_str(), cap - 1);
+  buf[cap - 1] = 0;
+  return (int)ctx->ktimes.size();
+}
+
+int mtd_ctx_timing_reset(mtd_ctx* ctx) {
+  if (!ctx) return MTD_ERR_ARG;
+  timing_collect(ctx);
+  ctx->ktimes.clear();
+  return MTD_OK;
+}
 
 int mtd_tukey1d(int n, double alpha, double* out) {
   if (n < 2 || !out) return MTD_ERR_ARG;
@@ -424,6 +489,7 @@ int mtd_geom_derive(double ccd_roix, double nxmax, double dim_final, double circ
 
 int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype,
                      int A, float* crops) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_HOLO));
   if (A == 0) return MTD_OK;
   if (!holos || !crops) return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -438,6 +504,7 @@ int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dty
 
 int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A,
                       const uint8_t* mask, int32_t* posspec, float* fields) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, 0));
   if (A == 0) return MTD_OK;
   if (!crops || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -454,6 +521,7 @@ int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A
 
 int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                    const uint8_t* mask, int32_t* posspec, float* fields) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_HOLO));
   if (A == 0) return MTD_OK;
   if (!holos || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -473,6 +541,7 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
 
 int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
                           const int32_t* posspec, int A, float* spectra) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, 0));
   if (A == 0) return MTD_OK;
   if (!fields || !posspec || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -513,6 +582,7 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
 
 int mtd_ewald_scatter(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                       const int32_t* posspec, int A, float* vol, int32_t* counts) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
   if (!vol || !counts || (A > 0 && (!spectra || !posspec)))
     return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -521,6 +591,7 @@ int mtd_ewald_scatter(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
 
 int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
                          const int32_t* posspec, int A, float* vol, int32_t* counts) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
   if (!vol || !counts || (A > 0 && (!fields || !posspec)))
     return set_err(ctx, MTD_ERR_ARG, "null array");
@@ -545,6 +616,7 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
 
 int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_t* counts,
                  float* index_re, float* index_im, uint8_t* otf) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, 0, NEED_VOL));
   if (!vol || !counts || !index_re || !index_im) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int V = g->dim_final;
@@ -563,6 +635,7 @@ int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                           const uint8_t* mask, int32_t* posspec, float* index_re,
                           float* index_im) {
+  CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_HOLO | NEED_VOL));
   if (!index_re || !index_im || (A > 0 && !holos)) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int N = g->n, C = 2 * g->nxmax, V = g->dim_final;

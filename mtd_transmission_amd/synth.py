@@ -219,3 +219,86 @@ def simulate(n, nxmax, circle_cx, circle_cy, pix, lam, n0, kx, ky, phantom,
             out[a] = (I * (0.8 * peak / full)).astype(dtype)
     mask = support_mask(n, nxmax, pix, lam, n0, kx, ky, phantom)
     return Acquisition(out, mask, np.asarray(kx), np.asarray(ky), phantom)
+
+
+# --------------------------------------------------------------------------
+# GPU (torch) variant of the same generator, for full-size benchmark inputs
+# --------------------------------------------------------------------------
+def _torch_chords(P, d, phantom, torch):
+    """Chord lengths for rays P(...,3) + t d(b,3): list of (b, ...) tensors."""
+    out = []
+    for e in phantom:
+        M = torch.tensor(e.quad, dtype=P.dtype, device=P.device)
+        c = torch.tensor(e.centre, dtype=P.dtype, device=P.device)
+        q = P - c
+        a = torch.einsum("bi,ij,bj->b", d, M, d)
+        qM = q @ M
+        b = torch.einsum("...i,bi->b...", qM, d)
+        cc = (qM * q).sum(-1) - 1.0
+        disc = b * b - a.view(-1, *([1] * cc.dim())) * cc
+        out.append(2.0 * torch.sqrt(torch.clamp(disc, min=0.0)) /
+                   a.view(-1, *([1] * cc.dim())))
+    return out
+
+
+def simulate_torch(n, nxmax, circle_cx, circle_cy, pix, lam, n0, kx, ky, phantom,
+                   seed=0, photons=20000.0, ref_amp=1.0, device="cuda", chunk=8,
+                   lowpass=True):
+    """Same physics as ``simulate`` with Poisson noise, evaluated on the GPU
+    with torch: returns a uint16 (A, N, N) device tensor and the uint8 mask."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    A = len(kx)
+    out = torch.empty((A, n, n), dtype=torch.uint16, device=device)
+    ar = torch.arange(n, device=device, dtype=torch.float32)
+    yy, xx = torch.meshgrid(ar, ar, indexing="ij")
+    P = torch.stack([(xx - n / 2) * pix, (yy - n / 2) * pix, torch.zeros_like(xx)], -1)
+    f = torch.fft.fftfreq(n, device=device) * n
+    pupil = ((f[None, :] ** 2 + f[:, None] ** 2) < nxmax * nxmax).to(torch.complex64)
+    fcx, fcy = n // 2 - circle_cx, n // 2 - circle_cy
+    full = (ref_amp + 1.0) ** 2
+    kvac = 2 * math.pi / lam
+    for a0 in range(0, A, chunk):
+        a1 = min(A, a0 + chunk)
+        kxs = torch.tensor(np.asarray(kx[a0:a1], dtype=np.float64), device=device)
+        kys = torch.tensor(np.asarray(ky[a0:a1], dtype=np.float64), device=device)
+        d = torch.tensor(_directions(np.asarray(kx[a0:a1], np.float64),
+                                     np.asarray(ky[a0:a1], np.float64), n, pix, lam, n0),
+                         dtype=torch.float32, device=device)
+        phi = torch.zeros((a1 - a0, n, n), dtype=torch.float32, device=device)
+        for e, L in zip(phantom, _torch_chords(P, d, phantom, torch)):
+            phi += float(kvac * e.dn) * L
+        ramp = 2 * math.pi * (kxs.view(-1, 1, 1).float() * xx + kys.view(-1, 1, 1).float() * yy) / n
+        u = torch.polar(torch.ones_like(phi), torch.remainder(ramp, 2 * math.pi) + phi)
+        if lowpass:
+            u = torch.fft.ifft2(torch.fft.fft2(u) * pupil)
+        psi = (math.pi * ((kxs - fcx) + (kys - fcy))).float().view(-1, 1, 1)
+        rphase = torch.remainder(2 * math.pi * (fcx * xx + fcy * yy) / n, 2 * math.pi)
+        r = torch.polar(torch.full_like(phi, ref_amp), rphase + psi)
+        inten = (r + u).abs() ** 2
+        v = torch.poisson(inten * (photons / full), generator=gen)
+        out[a0:a1] = torch.clamp(v, 0, 65535).to(torch.int32).to(torch.uint16)
+    # mask on the field grid, union over all illuminations
+    C = 2 * nxmax
+    arc = torch.arange(C, device=device, dtype=torch.float32)
+    cy_, cx_ = torch.meshgrid(arc, arc, indexing="ij")
+    Pm = torch.stack([(cx_ * n / C - n / 2) * pix, (cy_ * n / C - n / 2) * pix,
+                      torch.zeros_like(cx_)], -1)
+    inside = torch.zeros((C, C), dtype=torch.bool, device=device)
+    dall = torch.tensor(_directions(np.asarray(kx, np.float64), np.asarray(ky, np.float64),
+                                    n, pix, lam, n0), dtype=torch.float32, device=device)
+    for b0 in range(0, A, 64):
+        for L in _torch_chords(Pm, dall[b0:b0 + 64], phantom, torch):
+            inside |= (L > 0).any(0)
+    grown = inside.cpu().numpy()
+    for _ in range(2):
+        g = grown.copy()
+        g[1:, :] |= grown[:-1, :]
+        g[:-1, :] |= grown[1:, :]
+        g[:, 1:] |= grown[:, :-1]
+        g[:, :-1] |= grown[:, 1:]
+        grown = g
+    grown[:2, :] = grown[-2:, :] = True
+    grown[:, :2] = grown[:, -2:] = True
+    return out, np.where(grown, 0, 255).astype(np.uint8)

@@ -36,7 +36,11 @@ class Case:
 
 SPECS = {
     # name: N, NXMAX, V, A, n_ellipsoids, noise, dtype, scan
+    # tiny: C=16 is too small for a meaningful degree-4 fit (16 samples for
+    # 15 unknowns); it exercises the FFT, scatter and volume stages only.
     "tiny": (64, 8, 32, 6, 2, "gauss", np.float32, "circle"),
+    # nomask: no Image_mask.pgm (all pixels background, every 5th sampled)
+    "nomask": (128, 16, 64, 10, 2, "gauss", np.float32, "circle"),
     "small": (128, 16, 64, 16, 3, "gauss", np.float32, "circle"),
     "u16": (128, 16, 64, 12, 3, "poisson", np.uint16, "fermat"),
     "cfg1": (512, 64, 256, 100, 3, "gauss", np.float32, "circle"),
@@ -61,8 +65,8 @@ def make_case(name, born=1, c_aber=1, seed=7):
         kx, ky = synth.scan_fermat(A, nx, 0.95)
     acq = synth.simulate(n, nx, cx, cy, g.pix_holo, g.lam, g.n0, kx, ky, ph,
                          seed=seed + 1, noise=noise, dtype=dtype)
-    if 2 * nx < 32:
-        acq.mask = None  # too small for a masked fit: the absent-mask path
+    if name in ("tiny", "nomask"):
+        acq.mask = None  # the absent-mask path (Correction_aberration.cpp:48-52)
     params = dict(n=n, nxmax=nx, v=v, cx=cx, cy=cy, pix=pix, born=born, c_aber=c_aber)
     return Case(name, g, acq, params)
 

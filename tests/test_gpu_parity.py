@@ -61,7 +61,7 @@ def test_holo_to_crop(gpu_ctx, name):
     assert e32 < TOL
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "u16", "cfg1"])
+@pytest.mark.parametrize("name", ["nomask", "small", "u16", "cfg1"])
 def test_crop_to_field(gpu_ctx, name):
     import torch
     case = make_case(name)
@@ -103,7 +103,7 @@ def test_crop_to_field_modes(gpu_ctx, name, born, c_aber):
     assert e32 < TOL
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
+@pytest.mark.parametrize("name", ["nomask", "small", "cfg1"])
 def test_preprocess(gpu_ctx, name):
     import torch
     case = make_case(name)
@@ -188,11 +188,12 @@ def test_finalize(gpu_ctx, name):
     assert e32 < TOL
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
-def test_reconstruct_plain(gpu_ctx, name):
+@pytest.mark.parametrize("name,c_aber", [("nomask", 1), ("small", 1), ("cfg1", 1),
+                                         ("tiny", 0)])
+def test_reconstruct_plain(gpu_ctx, name, c_aber):
     """Holograms to index in one call vs the oracle's two programs."""
     import torch
-    case = make_case(name)
+    case = make_case(name, c_aber=c_aber)
     _say(f"reconstruct_plain case={name}")
     g = case.lib_geom()
     fields32, pos32 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f32")
