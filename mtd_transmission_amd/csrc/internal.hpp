@@ -98,7 +98,7 @@ struct Timed {
 // scratch slots
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
-  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP
+  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -120,7 +120,8 @@ hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
                             hipStream_t s);
 
 hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
-                         int* posy, hipStream_t s);
+                         int* posy, float2* part, hipStream_t s);
+size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
                         const int* posy, int A, const float2* tw,
                         float2* work, float2* iu, hipStream_t s);
@@ -148,10 +149,12 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, hipStream_t s);
+                         int32_t* cnt, void* scratch, int flags, hipStream_t s);
+size_t gather_scratch_bytes(int V, int A);
+// prenorm=1: vol already divided by counts and centring-signed (gather flag 1)
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
-                           float2* work, float df3, float cte, float gsign,
-                           float* out_re, float* out_im, uint8_t* otf,
-                           const float2* tw, hipStream_t s);
+                           int prenorm, float2* work, float df3, float cte,
+                           float gsign, float* out_re, float* out_im,
+                           uint8_t* otf, const float2* tw, hipStream_t s);
 
 }  // namespace mtd

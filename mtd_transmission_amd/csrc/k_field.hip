@@ -20,15 +20,22 @@ namespace mtd {
 // ---------------------------------------------------------------------------
 // argmax with the slot-0 squared quirk; first index of the maximum.
 // ---------------------------------------------------------------------------
+constexpr int ARGMAX_SEG = 16;  // blocks per crop in the first stage
+
+__device__ __forceinline__ bool better(float v2, int i2, float v1, int i1) {
+  return v2 > v1 || (v2 == v1 && i2 < i1);
+}
+
 template <int C>
 __global__ void __launch_bounds__(256)
-k_argmax(const float2* __restrict__ crop, int* __restrict__ posx,
-         int* __restrict__ posy) {
-  const int a = blockIdx.x;
+k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
+  const int a = blockIdx.y, seg = blockIdx.x;
+  constexpr int LEN = (C * C + ARGMAX_SEG - 1) / ARGMAX_SEG;
   const float2* p = crop + (size_t)a * C * C;
   float best = -1.f;
   int bi = 0x7fffffff;
-  for (int i = threadIdx.x; i < C * C; i += 256) {
+  const int i0 = seg * LEN, i1 = min(i0 + LEN, C * C);
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     const float2 v = p[i];
     const float m = (i == 0) ? __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y))
                              : hypotf(v.x, v.y);
@@ -43,17 +50,31 @@ k_argmax(const float2* __restrict__ crop, int* __restrict__ posx,
     if (threadIdx.x < s) {
       const float v2 = sv[threadIdx.x + s];
       const int i2 = si[threadIdx.x + s];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+      if (better(v2, i2, sv[threadIdx.x], si[threadIdx.x])) {
         sv[threadIdx.x] = v2;
         si[threadIdx.x] = i2;
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    posx[a] = si[0] % C;
-    posy[a] = si[0] / C;
+  if (threadIdx.x == 0) part[(size_t)a * ARGMAX_SEG + seg] = make_float2(sv[0], __int_as_float(si[0]));
+}
+
+// second stage: one thread per angle over its segments, in order
+template <int C>
+__global__ void k_argmax_fin(const float2* __restrict__ part, int A, int* __restrict__ posx,
+                             int* __restrict__ posy) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= A) return;
+  float best = -1.f;
+  int bi = 0x7fffffff;
+  for (int s = 0; s < ARGMAX_SEG; ++s) {
+    const float2 p = part[(size_t)a * ARGMAX_SEG + s];
+    const int i = __float_as_int(p.y);
+    if (better(p.x, i, best, bi)) { best = p.x; bi = i; }
   }
+  posx[a] = bi % C;
+  posy[a] = bi / C;
 }
 
 // ---------------------------------------------------------------------------
@@ -302,11 +323,17 @@ struct FwdColOp {
 // ---------------------------------------------------------------------------
 template <int C>
 static hipError_t argmax_c(const float2* crop, int A, int* posx, int* posy,
-                           hipStream_t s) {
-  Timed tm("argmax", s);
-  hipLaunchKernelGGL((k_argmax<C>), dim3(A), dim3(256), 0, s, crop, posx, posy);
+                           float2* part, hipStream_t s) {
+  {
+    Timed tm("argmax", s);
+    hipLaunchKernelGGL((k_argmax_part<C>), dim3(ARGMAX_SEG, A), dim3(256), 0, s, crop, part);
+  }
+  Timed tm("argmax_fin", s);
+  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, A, posx, posy);
   return hipGetLastError();
 }
+
+size_t argmax_scratch_bytes(int A) { return (size_t)(A > 0 ? A : 1) * ARGMAX_SEG * sizeof(float2); }
 
 template <int C>
 static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
@@ -370,9 +397,9 @@ static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
   X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(256) X(288) X(320) X(352) X(384) X(512)
 
 hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
-                         int* posy, hipStream_t s) {
+                         int* posy, float2* part, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return argmax_c<n>(crop, A, posx, posy, s);
+#define X(n) case n: return argmax_c<n>(crop, A, posx, posy, part, s);
     MTD_CROP_SIZES(X)
 #undef X
   }

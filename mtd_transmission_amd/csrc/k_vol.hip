@@ -6,13 +6,13 @@
 // angle a contributes at most one hit: at pupil pixel fd = q + fm0(a), at
 // height z = round(h(|fd|^2) - fzm0(a)) + V/2 with h(r2) = sqrt(R^2 - r2),
 // provided fd lies in the [-NXMAX, NXMAX)^2 crop, |fd|^2 < coef*NXMAX^2 and the
-// angle is not skipped. Each thread owns one column and walks the angles in
-// order, so every voxel receives its hits in the reference's angle order:
-// this is the sort-by-voxel of the accumulation done analytically (the
-// scatter map is invertible per angle), and the sum is deterministic and
-// equal to the sequential FP32 sum. Partial columns live in LDS windows of
-// ZW heights; every voxel of the V^3 spectrum and count volume is written
-// exactly once (zeros included), so no clearing pass is needed.
+// angle is not skipped. Each thread owns one column of a (column tile, height
+// window) work item and walks the angles in order, so every voxel receives
+// its hits in the reference's angle order: this is the sort-by-voxel of the
+// accumulation done analytically (the scatter map is invertible per angle),
+// and the sum is deterministic and equal to the sequential FP32 sum. Partial
+// columns live in LDS; every voxel of the V^3 spectrum and count volume is
+// written exactly once (zeros included), so no clearing pass is needed.
 //
 // Finalisation (Reconstruction/main.cpp:243-352): divide by counts (0 -> 1),
 // backward 3-D FFT with both fftshift3D folded into (-1)^(x+y+z) sign flips,
@@ -83,85 +83,141 @@ __device__ __forceinline__ bool gather_hit(const GatherArgs& g, const AngleGeo& 
   return true;
 }
 
-// Uniform (whole-tile) rejection of angle a.
-__device__ __forceinline__ bool tile_misses(const GatherArgs& g, const AngleGeo& ag,
-                                            int qx0, int qy0) {
-  if (ag.skip) return true;
-  const int lx = qx0 + ag.fx, hx = lx + GT - 1;
-  const int ly = qy0 + ag.fy, hy = ly + GT - 1;
-  if (hx < -g.nx || lx >= g.nx || hy < -g.nx || ly >= g.nx) return true;
-  const int dx = lx > 0 ? lx : (hx < 0 ? -hx : 0);
-  const int dy = ly > 0 ? ly : (hy < 0 ? -hy : 0);
-  return dx * dx + dy * dy > g.r2max;
-}
+// ---------------------------------------------------------------------------
+// Windowed gather. A prepass lists, per 16x16 column tile and in angle order,
+// the angles whose pupil disc touches the tile, with a conservative height
+// range derived from the tile box's nearest and farthest pupil radii (h is
+// decreasing in r^2 and round is monotonic). The main kernel gives each
+// block one (tile, ZW-height window); angles whose range misses the window
+// are skipped with scalar tests, so each hit is tested in about
+// 1 + span/ZW windows instead of every window of the band.
+// ---------------------------------------------------------------------------
+struct TileEntry {
+  int a;
+  int zpack;  // zlo | zhi << 16, absolute heights
+};
 
-template <int ZW>
-__global__ void __launch_bounds__(256) k_gather(GatherArgs g) {
-  __shared__ float2 acc[ZW][256];
-  __shared__ int cn[ZW][256];
-  __shared__ int zred[2][4];
-  const int tid = threadIdx.x;
-  const int tx = tid % GT, ty = tid / GT;
-  const int X = blockIdx.x * GT + tx, Y = blockIdx.y * GT + ty;
-  const int qx0 = blockIdx.x * GT - g.V / 2, qy0 = blockIdx.y * GT - g.V / 2;
-  const int qx = X - g.V / 2, qy = Y - g.V / 2;
-  const size_t plane = (size_t)g.V * g.V;
-  const size_t col = (size_t)Y * g.V + X;
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-  // height range of this tile
-  int zlo = 0x7fffffff, zhi = -0x7fffffff;
-  for (int a = 0; a < g.A; ++a) {
-    const AngleGeo ag = g.ageo[a];
-    if (tile_misses(g, ag, qx0, qy0)) continue;
-    int z;
-    float2 v;
-    if (gather_hit(g, ag, a, qx, qy, z, v)) {
-      zlo = min(zlo, z);
-      zhi = max(zhi, z);
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    zlo = min(zlo, __shfl_xor(zlo, off, 64));
-    zhi = max(zhi, __shfl_xor(zhi, off, 64));
-  }
-  if ((tid & 63) == 0) { zred[0][tid >> 6] = zlo; zred[1][tid >> 6] = zhi; }
-  __syncthreads();
-  zlo = min(min(zred[0][0], zred[0][1]), min(zred[0][2], zred[0][3]));
-  zhi = max(max(zred[1][0], zred[1][1]), max(zred[1][2], zred[1][3]));
-  if (zlo < 0) zlo = 0;
-  if (zhi > g.V - 1) zhi = g.V - 1;
-
-  // zero heights outside [zlo, zhi]
-  const float2 zero2 = make_float2(0.f, 0.f);
-  for (int z = 0; z < g.V; ++z) {
-    if (z >= zlo && z <= zhi) { z = zhi; continue; }
-    g.vol[z * plane + col] = zero2;
-    g.cnt[z * plane + col] = 0;
-  }
-  // windows of ZW heights
-  for (int w0 = zlo; w0 <= zhi; w0 += ZW) {
-#pragma unroll
-    for (int k = 0; k < ZW; ++k) { acc[k][tid] = zero2; cn[k][tid] = 0; }
-    for (int a = 0; a < g.A; ++a) {
+__global__ void __launch_bounds__(256)
+k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo) {
+  __shared__ int wsum[4];
+  __shared__ int zr[2][4];
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int qx0 = (tile % ntx) * GT - g.V / 2, qy0 = (tile / ntx) * GT - g.V / 2;
+  int running = 0, zmin = 0x7fffffff, zmax = -1;
+  for (int a0 = 0; a0 < g.A; a0 += 256) {
+    const int a = a0 + tid;
+    bool act = false;
+    int zl = 0, zh = 0;
+    if (a < g.A) {
       const AngleGeo ag = g.ageo[a];
-      if (tile_misses(g, ag, qx0, qy0)) continue;
-      int z;
-      float2 v;
-      if (gather_hit(g, ag, a, qx, qy, z, v)) {
-        const int k = z - w0;
-        if (k >= 0 && k < ZW) {
-          float2 s = acc[k][tid];
-          s.x = __fadd_rn(s.x, v.x);
-          s.y = __fadd_rn(s.y, v.y);
-          acc[k][tid] = s;
-          cn[k][tid] += 1;
+      if (!ag.skip) {
+        const int cxl = max(qx0 + ag.fx, -g.nx), cxh = min(qx0 + GT - 1 + ag.fx, g.nx - 1);
+        const int cyl = max(qy0 + ag.fy, -g.nx), cyh = min(qy0 + GT - 1 + ag.fy, g.nx - 1);
+        if (cxl <= cxh && cyl <= cyh) {
+          const int mx = cxl > 0 ? cxl : (cxh < 0 ? -cxh : 0);
+          const int my = cyl > 0 ? cyl : (cyh < 0 ? -cyh : 0);
+          const int Mx = max(abs(cxl), abs(cxh)), My = max(abs(cyl), abs(cyh));
+          const int dmin = mx * mx + my * my;
+          if (dmin <= g.r2max) {
+            const int dmax = min(Mx * Mx + My * My, g.r2max);
+            zl = clampi((int)round_away(g.htab[dmax] - ag.fz) + g.V / 2, 0, g.V - 1);
+            zh = clampi((int)round_away(g.htab[dmin] - ag.fz) + g.V / 2, 0, g.V - 1);
+            act = true;
+          }
         }
       }
     }
-    const int kend = min(ZW, zhi - w0 + 1);
-    for (int k = 0; k < kend; ++k) {
-      g.vol[(size_t)(w0 + k) * plane + col] = acc[k][tid];
-      g.cnt[(size_t)(w0 + k) * plane + col] = cn[k][tid];
+    // deterministic block scan of the active flags (angle order kept)
+    const unsigned long long bal = __ballot(act);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(bal);
+    __syncthreads();
+    int off = running;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    if (act) {
+      list[(size_t)tile * g.A + off + before] = TileEntry{a, zl | (zh << 16)};
+      zmin = min(zmin, zl);
+      zmax = max(zmax, zh);
+    }
+    running += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    zmin = min(zmin, __shfl_xor(zmin, o, 64));
+    zmax = max(zmax, __shfl_xor(zmax, o, 64));
+  }
+  if (lane == 0) { zr[0][wave] = zmin; zr[1][wave] = zmax; }
+  __syncthreads();
+  if (tid == 0) {
+    const int lo = min(min(zr[0][0], zr[0][1]), min(zr[0][2], zr[0][3]));
+    const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
+    tinfo[tile] = make_int4(running, lo, hi, 0);
+  }
+}
+
+// flags: bit0 = normalise (divide by counts, 0 -> 1) and apply the centring
+// sign (-1)^(x+y+z) of the 3-D inverse FFT; counts are then not written.
+template <int ZW>
+__global__ void __launch_bounds__(256)
+k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __restrict__ list,
+             const int4* __restrict__ tinfo, int flags) {
+  __shared__ float2 acc[ZW][256];
+  __shared__ int cn[ZW][256];
+  // consecutive work items on one XCD (blocks are dealt round-robin to the 8 XCDs)
+  const int per = (items + 7) / 8;
+  const int item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (item >= items) return;
+  const int tile = item / nwin, w = item % nwin;
+  const int tid = threadIdx.x;
+  const int X = (tile % ntx) * GT + tid % GT, Y = (tile / ntx) * GT + tid / GT;
+  const int qx = X - g.V / 2, qy = Y - g.V / 2;
+  const size_t plane = (size_t)g.V * g.V;
+  const size_t col = (size_t)Y * g.V + X;
+  const int z0 = w * ZW, z1 = min(z0 + ZW, g.V) - 1;
+  const int4 info = tinfo[tile];
+  const bool norm = flags & 1;
+  if (info.x == 0 || z1 < info.y || z0 > info.z) {
+    for (int z = z0; z <= z1; ++z) {
+      g.vol[z * plane + col] = make_float2(0.f, 0.f);
+      if (!norm) g.cnt[z * plane + col] = 0;
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < ZW; ++k) { acc[k][tid] = make_float2(0.f, 0.f); cn[k][tid] = 0; }
+  const TileEntry* lst = list + (size_t)tile * g.A;
+  for (int i = 0; i < info.x; ++i) {
+    const TileEntry e = lst[i];
+    const int zl = e.zpack & 0xffff, zh = e.zpack >> 16;
+    if (zh < z0 || zl > z1) continue;
+    const AngleGeo ag = g.ageo[e.a];
+    int z;
+    float2 v;
+    if (gather_hit(g, ag, e.a, qx, qy, z, v)) {
+      const int k = z - z0;
+      if (k >= 0 && k < ZW) {
+        float2 s = acc[k][tid];
+        s.x = __fadd_rn(s.x, v.x);
+        s.y = __fadd_rn(s.y, v.y);
+        acc[k][tid] = s;
+        cn[k][tid] += 1;
+      }
+    }
+  }
+  for (int z = z0; z <= z1; ++z) {
+    const float2 s = acc[z - z0][tid];
+    const int c = cn[z - z0][tid];
+    if (norm) {
+      const float d = (float)(c == 0 ? 1 : c);
+      float2 t = make_float2(s.x / d, s.y / d);
+      if ((X + Y + z) & 1) { t.x = -t.x; t.y = -t.y; }
+      g.vol[z * plane + col] = t;
+    } else {
+      g.vol[z * plane + col] = s;
+      g.cnt[z * plane + col] = c;
     }
   }
 }
@@ -179,13 +235,28 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, hipStream_t s) {
+                         int32_t* cnt, void* scratch, int flags, hipStream_t s) {
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
+  constexpr int ZW = 16;
+  const int ntx = V / GT, tiles = ntx * ntx;
+  TileEntry* list = (TileEntry*)scratch;
+  int4* tinfo = (int4*)((char*)scratch + gather_scratch_bytes(V, A) - (size_t)tiles * sizeof(int4));
+  {
+    Timed tm("ewald_tiles", s);
+    hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo);
+  }
+  const int nwin = (V + ZW - 1) / ZW, items = tiles * nwin;
   Timed tm("ewald_gather", s);
-  hipLaunchKernelGGL((k_gather<16>), dim3(V / GT, V / GT), dim3(256), 0, s, g);
+  hipLaunchKernelGGL((k_gather_win<ZW>), dim3(8 * ((items + 7) / 8)), dim3(256), 0, s, g,
+                     ntx, nwin, items, (const TileEntry*)list, (const int4*)tinfo, flags);
   return hipGetLastError();
+}
+
+size_t gather_scratch_bytes(int V, int A) {
+  const size_t tiles = (size_t)(V / GT) * (V / GT);
+  return tiles * (size_t)(A > 0 ? A : 1) * sizeof(TileEntry) + tiles * sizeof(int4);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,9 +269,15 @@ struct NormXOp {  // rows along x; a = z, line = y
   float2* out;
   uint8_t* otf;
   const float2* tw;
+  int prenorm;
   __device__ void begin(int) {}
   __device__ float2 load(int z, int y, int x) const {
     const size_t i = ((size_t)z * V + y) * V + x;
+    if (prenorm) {  // already t * (-1)^(x+y+z)
+      const float2 t = vol[i];
+      if (otf) otf[i] = (t.x != 0.f) ? 1 : 0;
+      return t;
+    }
     const float2 v = vol[i];
     const int c0 = cnt[i];
     const float c = (float)(c0 == 0 ? 1 : c0);
@@ -241,11 +318,11 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
 };
 
 template <int V>
-static hipError_t finalize_v(const float2* vol, const int32_t* cnt, float2* work,
+static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm, float2* work,
                              float df3, float cte, float gsign, float* re,
                              float* im, uint8_t* otf, const float2* tw,
                              hipStream_t s) {
-  NormXOp<V> x{vol, cnt, work, otf, tw};
+  NormXOp<V> x{vol, cnt, work, otf, tw, prenorm};
   {
     Timed tm("vol_x", s);
     hipLaunchKernelGGL((k_rows<V, true, NormXOp<V>>), rows_grid<V>(V), rows_block<V>(),
@@ -269,12 +346,12 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, float2* work
   X(32) X(64) X(128) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(704) X(768) X(1024)
 
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
-                           float2* work, float df3, float cte, float gsign,
-                           float* out_re, float* out_im, uint8_t* otf,
-                           const float2* tw, hipStream_t s) {
+                           int prenorm, float2* work, float df3, float cte,
+                           float gsign, float* out_re, float* out_im,
+                           uint8_t* otf, const float2* tw, hipStream_t s) {
   switch (V) {
 #define X(n) \
-  case n: return finalize_v<n>(vol, cnt, work, df3, cte, gsign, out_re, out_im, otf, tw, s);
+  case n: return finalize_v<n>(vol, cnt, prenorm, work, df3, cte, gsign, out_re, out_im, otf, tw, s);
     MTD_VOL_SIZES(X)
 #undef X
   }

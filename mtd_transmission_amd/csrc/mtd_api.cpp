@@ -346,8 +346,10 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)n * C * C * sizeof(float2));
   float2* iu = (float2*)ws_get(ctx, WS_IU, (size_t)n * C * C * sizeof(float2));
   double* coef = (double*)ws_get(ctx, WS_COEF, (size_t)n * 30 * sizeof(double));
-  if (!twC || !work || !iu || !coef) return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
-  MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, ctx->stream), "argmax"));
+  float2* part = (float2*)ws_get(ctx, WS_ARGMAX, argmax_scratch_bytes(n));
+  if (!twC || !work || !iu || !coef || !part)
+    return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
+  MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, part, ctx->stream), "argmax"));
   MTD_TRY(hip_check(ctx, launch_demod(C, crop, posx, posy, n, twC, work, iu, ctx->stream), "demodulation"));
   const int mode = field_mode(g);
   if (g->c_aber)
@@ -564,18 +566,22 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   return MTD_OK;
 }
 
+// flags: 1 = write the count-normalised, centring-signed spectrum (no counts)
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
-                        const int32_t* posspec, int A, float* vol, int32_t* counts) {
+                        const int32_t* posspec, int A, float* vol, int32_t* counts,
+                        int flags = 0) {
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
-  if (!ag) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
+  void* scratch = ws_get(ctx, WS_GATHER, gather_scratch_bytes(g->dim_final, A));
+  if (!ag || !scratch) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
   if (A > 0)
     MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
                                             t.r2tab, ag, ctx->stream), "angle geometry"));
   MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
                                        2 * g->nxmax, g->nxmax, g->dim_final, t.htab, t.beta,
-                                       t.r2max, (float2*)vol, counts, ctx->stream),
+                                       t.r2max, (float2*)vol, counts, scratch, flags,
+                                       ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
 }
@@ -614,22 +620,31 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   return scatter_impl(ctx, g, F, 0, posspec, A, vol, counts);
 }
 
+static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
+                         const int32_t* counts, int prenorm, float* index_re, float* index_im,
+                         uint8_t* otf) {
+  const int V = g->dim_final;
+  float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
+  const float2* twV = twiddle(ctx, V);
+  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory (volume work)");
+  // Reconstruction/main.cpp:346-347 (PI = M_PI there) and the delta_f^3 of
+  // TF3Dcplx_INV (src/FFT_fonctions.cpp:399)
+  const double kv = 2 * M_PI / g->lambda;
+  const double cte = -1.0 / (2 * g->n0 * kv * kv);
+  const float df3 = (float)std::pow(g->delta_f, 3);
+  const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
+  return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, prenorm, work, df3,
+                                        (float)cte, gsign, index_re, index_im, otf, twV,
+                                        ctx->stream),
+                   "finalize");
+}
+
 int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_t* counts,
                  float* index_re, float* index_im, uint8_t* otf) {
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, 0, NEED_VOL));
   if (!vol || !counts || !index_re || !index_im) return set_err(ctx, MTD_ERR_ARG, "null array");
-  const int V = g->dim_final;
-  float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
-  const float2* twV = twiddle(ctx, V);
-  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory (volume work)");
-  const double kv = 2 * M_PI / g->lambda;
-  const double cte = -1.0 / (2 * g->n0 * kv * kv);
-  const float df3 = (float)std::pow(g->delta_f, 3);
-  const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
-  return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, work, df3, (float)cte,
-                                        gsign, index_re, index_im, otf, twV, ctx->stream),
-                   "finalize");
+  return finalize_impl(ctx, g, vol, counts, 0, index_re, index_im, otf);
 }
 
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
@@ -649,16 +664,16 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, std::max(A, 1));
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
-  int32_t* cnt = (int32_t*)ws_get(ctx, WS_CNT, (size_t)V * V * V * sizeof(int32_t));
-  if (!F || !crop || !vol || !cnt) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!F || !crop || !vol) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
     MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
                         F + (size_t)a0 * C * C));
   }
-  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, cnt));
-  return mtd_finalize(ctx, g, (const float*)vol, cnt, index_re, index_im, nullptr);
+  // the gather writes the count-normalised, centring-signed spectrum
+  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1));
+  return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, index_re, index_im, nullptr);
 }
 
 }  // extern "C"

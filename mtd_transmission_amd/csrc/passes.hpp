@@ -52,7 +52,7 @@ inline dim3 rows_block() {
 }
 template <int C>
 constexpr int cols_nc() {
-  return C >= 64 ? 8 : 4;
+  return C >= 256 ? 16 : (C >= 64 ? 8 : 4);
 }
 
 }  // namespace mtd
