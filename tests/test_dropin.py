@@ -1,0 +1,123 @@
+"""The drop-in executables (tomo_pretraitement, tomo_reconstruction) driven
+exactly like the reference: a HOME with .config/gui_tomo.conf pointing at
+recon.txt / config_manip.txt, holograms i%03d.pgm and Image_mask.pgm, outputs
+in CHEMIN_RESULT in the reference's file formats."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import mtd_transmission_amd as m
+from cases import LAM, N0, NA, make_case, rel_l2
+from oracle import ref
+
+PRE = os.path.join(m.BIN_DIR, "tomo_pretraitement")
+REC = os.path.join(m.BIN_DIR, "tomo_reconstruction")
+
+
+def read_tiff_f32(path):
+    """Pages of an uncompressed little-endian float32 TIFF -> (pages, h, w)."""
+    b = open(path, "rb").read()
+    assert b[:4] == b"II*\x00"
+    off = struct.unpack_from("<I", b, 4)[0]
+    pages = []
+    while off:
+        n = struct.unpack_from("<H", b, off)[0]
+        tags = {}
+        for i in range(n):
+            tag, typ, cnt, val = struct.unpack_from("<HHII", b, off + 2 + 12 * i)
+            if typ == 3 and cnt == 1:
+                val &= 0xFFFF
+            tags[tag] = (typ, cnt, val)
+        w, h = tags[256][2], tags[257][2]
+        so = tags[273]
+        offs = [so[2]] if so[1] == 1 else list(struct.unpack_from(f"<{so[1]}I", b, so[2]))
+        rows = [np.frombuffer(b, "<f4", w * (len(offs) and h // len(offs)), o) for o in offs]
+        pages.append(np.concatenate(rows).reshape(h, w))
+        off = struct.unpack_from("<I", b, off + 2 + 12 * n)[0]
+    return np.stack(pages)
+
+
+def write_pgm16(path, img):
+    img = np.asarray(img, dtype=">u2")
+    with open(path, "wb") as f:
+        f.write(b"P5\n%d %d\n65535\n" % (img.shape[1], img.shape[0]))
+        f.write(img.tobytes())
+
+
+def write_pgm8(path, img):
+    img = np.asarray(img, dtype=np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"P5\n%d %d\n255\n" % (img.shape[1], img.shape[0]))
+        f.write(img.tobytes())
+
+
+def setup_home(tmp_path, case, drop=()):
+    home, acq, res, cfg = (tmp_path / d for d in ("home", "acquis", "result", "config"))
+    for d in (home / ".config", acq, res, cfg):
+        d.mkdir(parents=True)
+    (home / ".config" / "gui_tomo.conf").write_text(
+        f"CHEMIN_RESULT {res}\nCHEMIN_ACQUIS {acq}\nCHEMIN_CONFIG {cfg}\n"
+        f"CHEMIN_CONFIG_PC_ACQUIS {cfg}\n")
+    p = case.params
+    (cfg / "config_manip.txt").write_text(
+        f"NA {NA}\nN0 {N0}\nLAMBDA {LAM!r}\nF_TUBE 1\nF_OBJ 1\nTPCAM {p['pix']!r}\nRF 1\n"
+        f"NXMAX {p['nxmax']}\nCIRCLE_CX {p['cx']}\nCIRCLE_CY {p['cy']}\n"
+        f"NB_HOLO {len(case.acq.holos)}\n")
+    (cfg / "recon.txt").write_text(
+        f"CCD_ROIX {p['n']}\nDEROUL 0\nVOLKOV 1\nC_ABER 1\nBORN 1\nNB_THREADS 4\n"
+        f"PREMIER_ANGLE 0\nDIM_FINAL {p['v']}\nEXPORT_OTF 1\n")
+    for k, h in enumerate(case.acq.holos):
+        if k not in drop:
+            write_pgm16(acq / ("i%03d.pgm" % k), h)
+    if case.acq.mask is not None:
+        write_pgm8(acq / "Image_mask.pgm", case.acq.mask)
+    env = dict(os.environ, HOME=str(home))
+    return env, res
+
+
+def test_help_and_missing_inputs(tmp_path):
+    for exe in (PRE, REC):
+        r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and "-polar" in r.stdout
+    case = make_case("small")
+    env, res = setup_home(tmp_path, case)
+    r = subprocess.run([REC], env=env, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "parametres.raw" in r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_programs_round_trip(tmp_path):
+    case = make_case("small")
+    drop = (3,)  # a missing hologram is skipped and the angles compacted
+    print("drop-in round trip case=small", flush=True)
+    env, res = setup_home(tmp_path, case, drop)
+    r = subprocess.run([PRE], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    keep = [k for k in range(len(case.acq.holos)) if k not in drop]
+    holos = case.acq.holos[keep]
+    A, C, V = len(keep), case.g.C, case.g.dim_final
+    dim = f"{C}x{C}x{A}"
+    fields32, pos32 = ref.preprocess(holos, case.g, case.acq.mask, "f32")
+    param = np.fromfile(res / "parametres.raw", "<f8")
+    assert list(param[:4]) == [case.g.nxmax, A, case.g.rayon, case.g.n]
+    assert param[4] == case.g.pix_holo
+    assert np.array_equal(np.fromfile(res / "tab_posSpec.raw", "<f8"), pos32.astype(float))
+    fre = np.fromfile(res / f"UBornfinal_Re{dim}.raw", "<f8")
+    fim = np.fromfile(res / f"UBornfinal_Im{dim}.raw", "<f8")
+    got = (fre + 1j * fim).reshape(A, C, C)
+    assert rel_l2(got, fields32) < 1e-5
+    centres = read_tiff_f32(res / "centres.tif")[0]
+    assert centres[pos32[A - 1], pos32[2 * A - 1]] == A - 1
+
+    r = subprocess.run([REC], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the executable reads the f64 fields back; run the oracle on the same
+    idx32, _, cnt32, otf32 = ref.reconstruct(got.astype(np.complex64), pos32, case.g, "f32")
+    ire = read_tiff_f32(res / "indice.tif")
+    iim = read_tiff_f32(res / "absorption.tif")
+    assert ire.shape == (V, V, V)
+    assert rel_l2(ire + 1j * iim, idx32) < 1e-5
+    assert np.array_equal(read_tiff_f32(res / "OTF3D.tif").astype(bool), otf32)
