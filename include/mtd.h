@@ -11,7 +11,8 @@
  *
  * Conventions
  *  - Every call returns 0 (MTD_OK) or an error code; mtd_last_error() gives
- *    the message of the last failure on that context.
+ *    the message of the last failure on that context (mtd_last_error(NULL):
+ *    of the last failed mtd_ctx_create on the calling thread).
  *  - Array arguments are DEVICE pointers (hipMalloc / torch CUDA tensors)
  *    unless the comment says "host". The caller owns them; the context owns
  *    its scratch workspaces, twiddle tables and fit plans.
@@ -36,12 +37,19 @@ extern "C" {
 enum mtd_status {
   MTD_OK = 0,
   MTD_ERR_ARG = 1,          /* invalid argument or geometry            */
-  MTD_ERR_UNSUPPORTED = 2,  /* size/mode not built (e.g. DEROUL 1)     */
+  MTD_ERR_UNSUPPORTED = 2,  /* size/mode not built (e.g. Herraez)      */
   MTD_ERR_HIP = 3,          /* HIP runtime error                       */
   MTD_ERR_IO = 4            /* file could not be read or written       */
 };
 
 enum mtd_dtype { MTD_U8 = 0, MTD_U16 = 1, MTD_F32 = 2 };
+
+/* Phase unwrapping (pretraitement_hors_axe_CPU/main.cpp:260-277; only applied
+ * with C_ABER 1): DEROUL 0 -> OFF; DEROUL 1 with VOLKOV 1 -> VOLKOV
+ * (deroul_volkov4_total_sym_paire, src/deroulement_volkov4.cpp:14-92);
+ * DEROUL 1 with VOLKOV 0 -> HERRAEZ (phaseUnwrapping_Mat, not built:
+ * MTD_ERR_UNSUPPORTED). */
+enum mtd_deroul { MTD_DEROUL_OFF = 0, MTD_DEROUL_VOLKOV = 1, MTD_DEROUL_HERRAEZ = 2 };
 
 typedef struct mtd_ctx mtd_ctx;
 
@@ -56,7 +64,7 @@ typedef struct mtd_geom {
   int32_t circle_cy;   /* CIRCLE_CY                                        */
   int32_t born;        /* BORN 1: Born field, 0: Rytov                     */
   int32_t c_aber;      /* C_ABER: polynomial aberration correction         */
-  int32_t deroul;      /* DEROUL: phase unwrapping (only 0 is supported)   */
+  int32_t deroul;      /* enum mtd_deroul: DEROUL / VOLKOV of recon.txt     */
   double n0;           /* N0, float-rounded                                */
   double na;           /* NA, float-rounded                                */
   double lambda;       /* LAMBDA (m), float-rounded                        */

@@ -85,6 +85,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise MTDError(f"{LIB_PATH} is missing: build it first "
                            "(python -c 'import __graft_entry__ as g; g.build()')")
+        # torch's ROCm wheel carries its own libamdhip64 and asks for it as
+        # "libamdhip64.so", while libmtd.so asks for the soname
+        # "libamdhip64.so.7". Loading libmtd.so first would bring the system
+        # runtime in and torch would then load a second HIP/HSA runtime
+        # beside it (hipSetDevice then fails in one of them). Importing torch
+        # first makes libmtd.so bind to the runtime torch already holds.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -181,7 +188,8 @@ class Context:
         torch.cuda.set_device(device)
         rc = lib().mtd_ctx_create(device, ctypes.byref(h))
         if rc:
-            raise MTDError(f"mtd_ctx_create failed ({rc})")
+            msg = lib().mtd_last_error(None).decode()
+            raise MTDError(f"mtd_ctx_create failed: {msg} (code {rc})")
         self.h = h
 
     def close(self):

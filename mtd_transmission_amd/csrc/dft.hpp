@@ -1,17 +1,12 @@
-// Batched 1-D complex FFTs staged through LDS, for gfx950.
-//
-// One transform of length L is run by T threads of a workgroup; a workgroup
-// runs NT transforms side by side (NT*T threads). Data lives in LDS between
-// radix stages (Stockham autosort: natural order in, natural order out); each
-// stage reads its butterfly inputs into registers, applies twiddles, does a
-// radix-R DFT in registers and writes the outputs back.
+// Complex helpers and small in-register DFT kernels (radix 2, 3, 4, 5, 7, 8,
+// 11) used by the register FFTs of fft_reg.hpp.
 //
 // Sign convention (FFTW): forward  X[k] = sum x[n] exp(-2 pi i n k / L),
 //                         backward X[k] = sum x[n] exp(+2 pi i n k / L),
 // both unnormalised.
 //
-// Twiddles come from a table tw[k] = exp(-2 pi i k / L), k < L, computed on
-// the host in double precision and rounded once to float.
+// Twiddle tables tw[k] = exp(-2 pi i k / L), k < L, are computed on the host
+// in double precision and rounded once to float.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -170,87 +165,6 @@ template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
 template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
 template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
 
-// ---------------------------------------------------------------------------
-// Factorisation of L into radices, chosen at compile time.
-// ---------------------------------------------------------------------------
-__host__ __device__ constexpr int pick_radix(int rem) {
-  return (rem % 8 == 0 && rem != 16) ? 8
-       : (rem % 4 == 0) ? 4
-       : (rem % 2 == 0) ? 2
-       : (rem % 3 == 0) ? 3
-       : (rem % 5 == 0) ? 5
-       : (rem % 7 == 0) ? 7
-       : (rem % 11 == 0) ? 11 : 0;
-}
-
-// One Stockham stage: M = product of the radices already applied.
-template <int L, int T, int R, int M, bool INV>
-__device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __restrict__ tw) {
-  constexpr int NB = L / R;
-  constexpr int PER = (NB + T - 1) / T;
-  constexpr bool EVEN = (PER * T == NB);
-  constexpr int TWS = L / (M * R);
-  float2 v[PER][R];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int j = t + u * T;
-    if (EVEN || j < NB) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) v[u][q] = buf[j + q * NB];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int j = t + u * T;
-    if (EVEN || j < NB) {
-      const int k = j % M;
-      if (M > 1) {
-#pragma unroll
-        for (int q = 1; q < R; ++q) {
-          float2 w = tw[q * k * TWS];
-          if (INV) w.y = -w.y;
-          v[u][q] = cmul(v[u][q], w);
-        }
-      }
-      Dft<R, INV>::run(v[u]);
-      const int base = (j - k) * R + k;
-#pragma unroll
-      for (int p = 0; p < R; ++p) buf[base + p * M] = v[u][p];
-    }
-  }
-  __syncthreads();
-}
-
-template <int L, int T, int REM, int M, bool INV>
-struct FftStages {
-  __device__ __forceinline__ static void run(float2* buf, int t, const float2* __restrict__ tw) {
-    constexpr int R = pick_radix(REM);
-    static_assert(R != 0, "unsupported FFT length factor");
-    fft_stage<L, T, R, M, INV>(buf, t, tw);
-    FftStages<L, T, REM / R, M * R, INV>::run(buf, t, tw);
-  }
-};
-template <int L, int T, int M, bool INV>
-struct FftStages<L, T, 1, M, INV> {
-  __device__ __forceinline__ static void run(float2*, int, const float2* __restrict__) {}
-};
-
-// In-place FFT of buf[0..L) (LDS) by threads t = 0..T-1 of this transform.
-// Every thread of the workgroup must call it (it contains barriers).
-template <int L, int T, bool INV>
-__device__ __forceinline__ void fft_lds(float2* buf, int t, const float2* __restrict__ tw) {
-  FftStages<L, T, L, 1, INV>::run(buf, t, tw);
-}
-
-// Threads per transform used for a length-L pass (about 8-16 elements per
-// thread), and the number of transforms a 256-thread workgroup runs.
-__host__ __device__ constexpr int fft_threads(int L) {
-  return L >= 4096 ? 256 : (L / 16 >= 1 ? L / 16 : 1);
-}
-__host__ __device__ constexpr int fft_per_block(int L) {
-  return (256 / fft_threads(L)) >= 1 ? 256 / fft_threads(L) : 1;
-}
 // Largest d <= cap dividing n (at least 1).
 __host__ __device__ constexpr int divisor_le(int n, int cap) {
   for (int d = cap; d > 1; --d)

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mtd.h"
@@ -23,14 +24,14 @@ struct AngleGeo {
 };
 
 // Angle-independent least-squares plan (Correction_aberration.cpp:108-217):
-// sample list, LU factors of the normal matrix A^T A with row pivots.
+// sample list, LU factors of the normal matrix A^T A with its row permutation.
 struct LsqPlan {
   int M = 0;
   int C = 0;
   std::vector<uint8_t> mask;  // host copy used to detect changes
   int2* d_samples = nullptr;  // (x, y) field-frame coordinates
   double* d_lu = nullptr;     // 15x15 row-major, L unit-lower + U
-  int* d_piv = nullptr;       // 15 sequential row swaps
+  int* d_perm = nullptr;      // row permutation of the pivoting: (Pb)[i] = b[perm[i]]
 };
 
 struct Buffer {
@@ -55,9 +56,9 @@ struct mtd_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::map<int, float2*> twiddles;                       // per FFT length
-  std::map<std::pair<int, long long>, double*> tukey;     // (n, alpha*1e9)
-  std::map<std::pair<long long, long long>, double*> htab; // heights per (R, r2max)
-  std::map<long long, float*> betatab;                    // per geometry hash
+  std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)
+  std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
+  std::map<std::tuple<uint64_t, int, uint32_t>, float*> betatab;  // (R, r2max, k0) bits
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   mtd::LsqPlan lsq;
   // per-kernel event timing (mtd_ctx_set_timing)
@@ -98,7 +99,7 @@ struct Timed {
 // scratch slots
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
-  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX
+  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -125,19 +126,27 @@ size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
                         const int* posy, int A, const float2* tw,
                         float2* work, float2* iu, hipStream_t s);
-hipError_t launch_lsq(int C, const float2* iu, int A, const LsqPlan& plan,
+// phu: unwrapped phase [A][C][C] in the U frame, or nullptr for the wrapped
+// atan2 of IU (DEROUL 0)
+hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,
                       double* coef, hipStream_t s);
 // mode: bit0 born, bit1 c_aber
 hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
                              const int* posx, const int* posy, int A,
-                             const double* coef, int mode, float2* fields,
+                             const double* coef, int mode, const float* phu, float2* fields,
                              hipStream_t s);
+// Volkov unwrapping of the wrapped phase of IU into phu; scratch buffers:
+// gx, gy [A][2C][2C] float, P [A][2C][2C] float2, Q [A][2C][C] float2;
+// tw2C: twiddles of length 2C.
+hipError_t launch_unwrap(int C, const float2* iu, int A, const float2* tw2C, float* gx, float* gy,
+                         float2* P, float2* Q, float* phu, hipStream_t s);
+size_t unwrap_scratch_bytes(int C, int A);
 // Per-angle forward spectrum. from_iu=0: src are fields (U frame);
 // from_iu=1: src is IU and the field formation runs inside the load.
 // Output frame: shift_pos=0 -> unshifted F, 1 -> shifted by +posSpec.
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
-                               int mode, const int* posx, const int* posy,
+                               int mode, const float* phu, const int* posx, const int* posy,
                                int A, const double* win, const float2* tw,
                                float dx2, int shift_pos, float2* work,
                                float2* out, hipStream_t s);
@@ -145,15 +154,23 @@ hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
 hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
                             double R2, const double* htab, int r2tab,
                             AngleGeo* ageo, hipStream_t s);
+// Height band [band.x, band.y] that holds every hit of the non-skipped angles
+// (empty: x > y), from the pupil's extreme radii.
+hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, int r2max,
+                       int2* band, hipStream_t s);
+// band == nullptr: every voxel of the volume (and counts) is written;
+// otherwise only the height windows that meet the band are.
 hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, void* scratch, int flags, hipStream_t s);
+                         int32_t* cnt, void* scratch, int flags, const int2* band,
+                         hipStream_t s);
 size_t gather_scratch_bytes(int V, int A);
-// prenorm=1: vol already divided by counts and centring-signed (gather flag 1)
+// prenorm=1: vol already divided by counts and centring-signed (gather flag 1).
+// band != nullptr: heights outside the band are zero and are not read.
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
-                           int prenorm, float2* work, float df3, float cte,
+                           int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
                            uint8_t* otf, const float2* tw, hipStream_t s);
 

@@ -11,7 +11,7 @@
 //   form       main.cpp:289-316 (Born / Rytov) or main.cpp:332-337 (C_ABER 0).
 //   spectrum   Reconstruction/main.cpp:207-217: window, shift by NXMAX,
 //              forward 2-D FFT * pixUBorn^2, optional shift by +posSpec.
-#include "fft_lds.hpp"
+#include "dft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
@@ -29,6 +29,9 @@ __device__ __forceinline__ bool better(float v2, int i2, float v1, int i1) {
 template <int C>
 __global__ void __launch_bounds__(256)
 k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
+  // slot 0's |v|^2 is computed unfused, as the reference does (plain
+  // operators: the pragma is lexical)
+#pragma clang fp contract(off)
   const int a = blockIdx.y, seg = blockIdx.x;
   constexpr int LEN = (C * C + ARGMAX_SEG - 1) / ARGMAX_SEG;
   const float2* p = crop + (size_t)a * C * C;
@@ -37,8 +40,7 @@ k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
   const int i0 = seg * LEN, i1 = min(i0 + LEN, C * C);
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     const float2 v = p[i];
-    const float m = (i == 0) ? __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y))
-                             : hypotf(v.x, v.y);
+    const float m = (i == 0) ? v.x * v.x + v.y * v.y : hypotf(v.x, v.y);
     if (m > best) { best = m; bi = i; }
   }
   __shared__ float sv[256];
@@ -129,20 +131,20 @@ __device__ __forceinline__ void monomials(double X, double Y, double* mono) {
     for (int j = 0; j <= 4 - i; ++j) mono[k++] = xp[i] * yp[j];
 }
 
-__device__ __forceinline__ void lu_solve15(const double* __restrict__ lu,
-                                           const int* __restrict__ piv,
-                                           double* b) {
-  for (int k = 0; k < NCOEF; ++k) {
-    const int p = piv[k];
-    if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
-  }
+// Solves (LU) x = P b in place; b holds P b on entry. Fully unrolled so that
+// b stays in registers; lu is block-uniform (scalar loads).
+__device__ __forceinline__ void lu_solve15(const double* __restrict__ lu, double (&b)[NCOEF]) {
+#pragma unroll
   for (int i = 1; i < NCOEF; ++i) {
     double s = b[i];
+#pragma unroll
     for (int j = 0; j < i; ++j) s -= lu[i * NCOEF + j] * b[j];
     b[i] = s;
   }
+#pragma unroll
   for (int i = NCOEF - 1; i >= 0; --i) {
     double s = b[i];
+#pragma unroll
     for (int j = i + 1; j < NCOEF; ++j) s -= lu[i * NCOEF + j] * b[j];
     b[i] = s / lu[i * NCOEF + i];
   }
@@ -150,9 +152,9 @@ __device__ __forceinline__ void lu_solve15(const double* __restrict__ lu,
 
 template <int C>
 __global__ void __launch_bounds__(256)
-k_lsq(const float2* __restrict__ iu, const int2* __restrict__ samples, int M,
-      const double* __restrict__ lu, const int* __restrict__ piv,
-      double* __restrict__ coef) {
+k_lsq(const float2* __restrict__ iu, const float* __restrict__ phu,
+      const int2* __restrict__ samples, int M, const double* __restrict__ lu,
+      const int* __restrict__ perm, double* __restrict__ coef) {
   const int a = blockIdx.x;
   const float2* u = iu + (size_t)a * C * C;
   double rp[NCOEF], ra[NCOEF];
@@ -161,7 +163,8 @@ k_lsq(const float2* __restrict__ iu, const int2* __restrict__ samples, int M,
   for (int m = threadIdx.x; m < M; m += 256) {
     const int2 s = samples[m];
     const float2 v = u[((s.y + C / 2) % C) * C + (s.x + C / 2) % C];
-    const double ph = (double)atan2f(v.y, v.x);
+    // phase: wrapped (atan2) or, with unwrapping, the unwrapped U-frame phase
+    const double ph = phu ? (double)phu[((size_t)a * C + s.y) * C + s.x] : (double)atan2f(v.y, v.x);
     const double am = (double)hypotf(v.x, v.y);
     double mono[NCOEF];
     monomials((double)s.x, (double)s.y, mono);
@@ -173,6 +176,7 @@ k_lsq(const float2* __restrict__ iu, const int2* __restrict__ samples, int M,
   }
   // deterministic reduction: butterfly within the wave, then waves in order
   __shared__ double red[4][2 * NCOEF];
+  __shared__ double rhs[2 * NCOEF];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NCOEF; ++k) {
@@ -184,15 +188,19 @@ k_lsq(const float2* __restrict__ iu, const int2* __restrict__ samples, int M,
     if (lane == 0) { red[wave][k] = x; red[wave][NCOEF + k] = y; }
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 2 * NCOEF) {
+    const int idx = threadIdx.x;
+    rhs[idx] = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {  // thread 0: phase fit, thread 1: amplitude fit
+    const double* r = rhs + threadIdx.x * NCOEF;
     double b[NCOEF];
-    for (int k = 0; k < NCOEF; ++k) {
-      const int idx = threadIdx.x * NCOEF + k;
-      b[k] = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
-    }
-    lu_solve15(lu, piv, b);
-    for (int k = 0; k < NCOEF; ++k)
-      coef[((size_t)a * 2 + threadIdx.x) * NCOEF + k] = b[k];
+#pragma unroll
+    for (int k = 0; k < NCOEF; ++k) b[k] = r[perm[k]];
+    lu_solve15(lu, b);
+#pragma unroll
+    for (int k = 0; k < NCOEF; ++k) coef[((size_t)a * 2 + threadIdx.x) * NCOEF + k] = b[k];
   }
 }
 
@@ -204,15 +212,17 @@ struct FieldMode {
   int c_aber;
 };
 
+// phu_a: unwrapped phase of this angle (U frame, row length C) or nullptr
 __device__ __forceinline__ float2 form_pixel(float2 u, int X, int Y,
                                              const double* __restrict__ cph,
                                              const double* __restrict__ cam,
-                                             float2 spec_norm, int mode) {
+                                             float2 spec_norm, int mode,
+                                             const float* __restrict__ phu_a, int C) {
   if (!(mode & 2)) {  // C_ABER 0: U * conj(s)/|s|
     return make_float2(u.x * spec_norm.x - u.y * spec_norm.y,
                        u.x * spec_norm.y + u.y * spec_norm.x);
   }
-  const float phf = atan2f(u.y, u.x);
+  const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
   const float amf = hypotf(u.x, u.y);
   double mono[NCOEF];
   monomials((double)X, (double)Y, mono);
@@ -245,7 +255,7 @@ template <int C>
 __global__ void __launch_bounds__(256)
 k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
              const int* __restrict__ posx, const int* __restrict__ posy,
-             const double* __restrict__ coef, int mode,
+             const double* __restrict__ coef, int mode, const float* __restrict__ phu,
              float2* __restrict__ fields) {
   const int a = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -255,7 +265,8 @@ k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
   const float2 sn = (mode & 2) ? make_float2(0.f, 0.f)
                                : spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
   const float2 u = iu[((size_t)a * C + (Y + C / 2) % C) * C + (X + C / 2) % C];
-  fields[(size_t)a * C * C + i] = form_pixel(u, X, Y, cph, cph + NCOEF, sn, mode);
+  const float* phu_a = phu ? phu + (size_t)a * C * C : nullptr;
+  fields[(size_t)a * C * C + i] = form_pixel(u, X, Y, cph, cph + NCOEF, sn, mode, phu_a, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,10 +284,13 @@ struct FwdRowOp {
   float2* out;
   const float2* tw;
   int from_iu, mode;
+  const float* phu;  // unwrapped phase (U frame) or nullptr
   float2 sn;
   const double* cph;
+  const float* phu_a;
   __device__ void begin(int a) {
     cph = coef + (size_t)a * 2 * NCOEF;
+    phu_a = phu ? phu + (size_t)a * C * C : nullptr;
     if (from_iu && !(mode & 2))
       sn = spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
   }
@@ -285,7 +299,8 @@ struct FwdRowOp {
     const float w = (float)(win[X] * win[Y]);
     float2 f;
     if (from_iu) {
-      f = form_pixel(src[((size_t)a * C + y) * C + x], X, Y, cph, cph + NCOEF, sn, mode);
+      f = form_pixel(src[((size_t)a * C + y) * C + x], X, Y, cph, cph + NCOEF, sn, mode, phu_a,
+                     C);
     } else {
       f = src[((size_t)a * C + Y) * C + X];
     }
@@ -346,50 +361,48 @@ static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
                        rows_block<C>(), 0, s, r);
   }
   ColInPlaceOp<C> c{work, iu, tw};
-  constexpr int NC = cols_nc<C>();
   Timed tm("demod_cols", s);
-  hipLaunchKernelGGL((k_cols<C, true, NC, ColInPlaceOp<C>>), dim3(C / NC, A),
-                     dim3(256), 0, s, c);
+  hipLaunchKernelGGL((k_cols<C, true, ColInPlaceOp<C>>), cols_grid<C>(A), cols_block<C>(), 0, s,
+                     c);
   return hipGetLastError();
 }
 
 template <int C>
-static hipError_t lsq_c(const float2* iu, int A, const LsqPlan& plan,
+static hipError_t lsq_c(const float2* iu, const float* phu, int A, const LsqPlan& plan,
                         double* coef, hipStream_t s) {
   Timed tm("lsq", s);
-  hipLaunchKernelGGL((k_lsq<C>), dim3(A), dim3(256), 0, s, iu, plan.d_samples,
-                     plan.M, plan.d_lu, plan.d_piv, coef);
+  hipLaunchKernelGGL((k_lsq<C>), dim3(A), dim3(256), 0, s, iu, phu, plan.d_samples,
+                     plan.M, plan.d_lu, plan.d_perm, coef);
   return hipGetLastError();
 }
 
 template <int C>
 static hipError_t form_c(const float2* iu, const float2* crop, const int* posx,
                          const int* posy, int A, const double* coef, int mode,
-                         float2* fields, hipStream_t s) {
+                         const float* phu, float2* fields, hipStream_t s) {
   Timed tm("form_field", s);
   hipLaunchKernelGGL((k_form_field<C>), dim3((C * C + 255) / 256, A), dim3(256),
-                     0, s, iu, crop, posx, posy, coef, mode, fields);
+                     0, s, iu, crop, posx, posy, coef, mode, phu, fields);
   return hipGetLastError();
 }
 
 template <int C>
 static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
-                        const double* coef, int mode, const int* posx,
+                        const double* coef, int mode, const float* phu, const int* posx,
                         const int* posy, int A, const double* win,
                         const float2* tw, float dx2, int shift_pos,
                         float2* work, float2* out, hipStream_t s) {
-  FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode,
-                make_float2(0.f, 0.f), nullptr};
+  FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode, phu,
+                make_float2(0.f, 0.f), nullptr, nullptr};
   {
     Timed tm(from_iu ? "fwd_rows_field" : "fwd_rows", s);
     hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
                        rows_block<C>(), 0, s, r);
   }
   FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, 0, 0};
-  constexpr int NC = cols_nc<C>();
   Timed tm("fwd_cols", s);
-  hipLaunchKernelGGL((k_cols<C, false, NC, FwdColOp<C>>), dim3(C / NC, A),
-                     dim3(256), 0, s, c);
+  hipLaunchKernelGGL((k_cols<C, false, FwdColOp<C>>), cols_grid<C>(A), cols_block<C>(), 0, s,
+                     c);
   return hipGetLastError();
 }
 
@@ -417,10 +430,10 @@ hipError_t launch_demod(int C, const float2* crop, const int* posx,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_lsq(int C, const float2* iu, int A, const LsqPlan& plan,
+hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,
                       double* coef, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return lsq_c<n>(iu, A, plan, coef, s);
+#define X(n) case n: return lsq_c<n>(iu, phu, A, plan, coef, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
@@ -429,10 +442,10 @@ hipError_t launch_lsq(int C, const float2* iu, int A, const LsqPlan& plan,
 
 hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
                              const int* posx, const int* posy, int A,
-                             const double* coef, int mode, float2* fields,
+                             const double* coef, int mode, const float* phu, float2* fields,
                              hipStream_t s) {
   switch (C) {
-#define X(n) case n: return form_c<n>(iu, crop, posx, posy, A, coef, mode, fields, s);
+#define X(n) case n: return form_c<n>(iu, crop, posx, posy, A, coef, mode, phu, fields, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
@@ -441,14 +454,14 @@ hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
 
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
-                               int mode, const int* posx, const int* posy,
+                               int mode, const float* phu, const int* posx, const int* posy,
                                int A, const double* win, const float2* tw,
                                float dx2, int shift_pos, float2* work,
                                float2* out, hipStream_t s) {
   switch (C) {
-#define X(n)                                                                    \
-  case n:                                                                       \
-    return fwd_c<n>(src, from_iu, crop, coef, mode, posx, posy, A, win, tw, dx2, \
+#define X(n)                                                                         \
+  case n:                                                                            \
+    return fwd_c<n>(src, from_iu, crop, coef, mode, phu, posx, posy, A, win, tw, dx2, \
                     shift_pos, work, out, s);
     MTD_CROP_SIZES(X)
 #undef X

@@ -6,44 +6,63 @@
 // (-1)^(kx+ky) * FFT(h * (-1)^(x+y))[ky][kx] / N^2 for even N, so both shifts
 // become sign flips applied on load and on store.
 //
-// Pass 1 (rows): two real rows are packed into one complex FFT of length N;
-// only the C bins kx in [x0, x0+C) are separated and written, to
-// t1[a][y][d] (N x C per hologram).
+// Pass 1 (rows): two real rows are packed into one complex FFT of length N
+// (two-step register FFT); only the C bins kx in [x0, x0+C) are separated and
+// written, to t1[a][y][d] (N x C per hologram).
 // Pass 2 (columns): each of the C columns of t1 is transformed along y and
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
-#include "fft_lds.hpp"
+#include "fft_reg.hpp"
 #include "internal.hpp"
 
 namespace mtd {
+
+// LDS per block kept near 36 KiB so that four blocks fit a CU
+constexpr int kHoloLdsBudget = 36864;
+
+template <int N>
+constexpr int holo_pairs() {  // row pairs per block
+  constexpr int by_lds = kHoloLdsBudget / (Fft2<N, false>::LDS * 8);
+  constexpr int cap = 256 / Fft2<N, false>::T;
+  return divisor_le(N / 2, by_lds < cap ? (by_lds > 0 ? by_lds : 1) : cap);
+}
 
 template <int N, typename TIn>
 __global__ void __launch_bounds__(256)
 k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
             const float2* __restrict__ tw, int x0, int C,
             float2* __restrict__ t1) {
-  constexpr int T = fft_threads(N);
-  constexpr int NP = divisor_le(N / 2, fft_per_block(N));  // row pairs per block
-  __shared__ float2 lds[NP][N];
+  using F = Fft2<N, false>;
+  constexpr int NP = holo_pairs<N>();
+  static_assert(F::LDS >= N, "transpose buffer must hold a full row");
+  __shared__ float2 lds[NP * F::LDS];
   const int a = blockIdx.y;
-  const int p = threadIdx.x / T, t = threadIdx.x % T;
+  const int p = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
   const int y0 = 2 * (blockIdx.x * NP + p), y1 = y0 + 1;
   const TIn* h0 = holos + ((size_t)a * N + y0) * N;
   const TIn* h1 = h0 + N;
   const double wy0 = win[y0], wy1 = win[y1];
-  float2* buf = lds[p];
-  for (int x = t; x < N; x += T) {
+  float2 v[F::L1], u[F::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F::L1; ++n1) {
+    const int x = F::L2 * n1 + n2;
     const double wx = win[x];
     const float w0 = (float)(wx * wy0), w1 = (float)(wx * wy1);
     float g0 = (float)h0[x] * w0;
     float g1 = (float)h1[x] * w1;
-    if ((x + y0) & 1) g0 = -g0; else g1 = -g1;
-    buf[x] = make_float2(g0, g1);
+    if ((x + y0) & 1) g0 = -g0; else g1 = -g1;  // centring (-1)^(x+y)
+    v[n1] = make_float2(g0, g1);
+  }
+  float2* buf = lds + p * F::LDS;
+  F::run(v, u, n2, buf, tw);
+  __syncthreads();  // every thread has read its transposed inputs
+  if (n2 < F::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < F::L2; ++k2) buf[n2 + F::L1 * k2] = u[k2];
   }
   __syncthreads();
-  fft_lds<N, T, false>(buf, t, tw);
   float2* o0 = t1 + ((size_t)a * N + y0) * C;
   float2* o1 = o0 + C;
-  for (int d = t; d < C; d += T) {
+  for (int d = n2; d < C; d += F::T) {
     const int k = x0 + d;
     const int m = (N - k) % N;
     const float2 zk = buf[k], zm = buf[m];
@@ -54,28 +73,39 @@ k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
   }
 }
 
-template <int N, int NC>
+template <int N>
+constexpr int holo_cols_nc() {  // a power of two <= 16: divides every crop side
+  constexpr int by_thr = 256 / Fft2<N, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS * 8);
+  return divisor_le(16, by_thr < by_lds ? by_thr : by_lds);
+}
+
+template <int N>
 __global__ void __launch_bounds__(256)
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
             int C, int x0, int y0, float scale, float2* __restrict__ crop) {
-  constexpr int T = 256 / NC;
-  __shared__ float2 lds[NC][N + 1];
+  using F = Fft2<N, false>;
+  constexpr int NC = holo_cols_nc<N>();
+  __shared__ float2 lds[NC * F::LDS];
   const int a = blockIdx.y, d0 = blockIdx.x * NC;
-  const float2* src = t1 + (size_t)a * N * C;
-  for (int e = threadIdx.x; e < N * NC; e += 256) {
-    const int y = e / NC, c = e % NC;
-    lds[c][y] = src[(size_t)y * C + d0 + c];
-  }
-  __syncthreads();
-  const int c = threadIdx.x / T, t = threadIdx.x % T;
-  fft_lds<N, T, false>(lds[c], t, tw);
-  float2* dst = crop + (size_t)a * C * C;
-  for (int f = threadIdx.x; f < C * NC; f += 256) {
-    const int e = f / NC, cc = f % NC;
-    const int ky = y0 + e, kx = x0 + d0 + cc;
-    const float2 v = lds[cc][ky];
-    const float s = ((kx + ky) & 1) ? -scale : scale;
-    dst[(size_t)e * C + d0 + cc] = make_float2(v.x * s, v.y * s);
+  const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
+  const float2* src = t1 + (size_t)a * N * C + d0 + c;
+  float2 v[F::L1], u[F::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = src[(size_t)(F::L2 * n1 + n2) * C];
+  F::run(v, u, n2, lds + c * F::LDS, tw);
+  if (n2 < F::L1) {
+    float2* dst = crop + (size_t)a * C * C + d0 + c;
+    const int kx = x0 + d0 + c;
+#pragma unroll
+    for (int k2 = 0; k2 < F::L2; ++k2) {
+      const int ky = n2 + F::L1 * k2;
+      const int e = ky - y0;
+      if (e >= 0 && e < C) {
+        const float s = ((kx + ky) & 1) ? -scale : scale;
+        dst[(size_t)e * C] = make_float2(u[k2].x * s, u[k2].y * s);
+      }
+    }
   }
 }
 
@@ -83,8 +113,8 @@ template <int N>
 static hipError_t rows_n(int dtype, const void* holos, int A, const double* win,
                          const float2* tw, int x0, int C, float2* t1,
                          hipStream_t s) {
-  constexpr int NP = divisor_le(N / 2, fft_per_block(N));
-  dim3 grid(N / (2 * NP), A), block(NP * fft_threads(N));
+  constexpr int NP = holo_pairs<N>();
+  dim3 grid(N / (2 * NP), A), block(NP * Fft2<N, false>::T);
   Timed tm("holo_rows", s);
   switch (dtype) {
     case MTD_U8:
@@ -106,10 +136,11 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const double* win,
 template <int N>
 static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
                          int x0, int y0, float2* crop, hipStream_t s) {
-  constexpr int NC = N >= 2048 ? 4 : 8;
+  constexpr int NC = holo_cols_nc<N>();
+  if (C % NC) return hipErrorInvalidValue;
   const float scale = (float)(0.5 / ((double)N * (double)N));
   Timed tm("holo_cols", s);
-  hipLaunchKernelGGL((k_holo_cols<N, NC>), dim3(C / NC, A), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(NC * Fft2<N, false>::T), 0, s,
                      t1, tw, C, x0, y0, scale, crop);
   return hipGetLastError();
 }

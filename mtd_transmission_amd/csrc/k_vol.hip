@@ -13,11 +13,15 @@
 // and the sum is deterministic and equal to the sequential FP32 sum. Partial
 // columns live in LDS; every voxel of the V^3 spectrum and count volume is
 // written exactly once (zeros included), so no clearing pass is needed.
+// Banded mode (the fused plain method): hits of all angles lie in a height
+// band about V/2 of half-width about R - sqrt(R^2 - NXMAX^2) (a third of the
+// volume for the usual geometry); only the windows meeting it are written and
+// the inverse transform reads nothing else.
 //
 // Finalisation (Reconstruction/main.cpp:243-352): divide by counts (0 -> 1),
 // backward 3-D FFT with both fftshift3D folded into (-1)^(x+y+z) sign flips,
 // times delta_f^3 and -1/(2 n0 kv^2), split into index and absorption.
-#include "fft_lds.hpp"
+#include "dft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
@@ -60,29 +64,6 @@ struct GatherArgs {
 
 constexpr int GT = 16;  // tile side (columns per block = 256)
 
-// One hit test; returns true and fills z/val when angle a hits this column.
-__device__ __forceinline__ bool gather_hit(const GatherArgs& g, const AngleGeo& ag,
-                                           int a, int qx, int qy, int& z,
-                                           float2& val) {
-  const int fdx = qx + ag.fx, fdy = qy + ag.fy;
-  if (fdx < -g.nx || fdx >= g.nx || fdy < -g.nx || fdy >= g.nx) return false;
-  const int r2 = fdx * fdx + fdy * fdy;
-  if (r2 > g.r2max) return false;
-  z = (int)round_away(g.htab[r2] - ag.fz) + g.V / 2;
-  int row, col;
-  if (g.frame_pos) {
-    row = fdy + g.nx;
-    col = fdx + g.nx;
-  } else {
-    row = ((qy % g.C) + g.C) % g.C;
-    col = ((qx % g.C) + g.C) % g.C;
-  }
-  const float2 s = g.spec[((size_t)a * g.C + row) * g.C + col];
-  const float b = g.beta[r2];
-  val = make_float2(-__fmul_rn(b, s.y), __fmul_rn(b, s.x));
-  return true;
-}
-
 // ---------------------------------------------------------------------------
 // Windowed gather. A prepass lists, per 16x16 column tile and in angle order,
 // the angles whose pupil disc touches the tile, with a conservative height
@@ -99,8 +80,21 @@ struct TileEntry {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Height of angle a's hit in column q, as an offset from the entry's lower
+// bound zl (255 = no hit). Exact: FP64 heights and round-half-away.
+__device__ __forceinline__ uint8_t hit_offset(const GatherArgs& g, const AngleGeo& ag, int qx,
+                                              int qy, int zl) {
+  const int fdx = qx + ag.fx, fdy = qy + ag.fy;
+  if (fdx < -g.nx || fdx >= g.nx || fdy < -g.nx || fdy >= g.nx) return 255;
+  const int r2 = fdx * fdx + fdy * fdy;
+  if (r2 > g.r2max) return 255;
+  const int z = (int)round_away(g.htab[r2] - ag.fz) + g.V / 2;
+  return (uint8_t)(z - zl);
+}
+
 __global__ void __launch_bounds__(256)
-k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo) {
+k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo,
+              uint8_t* __restrict__ zoff) {
   __shared__ int wsum[4];
   __shared__ int zr[2][4];
   const int tile = blockIdx.x;
@@ -156,6 +150,16 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
     const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
     tinfo[tile] = make_int4(running, lo, hi, 0);
   }
+  // per listed angle, each column's exact height offset (one byte)
+  __threadfence_block();
+  __syncthreads();
+  const int qx = qx0 + tid % GT, qy = qy0 + tid / GT;
+  const TileEntry* lst = list + (size_t)tile * g.A;
+  uint8_t* zo = zoff + (size_t)tile * g.A * 256;
+  for (int i = 0; i < running; ++i) {
+    const TileEntry e = lst[i];
+    zo[(size_t)i * 256 + tid] = hit_offset(g, g.ageo[e.a], qx, qy, e.zpack & 0xffff);
+  }
 }
 
 // flags: bit0 = normalise (divide by counts, 0 -> 1) and apply the centring
@@ -163,7 +167,13 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
 template <int ZW>
 __global__ void __launch_bounds__(256)
 k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __restrict__ list,
-             const int4* __restrict__ tinfo, int flags) {
+             const int4* __restrict__ tinfo, const uint8_t* __restrict__ zoff, int flags,
+             const int2* __restrict__ band) {
+  // The sums must be the sequential FP32 sums of the reference: no fused
+  // multiply-add. The pragma is lexical, so the products and sums are written
+  // here as plain operators (HIP's __fmul_rn/__fadd_rn are header functions
+  // outside its scope and would still be contracted).
+#pragma clang fp contract(off)
   __shared__ float2 acc[ZW][256];
   __shared__ int cn[ZW][256];
   // consecutive work items on one XCD (blocks are dealt round-robin to the 8 XCDs)
@@ -177,6 +187,10 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
   const size_t plane = (size_t)g.V * g.V;
   const size_t col = (size_t)Y * g.V + X;
   const int z0 = w * ZW, z1 = min(z0 + ZW, g.V) - 1;
+  if (band) {  // banded output: windows outside the hit band are not written
+    const int2 b = *band;
+    if (z1 < b.x || z0 > b.y) return;
+  }
   const int4 info = tinfo[tile];
   const bool norm = flags & 1;
   if (info.x == 0 || z1 < info.y || z0 > info.z) {
@@ -189,22 +203,45 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
 #pragma unroll
   for (int k = 0; k < ZW; ++k) { acc[k][tid] = make_float2(0.f, 0.f); cn[k][tid] = 0; }
   const TileEntry* lst = list + (size_t)tile * g.A;
-  for (int i = 0; i < info.x; ++i) {
-    const TileEntry e = lst[i];
-    const int zl = e.zpack & 0xffff, zh = e.zpack >> 16;
-    if (zh < z0 || zl > z1) continue;
-    const AngleGeo ag = g.ageo[e.a];
-    int z;
-    float2 v;
-    if (gather_hit(g, ag, e.a, qx, qy, z, v)) {
-      const int k = z - z0;
-      if (k >= 0 && k < ZW) {
-        float2 s = acc[k][tid];
-        s.x = __fadd_rn(s.x, v.x);
-        s.y = __fadd_rn(s.y, v.y);
-        acc[k][tid] = s;
-        cn[k][tid] += 1;
-      }
+  const uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
+  // spectrum element read by this column: the unshifted frame holds it at
+  // (qy mod C, qx mod C) for every angle; the +posSpec frame at fd + NXMAX
+  const int frow = ((qy % g.C) + g.C) % g.C, fcol = ((qx % g.C) + g.C) % g.C;
+  // Each wave tests 64 entries against the window at once and then visits
+  // the overlapping ones lowest index first, which keeps the angle order.
+  const int lane = tid & 63;
+  for (int j0 = 0; j0 < info.x; j0 += 64) {
+    const int j = j0 + lane;
+    TileEntry e{0, 0};
+    bool ov = false;
+    if (j < info.x) {
+      e = lst[j];
+      const int zl = e.zpack & 0xffff, zh = e.zpack >> 16;
+      ov = !(zh < z0 || zl > z1);
+    }
+    unsigned long long m = __ballot(ov);  // wave-uniform
+    while (m) {
+      const int bit = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      const int i = j0 + bit;
+      const int ea = __shfl(e.a, bit, 64);
+      const int zl = __shfl(e.zpack, bit, 64) & 0xffff;
+      const int off = zo[(size_t)i * 256];
+      if (off == 255) continue;
+      const int k = zl + off - z0;
+      if (k < 0 || k >= ZW) continue;
+      const AngleGeo ag = g.ageo[ea];
+      const int fdx = qx + ag.fx, fdy = qy + ag.fy;
+      const size_t base = (size_t)ea * g.C * g.C;
+      const float2 sp = g.frame_pos ? g.spec[base + (size_t)(fdy + g.nx) * g.C + fdx + g.nx]
+                                    : g.spec[base + (size_t)frow * g.C + fcol];
+      const float b = g.beta[fdx * fdx + fdy * fdy];
+      const float pim = b * sp.y, pre = b * sp.x;  // value (-b Si, b Sr)
+      float2 s = acc[k][tid];
+      s.x = s.x + (-pim);
+      s.y = s.y + pre;
+      acc[k][tid] = s;
+      cn[k][tid] += 1;
     }
   }
   for (int z = z0; z <= z1; ++z) {
@@ -231,37 +268,85 @@ hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256)
+k_band(const AngleGeo* __restrict__ ageo, int A, int V, const double* __restrict__ htab,
+       int r2max, int2* __restrict__ band) {
+  __shared__ int red[2][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int lo = 0x7fffffff, hi = -0x7fffffff;
+  for (int a = tid; a < A; a += 256) {
+    const AngleGeo ag = ageo[a];
+    if (ag.skip) continue;
+    // hits of angle a lie between the heights at the pupil's rim and centre
+    lo = min(lo, (int)round_away(htab[r2max] - ag.fz) + V / 2);
+    hi = max(hi, (int)round_away(htab[0] - ag.fz) + V / 2);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if (lane == 0) { red[0][wave] = lo; red[1][wave] = hi; }
+  __syncthreads();
+  if (tid == 0) {
+    lo = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+    hi = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+    lo = max(lo, 0);
+    hi = min(hi, V - 1);
+    band[0] = lo <= hi ? make_int2(lo, hi) : make_int2(1, 0);
+  }
+}
+
+hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, int r2max,
+                       int2* band, hipStream_t s) {
+  Timed tm("ewald_band", s);
+  hipLaunchKernelGGL(k_band, dim3(1), dim3(256), 0, s, ageo, A, V, htab, r2max, band);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, void* scratch, int flags, hipStream_t s) {
+                         int32_t* cnt, void* scratch, int flags, const int2* band,
+                         hipStream_t s) {
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
   constexpr int ZW = 16;
   const int ntx = V / GT, tiles = ntx * ntx;
+  // scratch: [tiles][A] entries | [tiles][A][256] height offsets | [tiles] info
+  const size_t nA = (size_t)(A > 0 ? A : 1);
   TileEntry* list = (TileEntry*)scratch;
-  int4* tinfo = (int4*)((char*)scratch + gather_scratch_bytes(V, A) - (size_t)tiles * sizeof(int4));
+  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
+  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
   {
     Timed tm("ewald_tiles", s);
-    hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo);
+    hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
   }
   const int nwin = (V + ZW - 1) / ZW, items = tiles * nwin;
   Timed tm("ewald_gather", s);
   hipLaunchKernelGGL((k_gather_win<ZW>), dim3(8 * ((items + 7) / 8)), dim3(256), 0, s, g,
-                     ntx, nwin, items, (const TileEntry*)list, (const int4*)tinfo, flags);
+                     ntx, nwin, items, (const TileEntry*)list, (const int4*)tinfo,
+                     (const uint8_t*)zoff, flags, band);
   return hipGetLastError();
 }
 
 size_t gather_scratch_bytes(int V, int A) {
   const size_t tiles = (size_t)(V / GT) * (V / GT);
-  return tiles * (size_t)(A > 0 ? A : 1) * sizeof(TileEntry) + tiles * sizeof(int4);
+  const size_t nA = (size_t)(A > 0 ? A : 1);
+  return tiles * nA * (sizeof(TileEntry) + 256) + tiles * sizeof(int4);
 }
 
 // ---------------------------------------------------------------------------
 // finalisation passes
 // ---------------------------------------------------------------------------
+// heights outside [band.x, band.y] hold zeros; band == nullptr: none
+__device__ __forceinline__ bool outside_band(const int2* band, int z) {
+  if (!band) return false;
+  const int2 b = *band;
+  return z < b.x || z > b.y;
+}
+
 template <int V>
 struct NormXOp {  // rows along x; a = z, line = y
   const float2* vol;
@@ -270,6 +355,8 @@ struct NormXOp {  // rows along x; a = z, line = y
   uint8_t* otf;
   const float2* tw;
   int prenorm;
+  const int2* band;
+  __device__ bool skip(int z) const { return outside_band(band, z); }
   __device__ void begin(int) {}
   __device__ float2 load(int z, int y, int x) const {
     const size_t i = ((size_t)z * V + y) * V + x;
@@ -295,6 +382,8 @@ template <int V>
 struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
   float2* buf;
   const float2* tw;
+  const int2* band;
+  __device__ bool skip(int z) const { return outside_band(band, z); }
   __device__ void begin(int) {}
   __device__ float2 load(int z, int x, int y) const { return buf[((size_t)z * V + y) * V + x]; }
   __device__ void store(int z, int x, int y, float2 v) const { buf[((size_t)z * V + y) * V + x] = v; }
@@ -307,8 +396,22 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
   float* im;
   const float2* tw;
   float df3, cte, gsign;
-  __device__ void begin(int) {}
-  __device__ float2 load(int y, int x, int z) const { return buf[((size_t)z * V + y) * V + x]; }
+  const int2* band;
+  int lo, hi;
+  __device__ void begin(int) {
+    if (band) {
+      const int2 b = *band;
+      lo = b.x;
+      hi = b.y;
+    } else {
+      lo = 0;
+      hi = V - 1;
+    }
+  }
+  __device__ float2 load(int y, int x, int z) const {
+    if (z < lo || z > hi) return make_float2(0.f, 0.f);
+    return buf[((size_t)z * V + y) * V + x];
+  }
   __device__ void store(int y, int x, int z, float2 v) const {
     const float sg = ((x + y + z) & 1) ? -gsign : gsign;
     const size_t i = ((size_t)z * V + y) * V + x;
@@ -318,27 +421,26 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
 };
 
 template <int V>
-static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm, float2* work,
+static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
+                             const int2* band, float2* work,
                              float df3, float cte, float gsign, float* re,
                              float* im, uint8_t* otf, const float2* tw,
                              hipStream_t s) {
-  NormXOp<V> x{vol, cnt, work, otf, tw, prenorm};
+  NormXOp<V> x{vol, cnt, work, otf, tw, prenorm, band};
   {
     Timed tm("vol_x", s);
     hipLaunchKernelGGL((k_rows<V, true, NormXOp<V>>), rows_grid<V>(V), rows_block<V>(),
                        0, s, x);
   }
-  constexpr int NC = cols_nc<V>();
-  PlaneYOp<V> y{work, tw};
+  PlaneYOp<V> y{work, tw, band};
   {
     Timed tm("vol_y", s);
-    hipLaunchKernelGGL((k_cols<V, true, NC, PlaneYOp<V>>), dim3(V / NC, V), dim3(256),
-                       0, s, y);
+    hipLaunchKernelGGL((k_cols<V, true, PlaneYOp<V>>), cols_grid<V>(V), cols_block<V>(), 0, s,
+                       y);
   }
-  ZOutOp<V> z{work, re, im, tw, df3, cte, gsign};
+  ZOutOp<V> z{work, re, im, tw, df3, cte, gsign, band, 0, V - 1};
   Timed tm("vol_z", s);
-  hipLaunchKernelGGL((k_cols<V, true, NC, ZOutOp<V>>), dim3(V / NC, V), dim3(256),
-                     0, s, z);
+  hipLaunchKernelGGL((k_cols<V, true, ZOutOp<V>>), cols_grid<V>(V), cols_block<V>(), 0, s, z);
   return hipGetLastError();
 }
 
@@ -346,12 +448,13 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
   X(32) X(64) X(128) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(704) X(768) X(1024)
 
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
-                           int prenorm, float2* work, float df3, float cte,
+                           int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
                            uint8_t* otf, const float2* tw, hipStream_t s) {
   switch (V) {
 #define X(n) \
-  case n: return finalize_v<n>(vol, cnt, prenorm, work, df3, cte, gsign, out_re, out_im, otf, tw, s);
+  case n:    \
+    return finalize_v<n>(vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, out_im, otf, tw, s);
     MTD_VOL_SIZES(X)
 #undef X
   }

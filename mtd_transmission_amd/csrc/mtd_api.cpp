@@ -66,8 +66,8 @@ void* ws_get(mtd_ctx* ctx, int slot, size_t bytes) {
   Buffer& b = ctx->ws[slot];
   if (b.bytes >= bytes && b.ptr) return b.ptr;
   if (b.ptr) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(b.ptr);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
   }
@@ -79,6 +79,29 @@ void* ws_get(mtd_ctx* ctx, int slot, size_t bytes) {
   return b.ptr;
 }
 
+// device copy of a host table; nullptr when allocation or copy fails
+template <class T>
+static T* upload(const T* h, size_t n) {
+  T* d = nullptr;
+  if (hipMalloc(&d, sizeof(T) * n) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  return d;
+}
+
+static uint64_t bits_of(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, sizeof u);
+  return u;
+}
+static uint32_t bits_of(float v) {
+  uint32_t u;
+  std::memcpy(&u, &v, sizeof u);
+  return u;
+}
+
 const float2* twiddle(mtd_ctx* ctx, int L) {
   auto it = ctx->twiddles.find(L);
   if (it != ctx->twiddles.end()) return it->second;
@@ -88,9 +111,8 @@ const float2* twiddle(mtd_ctx* ctx, int L) {
     const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
     h[k] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
   }
-  float2* d = nullptr;
-  if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) return nullptr;
-  hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice);
+  float2* d = upload(h.data(), h.size());
+  if (!d) return nullptr;
   ctx->twiddles[L] = d;
   return d;
 }
@@ -115,14 +137,14 @@ void tukey1d_host(int N, double alpha_d, double* out) {
 }
 
 const double* tukey_table(mtd_ctx* ctx, int n, double alpha) {
-  const auto key = std::make_pair(n, (long long)std::llround(alpha * 1e9));
+  // the window depends on alpha only through (float)alpha
+  const auto key = std::make_pair(n, bits_of((float)alpha));
   auto it = ctx->tukey.find(key);
   if (it != ctx->tukey.end()) return it->second;
   std::vector<double> h(n);
   tukey1d_host(n, alpha, h.data());
-  double* d = nullptr;
-  if (hipMalloc(&d, sizeof(double) * n) != hipSuccess) return nullptr;
-  hipMemcpy(d, h.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  double* d = upload(h.data(), h.size());
+  if (!d) return nullptr;
   ctx->tukey[key] = d;
   return d;
 }
@@ -171,19 +193,21 @@ int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
       for (int j = k + 1; j < 15; ++j) G[i][j] -= G[i][k] * G[k][j];
     }
   }
-  hipFree(p.d_samples);
-  hipFree(p.d_lu);
-  hipFree(p.d_piv);
-  p.d_samples = nullptr;
-  p.d_lu = nullptr;
-  p.d_piv = nullptr;
-  if (hipMalloc(&p.d_samples, sizeof(int2) * M) != hipSuccess ||
-      hipMalloc(&p.d_lu, sizeof(double) * 225) != hipSuccess ||
-      hipMalloc(&p.d_piv, sizeof(int) * 15) != hipSuccess)
+  // the plan is about to change: no queued fit may still read the old one
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(p.d_samples);
+  (void)hipFree(p.d_lu);
+  (void)hipFree(p.d_perm);
+  p.d_samples = upload(s.data(), s.size());
+  p.d_lu = upload(&G[0][0], 225);
+  // the sequential swaps as one permutation: (Pb)[i] = b[perm[i]]
+  int perm[15];
+  for (int k = 0; k < 15; ++k) perm[k] = k;
+  for (int k = 0; k < 15; ++k) std::swap(perm[k], perm[piv[k]]);
+  p.d_perm = upload(perm, 15);
+  p.C = 0;
+  if (!p.d_samples || !p.d_lu || !p.d_perm)
     return set_err(ctx, MTD_ERR_HIP, "out of device memory (fit plan)");
-  hipMemcpy(p.d_samples, s.data(), sizeof(int2) * M, hipMemcpyHostToDevice);
-  hipMemcpy(p.d_lu, &G[0][0], sizeof(double) * 225, hipMemcpyHostToDevice);
-  hipMemcpy(p.d_piv, piv, sizeof(int) * 15, hipMemcpyHostToDevice);
   p.M = M;
   p.C = C;
   p.mask = m;
@@ -216,7 +240,7 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
                    "COEF_NA_OBJ_LIMIT*NXMAX^2 exceeds rayon^2: the reference would take "
                    "the square root of a negative height");
   const int r2tab = 2 * nx * nx + 1;
-  const auto key = std::make_pair((long long)std::llround(R), (long long)r2tab);
+  const auto key = std::make_pair(bits_of(R), r2tab);
   double* d_h;
   auto it = ctx->htab.find(key);
   std::vector<double> h(r2tab);
@@ -225,9 +249,8 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
     h[r] = v >= 0 ? std::sqrt(v) : 0.0;
   }
   if (it == ctx->htab.end()) {
-    if (hipMalloc(&d_h, sizeof(double) * r2tab) != hipSuccess)
-      return set_err(ctx, MTD_ERR_HIP, "out of device memory (height table)");
-    hipMemcpy(d_h, h.data(), sizeof(double) * r2tab, hipMemcpyHostToDevice);
+    d_h = upload(h.data(), h.size());
+    if (!d_h) return set_err(ctx, MTD_ERR_HIP, "out of device memory (height table)");
     ctx->htab[key] = d_h;
   } else {
     d_h = it->second;
@@ -241,14 +264,12 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
   const double inv_r = 1.0 / R;
   std::vector<float> b(r2max + 1);
   for (int r = 0; r <= r2max; ++r) b[r] = (float)(gamma * (h[r] * inv_r));
-  long long bkey = std::llround(R) * 1000003LL + r2max;
-  bkey = bkey * 1000003LL + (long long)(*(const uint32_t*)&k0);
+  const auto bkey = std::make_tuple(bits_of(R), r2max, bits_of(k0));
   float* d_b;
   auto bt = ctx->betatab.find(bkey);
   if (bt == ctx->betatab.end()) {
-    if (hipMalloc(&d_b, sizeof(float) * (r2max + 1)) != hipSuccess)
-      return set_err(ctx, MTD_ERR_HIP, "out of device memory (beta table)");
-    hipMemcpy(d_b, b.data(), sizeof(float) * (r2max + 1), hipMemcpyHostToDevice);
+    d_b = upload(b.data(), b.size());
+    if (!d_b) return set_err(ctx, MTD_ERR_HIP, "out of device memory (beta table)");
     ctx->betatab[bkey] = d_b;
   } else {
     d_b = bt->second;
@@ -274,8 +295,11 @@ static int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
   if (g->nxmax <= 0) return set_err(ctx, MTD_ERR_ARG, "NXMAX must be positive");
   if (!crop_len_supported(C))
     return set_err(ctx, MTD_ERR_UNSUPPORTED, "crop side 2*NXMAX=" + std::to_string(C) + " not built");
-  if (g->deroul)
-    return set_err(ctx, MTD_ERR_UNSUPPORTED, "DEROUL 1 (phase unwrapping) is not implemented");
+  if (g->deroul == MTD_DEROUL_HERRAEZ)
+    return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                   "DEROUL 1 with VOLKOV 0 (Herraez unwrapping) is not implemented");
+  if (g->deroul != MTD_DEROUL_OFF && g->deroul != MTD_DEROUL_VOLKOV)
+    return set_err(ctx, MTD_ERR_ARG, "deroul must be MTD_DEROUL_OFF, _VOLKOV or _HERRAEZ");
   if (need & NEED_HOLO) {
     if (!holo_len_supported(g->n))
       return set_err(ctx, MTD_ERR_UNSUPPORTED, "hologram side " + std::to_string(g->n) + " not built");
@@ -352,16 +376,32 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
   MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, part, ctx->stream), "argmax"));
   MTD_TRY(hip_check(ctx, launch_demod(C, crop, posx, posy, n, twC, work, iu, ctx->stream), "demodulation"));
   const int mode = field_mode(g);
+  // DEROUL 1 / VOLKOV 1: the phase is unwrapped before the fit; the
+  // reference only unwraps when the aberration correction runs (main.cpp:253-277)
+  float* phu = nullptr;
+  if (g->c_aber && g->deroul == MTD_DEROUL_VOLKOV) {
+    const float2* tw2 = twiddle(ctx, 2 * C);
+    phu = (float*)ws_get(ctx, WS_PHU, (size_t)n * C * C * sizeof(float));
+    char* us = (char*)ws_get(ctx, WS_UNWRAP, unwrap_scratch_bytes(C, n));
+    if (!tw2 || !phu || !us) return set_err(ctx, MTD_ERR_HIP, "out of device memory (unwrapping)");
+    const size_t D2 = (size_t)4 * C * C * n;  // (2C)^2 per angle
+    float* gx = (float*)us;
+    float* gy = gx + D2;
+    float2* P = (float2*)(gy + D2);
+    float2* Q = P + D2;
+    MTD_TRY(hip_check(ctx, launch_unwrap(C, iu, n, tw2, gx, gy, P, Q, phu, ctx->stream),
+                      "phase unwrapping"));
+  }
   if (g->c_aber)
-    MTD_TRY(hip_check(ctx, launch_lsq(C, iu, n, ctx->lsq, coef, ctx->stream), "aberration fit"));
+    MTD_TRY(hip_check(ctx, launch_lsq(C, iu, phu, n, ctx->lsq, coef, ctx->stream), "aberration fit"));
   if (out_fields)
-    MTD_TRY(hip_check(ctx, launch_form_field(C, iu, crop, posx, posy, n, coef, mode, out_fields,
+    MTD_TRY(hip_check(ctx, launch_form_field(C, iu, crop, posx, posy, n, coef, mode, phu, out_fields,
                                              ctx->stream), "field formation"));
   if (out_F) {
     const double* winC = tukey_table(ctx, C, g->alpha_field);
     if (!winC) return set_err(ctx, MTD_ERR_HIP, "out of device memory (window)");
     const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
-    MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, iu, 1, crop, coef, mode, posx, posy, n, winC,
+    MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, iu, 1, crop, coef, mode, phu, posx, posy, n, winC,
                                                twC, dx2, 0, work, out_F, ctx->stream),
                       "forward spectrum"));
   }
@@ -378,10 +418,18 @@ extern "C" {
 
 const char* mtd_version(void) { return "mtd-mi355x 0.1 (gfx950)"; }
 
+// message of the last failed mtd_ctx_create on this thread
+static thread_local std::string tl_create_err;
+
 int mtd_ctx_create(int device, mtd_ctx** out) {
   if (!out) return MTD_ERR_ARG;
   *out = nullptr;
-  if (hipSetDevice(device) != hipSuccess) return MTD_ERR_HIP;
+  const hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    tl_create_err = "hipSetDevice(" + std::to_string(device) + "): " + hipGetErrorName(e) +
+                    " (" + hipGetErrorString(e) + ")";
+    return MTD_ERR_HIP;
+  }
   mtd_ctx* c = new mtd_ctx();
   c->device = device;
   c->stream = nullptr;
@@ -391,18 +439,19 @@ int mtd_ctx_create(int device, mtd_ctx** out) {
 
 int mtd_ctx_destroy(mtd_ctx* ctx) {
   if (!ctx) return MTD_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (auto& kv : ctx->twiddles) hipFree(kv.second);
-  for (auto& kv : ctx->tukey) hipFree(kv.second);
-  for (auto& kv : ctx->htab) hipFree(kv.second);
-  for (auto& kv : ctx->betatab) hipFree(kv.second);
-  for (auto& kv : ctx->ws) hipFree(kv.second.ptr);
+  // teardown is best effort: errors here have no caller left to act on them
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   timing_collect(ctx);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
-  hipFree(ctx->lsq.d_samples);
-  hipFree(ctx->lsq.d_lu);
-  hipFree(ctx->lsq.d_piv);
+  for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (auto& kv : ctx->tukey) (void)hipFree(kv.second);
+  for (auto& kv : ctx->htab) (void)hipFree(kv.second);
+  for (auto& kv : ctx->betatab) (void)hipFree(kv.second);
+  for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);
+  (void)hipFree(ctx->lsq.d_samples);
+  (void)hipFree(ctx->lsq.d_lu);
+  (void)hipFree(ctx->lsq.d_perm);
   delete ctx;
   return MTD_OK;
 }
@@ -418,7 +467,10 @@ int mtd_ctx_sync(mtd_ctx* ctx) {
   return hip_check(ctx, hipStreamSynchronize(ctx->stream), "stream synchronize");
 }
 
-const char* mtd_last_error(const mtd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* mtd_last_error(const mtd_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return tl_create_err.empty() ? "null context" : tl_create_err.c_str();
+}
 
 int mtd_ctx_set_timing(mtd_ctx* ctx, int on) {
   if (!ctx) return MTD_ERR_ARG;
@@ -557,7 +609,7 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
-                                               nullptr, nullptr, field_mode(g), posspec + a0,
+                                               nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
                                                posspec + A + a0, n, winC, twC, dx2, 1, work,
                                                (float2*)spectra + (size_t)a0 * C * C,
                                                ctx->stream),
@@ -566,10 +618,12 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   return MTD_OK;
 }
 
-// flags: 1 = write the count-normalised, centring-signed spectrum (no counts)
+// flags: 1 = write the count-normalised, centring-signed spectrum (no counts).
+// band != nullptr: the hit band is computed into it and only the heights that
+// meet it are written.
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
                         const int32_t* posspec, int A, float* vol, int32_t* counts,
-                        int flags = 0) {
+                        int flags = 0, int2* band = nullptr) {
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
@@ -578,9 +632,12 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
   if (A > 0)
     MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
                                             t.r2tab, ag, ctx->stream), "angle geometry"));
+  if (band)
+    MTD_TRY(hip_check(ctx, launch_band(ag, A, g->dim_final, t.htab, t.r2max, band, ctx->stream),
+                      "hit band"));
   MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
                                        2 * g->nxmax, g->nxmax, g->dim_final, t.htab, t.beta,
-                                       t.r2max, (float2*)vol, counts, scratch, flags,
+                                       t.r2max, (float2*)vol, counts, scratch, flags, band,
                                        ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
@@ -612,7 +669,7 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
-                                               nullptr, nullptr, field_mode(g), posspec + a0,
+                                               nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
                                                posspec + A + a0, n, winC, twC, dx2, 0, work,
                                                F + (size_t)a0 * C * C, ctx->stream),
                       "forward spectrum"));
@@ -621,8 +678,8 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
 }
 
 static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
-                         const int32_t* counts, int prenorm, float* index_re, float* index_im,
-                         uint8_t* otf) {
+                         const int32_t* counts, int prenorm, const int2* band, float* index_re,
+                         float* index_im, uint8_t* otf) {
   const int V = g->dim_final;
   float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
   const float2* twV = twiddle(ctx, V);
@@ -633,7 +690,7 @@ static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
   const double cte = -1.0 / (2 * g->n0 * kv * kv);
   const float df3 = (float)std::pow(g->delta_f, 3);
   const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
-  return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, prenorm, work, df3,
+  return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, prenorm, band, work, df3,
                                         (float)cte, gsign, index_re, index_im, otf, twV,
                                         ctx->stream),
                    "finalize");
@@ -644,7 +701,7 @@ int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, 0, NEED_VOL));
   if (!vol || !counts || !index_re || !index_im) return set_err(ctx, MTD_ERR_ARG, "null array");
-  return finalize_impl(ctx, g, vol, counts, 0, index_re, index_im, otf);
+  return finalize_impl(ctx, g, vol, counts, 0, nullptr, index_re, index_im, otf);
 }
 
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
@@ -664,16 +721,19 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, std::max(A, 1));
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
-  if (!F || !crop || !vol) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+  if (!F || !crop || !vol || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
     MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
                         F + (size_t)a0 * C * C));
   }
-  // the gather writes the count-normalised, centring-signed spectrum
-  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1));
-  return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, index_re, index_im, nullptr);
+  // The gather writes the count-normalised, centring-signed spectrum, and
+  // only in the height band that holds hits; the inverse transform treats the
+  // rest as the zeros it is and never reads it.
+  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1, band));
+  return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, index_re, index_im, nullptr);
 }
 
 }  // extern "C"

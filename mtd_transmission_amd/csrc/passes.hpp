@@ -1,58 +1,103 @@
 // Generic batched 1-D FFT passes over stacks of lines, parameterised by an
 // operation object that maps (batch, line, element) to memory on load and on
-// store. Rows: each transform is one contiguous line. Columns: NC adjacent
-// columns are loaded as a tile (coalesced along the row) and transposed into
-// LDS.
+// store. Transforms use the two-step register FFT (fft_reg.hpp): inputs are
+// loaded straight from global memory into registers and outputs stored
+// straight from registers, with one LDS transpose in between.
+//   rows:    each transform is one contiguous line; NT lines per block.
+//   columns: NC adjacent columns per block; lane = column fastest, so every
+//            load and store touches NC contiguous elements of a row.
 #pragma once
-#include "fft_lds.hpp"
+#include <type_traits>
+#include <utility>
+
+#include "fft_reg.hpp"
 
 namespace mtd {
 
-template <int C, bool INV, class Op>
+// transforms per block: at most 256 threads and 64 KiB of LDS per block
+template <int L>
+constexpr int lines_per_block() {
+  constexpr int by_thr = 256 / Fft2<L, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS * 8);
+  return divisor_le(L, by_thr < by_lds ? by_thr : by_lds);
+}
+template <int L>
+constexpr int rows_nt() {
+  return lines_per_block<L>();
+}
+template <int L>
+constexpr int cols_nc() {
+  return lines_per_block<L>();
+}
+
+// An operation may define skip(batch): the whole batch is then left alone
+// (neither read nor written) when it returns true.
+template <class Op, class = void>
+struct has_skip : std::false_type {};
+template <class Op>
+struct has_skip<Op, std::void_t<decltype(std::declval<const Op&>().skip(0))>>
+    : std::true_type {};
+
+template <class Op>
+__device__ __forceinline__ bool skip_batch(const Op& op, int a) {
+  if constexpr (has_skip<Op>::value) return op.skip(a);
+  return false;
+}
+
+template <int L, bool INV, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op) {
-  constexpr int T = fft_threads(C);
-  constexpr int NT = divisor_le(C, fft_per_block(C));
-  __shared__ float2 lds[NT][C];
+  using F = Fft2<L, INV>;
+  constexpr int NT = rows_nt<L>();
+  __shared__ float2 lds[NT * F::LDS];
   const int a = blockIdx.y;
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  if (skip_batch(op, a)) return;  // block-uniform, before the barrier
+  const int r = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
   const int line = blockIdx.x * NT + r;
   op.begin(a);
-  for (int i = t; i < C; i += T) lds[r][i] = op.load(a, line, i);
-  __syncthreads();
-  fft_lds<C, T, INV>(lds[r], t, op.tw);
-  for (int i = t; i < C; i += T) op.store(a, line, i, lds[r][i]);
+  float2 v[F::L1], u[F::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = op.load(a, line, F::L2 * n1 + n2);
+  F::run(v, u, n2, lds + r * F::LDS, op.tw);
+  if (n2 < F::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < F::L2; ++k2) op.store(a, line, n2 + F::L1 * k2, u[k2]);
+  }
 }
 
-template <int C, bool INV, int NC, class Op>
+template <int L, bool INV, class Op>
 __global__ void __launch_bounds__(256) k_cols(Op op) {
-  constexpr int T = 256 / NC;
-  __shared__ float2 lds[NC][C + 1];
+  using F = Fft2<L, INV>;
+  constexpr int NC = cols_nc<L>();
+  __shared__ float2 lds[NC * F::LDS];
   const int a = blockIdx.y, c0 = blockIdx.x * NC;
+  if (skip_batch(op, a)) return;  // block-uniform, before the barrier
+  const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
   op.begin(a);
-  for (int e = threadIdx.x; e < C * NC; e += 256) {
-    const int y = e / NC, c = e % NC;
-    lds[c][y] = op.load(a, c0 + c, y);
-  }
-  __syncthreads();
-  const int c = threadIdx.x / T, t = threadIdx.x % T;
-  fft_lds<C, T, INV>(lds[c], t, op.tw);
-  for (int e = threadIdx.x; e < C * NC; e += 256) {
-    const int y = e / NC, cc = e % NC;
-    op.store(a, c0 + cc, y, lds[cc][y]);
+  float2 v[F::L1], u[F::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = op.load(a, c0 + c, F::L2 * n1 + n2);
+  F::run(v, u, n2, lds + c * F::LDS, op.tw);
+  if (n2 < F::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < F::L2; ++k2) op.store(a, c0 + c, n2 + F::L1 * k2, u[k2]);
   }
 }
 
-template <int C>
+template <int L>
 inline dim3 rows_grid(int A) {
-  return dim3(C / divisor_le(C, fft_per_block(C)), A);
+  return dim3(L / rows_nt<L>(), A);
 }
-template <int C>
+template <int L>
 inline dim3 rows_block() {
-  return dim3(divisor_le(C, fft_per_block(C)) * fft_threads(C));
+  return dim3(rows_nt<L>() * Fft2<L, false>::T);
 }
-template <int C>
-constexpr int cols_nc() {
-  return C >= 256 ? 16 : (C >= 64 ? 8 : 4);
+template <int L>
+inline dim3 cols_grid(int A) {
+  return dim3(L / cols_nc<L>(), A);
+}
+template <int L>
+inline dim3 cols_block() {
+  return dim3(cols_nc<L>() * Fft2<L, false>::T);
 }
 
 }  // namespace mtd

@@ -34,7 +34,11 @@ int main(int argc, char** argv) {
   const double cx = val(cfg, "CIRCLE_CX"), cy = val(cfg, "CIRCLE_CY");
   const long premier = (long)(size_t)val(recon, "PREMIER_ANGLE");
   const long nb_holo = (long)(size_t)val(cfg, "NB_HOLO");
-  const int c_aber = val(recon, "C_ABER") != 0, deroul = val(recon, "DEROUL") != 0;
+  const int c_aber = val(recon, "C_ABER") != 0;
+  // DEROUL / VOLKOV are read as booleans (manip.cpp:77-79); VOLKOV 0 selects
+  // Herraez, which the library reports as not built
+  const int deroul = val(recon, "DEROUL") == 0 ? MTD_DEROUL_OFF
+                     : val(recon, "VOLKOV") != 0 ? MTD_DEROUL_VOLKOV : MTD_DEROUL_HERRAEZ;
   const int born = val(recon, "BORN") != 0;
   if ((size_t)nxmax == 0)
     std::cout << "----- ERREUR : NXMAX absent du fichier de configuration" << cfg << "\n";

@@ -372,7 +372,7 @@ def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
     """main.cpp:254-341 for one angle, from the demodulated field U.
 
     C_ABER 1: phase = atan2 (fonctions.cpp:39-43); DEROUL 0 keeps it wrapped
-    (main.cpp:277); phase and amplitude backgrounds by aberCorr2 / ampliCorr2
+    (main.cpp:277), DEROUL 1 / VOLKOV 1 unwraps it (main.cpp:271); phase and amplitude backgrounds by aberCorr2 / ampliCorr2
     (Correction_aberration.cpp:166-184); amplitude clamp (main.cpp:301-302);
     Born (main.cpp:307,309) or Rytov (:312-313).
     C_ABER 0: the spectrum is multiplied by conj(s)/|s| before the demodulation
@@ -384,6 +384,10 @@ def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
         f = ct(complex(s.real / mod, -s.imag / mod))
         return (U.astype(ct) * f).astype(ct)
     phi = np.arctan2(U.imag, U.real).astype(rt)
+    if g.deroul == DEROUL_VOLKOV:
+        phi = volkov_unwrap(phi, prec)
+    elif g.deroul:
+        raise NotImplementedError("DEROUL 1 with VOLKOV 0 (Herraez) is not restated")
     amp = np.abs(U).astype(rt)
     bg_phi, _ = lsq_background(phi, plan)
     bg_amp, _ = lsq_background(amp, plan)
@@ -407,6 +411,101 @@ def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
     return _mk(np.log(aa), phic, np.complex128)
 
 
+# --------------------------------------------------------------------------
+# A8: phase unwrapping, DEROUL 1 with VOLKOV 1 (deroul_volkov4_total_sym_paire)
+# --------------------------------------------------------------------------
+DEROUL_OFF, DEROUL_VOLKOV, DEROUL_HERRAEZ = 0, 1, 2
+
+
+def kvect_shift(D):
+    """init_kvect_shift (FFT_fonctions.cpp:7-17): (i - round(D/2))/D, then
+    fftshift2D: the frequency, in cycles per pixel, of FFT index i (the
+    Nyquist index D/2 gets -1/2). Same for x and y (square images)."""
+    i = np.arange(D)
+    return (((i + D // 2) % D) - D // 2).astype(np.float64) / D
+
+
+def symmetrise_mirror(img):
+    """Symetrise_mirror (src/deroulement_volkov4.cpp:221-240): the 2C x 2C
+    even extension; the original is the [C,2C)^2 quadrant, the other three are
+    its mirror images about the half-pixel lines."""
+    top = np.concatenate([img[::-1, ::-1], img[::-1, :]], axis=1)
+    bottom = np.concatenate([img[:, ::-1], img], axis=1)
+    return np.concatenate([top, bottom], axis=0)
+
+
+def _fft2_fwd(x, prec):
+    """TF2Dcplx (FFT_fonctions.cpp:366-400): forward, divided by the pixel count."""
+    return np.fft.fft2(x.astype(_cplx(prec)), norm="forward").astype(_cplx(prec))
+
+
+def _fft2_bwd(x, prec):
+    """TF2Dcplx_INV (FFT_fonctions.cpp:420-438): backward, unnormalised."""
+    return np.fft.ifft2(x.astype(_cplx(prec)), norm="forward").astype(_cplx(prec))
+
+
+def gradient_fft(f, prec="f64"):
+    """gradient_fft4 (src/deroulement_volkov4.cpp:98-159), real or complex
+    input: spectrum = TF2Dcplx(f); grad = TF2Dcplx_INV(2 pi k * spectrum * i)
+    for k = kx and k = ky. Returns (gx, gy), complex."""
+    rt = _real(prec)
+    D = f.shape[0]
+    k = kvect_shift(D).astype(rt)
+    two_pi = rt(2 * math.pi)
+    S = _fft2_fwd(f, prec)
+    jx = (two_pi * k)[None, :].astype(rt)
+    jy = (two_pi * k)[:, None].astype(rt)
+    gx = _fft2_bwd(_mk(-jx * S.imag, jx * S.real, _cplx(prec)), prec)
+    gy = _fft2_bwd(_mk(-jy * S.imag, jy * S.real, _cplx(prec)), prec)
+    return gx, gy
+
+
+def integ_grad(gx, gy, prec="f64"):
+    """integ_grad4 (src/deroulement_volkov4.cpp:162-191): least-squares
+    integration of a gradient field, tampon = -i (TFx kx + TFy ky) /
+    (2 pi |k|^2), then the unnormalised backward FFT.
+
+    Deviation: the reference sets the zero frequency to
+    (tampon[nbPix] + tampon[1]) / 2, reading one element past the array
+    (:189). It only adds a constant to the result, which the phase fit that
+    always follows unwrapping removes (the constant monomial is in the basis);
+    here it is 0."""
+    rt, ct = _real(prec), _cplx(prec)
+    D = gx.shape[0]
+    k = kvect_shift(D).astype(rt)
+    kx = k[None, :]
+    ky = k[:, None]
+    Fx = _fft2_fwd(gx, prec)
+    Fy = _fft2_fwd(gy, prec)
+    num = Fx * kx.astype(ct) + Fy * ky.astype(ct)            # complex
+    den = (rt(2 * math.pi) * (kx * kx + ky * ky)).astype(rt)  # real
+    den[0, 0] = 1
+    t = _mk(num.imag / den, -num.real / den, ct)              # -i num / den
+    t[0, 0] = 0
+    return _fft2_bwd(t, prec)
+
+
+def volkov_unwrap(phi, prec="f64"):
+    """deroul_volkov4_total_sym_paire (src/deroulement_volkov4.cpp:14-92):
+    mirror-extend the wrapped phase, compare the FFT gradient of the phase
+    with the FFT gradient of exp(i phase) to get the gradient of the 2 pi
+    multiples, integrate it (least squares, FFT Poisson solve), cut the
+    original quadrant and add 2 pi times it. The multiples are not rounded."""
+    rt, ct = _real(prec), _cplx(prec)
+    C = phi.shape[0]
+    ps = symmetrise_mirror(np.asarray(phi, dtype=rt))
+    gxw, gyw = gradient_fft(ps, prec)
+    Z = _mk(np.cos(ps).astype(rt), np.sin(ps).astype(rt), ct)
+    gzx, gzy = gradient_fft(Z, prec)
+    two_pi = rt(2 * math.pi)
+    ax = (-1j * (gzx / Z)).astype(ct)
+    ay = (-1j * (gzy / Z)).astype(ct)
+    gix = ((ax.real - gxw.real) / two_pi).astype(rt)
+    giy = ((ay.real - gyw.real) / two_pi).astype(rt)
+    intm = integ_grad(gix, giy, prec)[C:, C:].real.astype(rt)
+    return (np.asarray(phi, dtype=rt) + two_pi * intm).astype(rt)
+
+
 def load_mask(C, mask=None):
     """init_mask_aber (Correction_aberration.cpp:43-70): absent mask => 255."""
     if mask is None:
@@ -418,8 +517,6 @@ def preprocess(holos, g: Geom, mask=None, prec="f64", return_crops=False):
     """The two hot loops of pretraitement_hors_axe_CPU/main.cpp:130-342 over
     holograms that all exist. Returns (fields[A,C,C], posspec[2A] as ints:
     all x then all y (main.cpp:250-251))."""
-    if g.deroul:
-        raise NotImplementedError("DEROUL 1 (phase unwrapping) is not restated")
     C = g.C
     A = len(holos)
     window = tukey2d(g.n, g.alpha_holo)

@@ -30,7 +30,7 @@ class Case:
         p = self.params
         gl = m.geom_derive(p["n"], p["nxmax"], p["v"], p["cx"], p["cy"], NA, N0, LAM,
                            p["pix"], 1.0, 1.0, 1.0, p.get("coef", 1.0), p.get("born", 1),
-                           p.get("c_aber", 1), 0)
+                           p.get("c_aber", 1), p.get("deroul", 0))
         return gl
 
 
@@ -48,12 +48,12 @@ SPECS = {
 
 
 @functools.lru_cache(maxsize=None)
-def make_case(name, born=1, c_aber=1, seed=7):
+def make_case(name, born=1, c_aber=1, seed=7, deroul=0):
     n, nx, v, A, n_ell, noise, dtype, scan = SPECS[name]
     pix = nx * LAM / (NA * n)
     cx, cy = n // 2 - round(0.3 * n), n // 2 + round(0.3 * n)
     g = ref.derive_geom(n, nx, v, cx, cy, NA, N0, LAM, pix, 1, 1, 1, born=born,
-                        c_aber=c_aber)
+                        c_aber=c_aber, deroul=deroul)
     fov = n * pix
     # small fields of view get a proportionally smaller phantom so the mask
     # keeps enough background samples for the degree-4 fit
@@ -67,7 +67,8 @@ def make_case(name, born=1, c_aber=1, seed=7):
                          seed=seed + 1, noise=noise, dtype=dtype)
     if name in ("tiny", "nomask"):
         acq.mask = None  # the absent-mask path (Correction_aberration.cpp:48-52)
-    params = dict(n=n, nxmax=nx, v=v, cx=cx, cy=cy, pix=pix, born=born, c_aber=c_aber)
+    params = dict(n=n, nxmax=nx, v=v, cx=cx, cy=cy, pix=pix, born=born, c_aber=c_aber,
+                  deroul=deroul)
     return Case(name, g, acq, params)
 
 

@@ -103,6 +103,46 @@ def test_crop_to_field_modes(gpu_ctx, name, born, c_aber):
     assert e32 < TOL
 
 
+@pytest.mark.parametrize("name,born", [("small", 1), ("small", 0), ("u16", 0), ("cfg1", 1),
+                                       ("cfg1", 0)])
+def test_crop_to_field_volkov(gpu_ctx, name, born):
+    """DEROUL 1 / VOLKOV 1: the wrapped phase is unwrapped
+    (deroul_volkov4_total_sym_paire) before the fit (main.cpp:260-277). With
+    Rytov (BORN 0) the unwrapped phase is the field's imaginary part."""
+    import torch
+    case = make_case(name, born=born, deroul=1)
+    _say(f"crop_to_field volkov case={name} born={born}")
+    g = case.lib_geom()
+    crops32, pos32, fields32 = oracle_front(case, "f32")
+    _, _, fields64 = oracle_front(case, "f64")
+    A, C = crops32.shape[0], g.C
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.crop_to_field(g, _t(crops32), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), pos32)
+    got = fields.cpu().numpy()
+    e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
+    _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}  (f32 oracle vs f64 "
+         f"{rel_l2(fields32, fields64):.3e})")
+    assert e32 < TOL
+
+
+def test_herraez_is_unsupported(gpu_ctx):
+    import mtd_transmission_amd as m
+    import torch
+    case = make_case("small")
+    p = case.params
+    g = m.geom_derive(p["n"], p["nxmax"], p["v"], p["cx"], p["cy"], 1.2, 1.333, 6.33e-7,
+                      p["pix"], deroul=2)
+    A, C = 2, g.C
+    crops = _empty((A, C, C), torch.complex64)
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    with pytest.raises(m.MTDError, match="Herraez"):
+        gpu_ctx.crop_to_field(g, crops, case.acq.mask, pos, fields)
+
+
 @pytest.mark.parametrize("name", ["nomask", "small", "cfg1"])
 def test_preprocess(gpu_ctx, name):
     import torch
@@ -292,8 +332,8 @@ def test_invalid_geometry_errors(gpu_ctx):
         gpu_ctx.ewald_scatter(g, _empty((1, g.C, g.C), torch.complex64),
                               _empty((2,), torch.int32), vol, cnt)
     g = case.lib_geom()
-    g.deroul = 1
-    with pytest.raises(m.MTDError, match="DEROUL"):
+    g.deroul = 3  # not an mtd_deroul value
+    with pytest.raises(m.MTDError, match="deroul must be"):
         gpu_ctx.holo_to_crop(g, _t(case.acq.holos), _empty((6, g.C, g.C), torch.complex64))
     g = case.lib_geom()
     g.circle_cx = 2  # crop window leaves the hologram

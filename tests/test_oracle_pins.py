@@ -374,3 +374,67 @@ def test_finalize_parseval():
     lhs = (np.abs(idx) ** 2).sum()
     rhs = cte ** 2 * g.delta_f ** 6 * V ** 3 * (np.abs(t) ** 2).sum()
     assert lhs == pytest.approx(rhs, rel=1e-10)
+
+
+# --------------------------------------------------------------------------
+# phase unwrapping (DEROUL 1, VOLKOV 1)
+# --------------------------------------------------------------------------
+def test_kvect_and_mirror_known_answers():
+    assert np.array_equal(ref.kvect_shift(8),
+                          np.array([0, 1, 2, 3, -4, -3, -2, -1], dtype=np.float64) / 8)
+    img = np.array([[1.0, 2.0], [3.0, 4.0]])
+    # hand-written 4x4 even extension; the original is the lower-right quadrant
+    want = np.array([[4, 3, 3, 4],
+                     [2, 1, 1, 2],
+                     [2, 1, 1, 2],
+                     [4, 3, 3, 4]], dtype=np.float64)
+    assert np.array_equal(ref.symmetrise_mirror(img), want)
+
+
+def test_spectral_gradient_closed_form():
+    D = 32
+    y, x = np.mgrid[0:D, 0:D]
+    f = np.cos(2 * np.pi * 3 * x / D) * np.cos(2 * np.pi * 2 * y / D) + 0.5 * np.sin(2 * np.pi * 5 * y / D)
+    gx, gy = ref.gradient_fft(f)
+    ex = -2 * np.pi * 3 / D * np.sin(2 * np.pi * 3 * x / D) * np.cos(2 * np.pi * 2 * y / D)
+    ey = (-2 * np.pi * 2 / D * np.cos(2 * np.pi * 3 * x / D) * np.sin(2 * np.pi * 2 * y / D)
+          + 0.5 * 2 * np.pi * 5 / D * np.cos(2 * np.pi * 5 * y / D))
+    assert np.abs(gx - ex).max() < 1e-12
+    assert np.abs(gy - ey).max() < 1e-12
+    # a complex input differentiates term by term
+    z = np.exp(1j * 2 * np.pi * 4 * x / D)
+    zx, zy = ref.gradient_fft(z)
+    assert np.abs(zx - 1j * 2 * np.pi * 4 / D * z).max() < 1e-12
+    assert np.abs(zy).max() < 1e-12
+
+
+def test_integration_inverts_the_gradient():
+    D = 32
+    y, x = np.mgrid[0:D, 0:D]
+    f = (np.cos(2 * np.pi * 3 * x / D) * np.cos(2 * np.pi * 2 * y / D)
+         + 0.5 * np.sin(2 * np.pi * 5 * y / D) + 0.25 * np.cos(2 * np.pi * (x + 2 * y) / D))
+    gx, gy = ref.gradient_fft(f)
+    r = ref.integ_grad(gx.real, gy.real)
+    assert np.abs(r.real - (f - f.mean())).max() < 1e-12
+    assert np.abs(r.imag).max() < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["bowl", "ramp", "bump"])
+def test_volkov_recovers_smooth_phases(kind):
+    C = 64
+    y, x = np.mgrid[0:C, 0:C].astype(np.float64)
+    true = {"bowl": 0.004 * ((x - 30) ** 2 + (y - 36) ** 2),
+            "ramp": 0.3 * x + 0.2 * y,
+            "bump": 12 * np.exp(-((x - 30) ** 2 + (y - 34) ** 2) / (2 * 9.0 ** 2))}[kind]
+    assert true.max() - true.min() > 3 * np.pi  # several wraps
+    wrapped = np.angle(np.exp(1j * true))
+    for prec in ("f64", "f32"):
+        u = ref.volkov_unwrap(wrapped, prec).astype(np.float64)
+        d = u - true
+        d -= d.mean()  # unwrapping is defined up to a constant
+        assert np.abs(d).max() < 2e-3, (kind, prec, np.abs(d).max())
+    # without wraps the 2 pi multiples vanish
+    small = 0.4 * true / (true.max() - true.min())
+    u = ref.volkov_unwrap(small)
+    d = u - small
+    assert np.abs(d - d.mean()).max() < 1e-3
