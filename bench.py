@@ -130,6 +130,22 @@ def cpu_baseline(holos_dev, mask, sample):
     return r
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
+    """Device-to-device copy of nbytes (read + write): the achievable HBM
+    figure on this box, and the reference kernel for counter scaling."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del src, dst
+    return {"bytes_per_copy": 2 * nbytes, "GBps": round(2 * nbytes / dt / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,9 +155,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16,
                     help="holograms timed for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-child", nargs=2, help=argparse.SUPPRESS)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 3 device copies of 1 GiB (reference kernel for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
     if args.cpu_child:
         cpu_baseline_child(*args.cpu_child)
+        return
+    if args.copy_probe:
+        import torch
+        print(json.dumps(copy_bandwidth(torch, reps=3)))
         return
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -249,6 +272,8 @@ def main():
                           f"pocketfft, 1 thread"),
                "ms_per_volume": round(cpu_ms_volume, 1)}
 
+    hbm = copy_bandwidth(torch)
+
     out = {
         "metric": METRIC, "value": round(A / (ms / 1e3), 2), "unit": "holograms/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -262,6 +287,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
+        "hbm_copy": hbm,
         "kernels": per_kernel,
     }
     print(json.dumps(out))

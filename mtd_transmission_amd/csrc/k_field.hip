@@ -160,6 +160,8 @@ k_lsq(const float2* __restrict__ iu, const float* __restrict__ phu,
   double rp[NCOEF], ra[NCOEF];
 #pragma unroll
   for (int k = 0; k < NCOEF; ++k) { rp[k] = 0.0; ra[k] = 0.0; }
+  // unrolled so that several (sample, IU) load chains are in flight
+#pragma unroll 4
   for (int m = threadIdx.x; m < M; m += 256) {
     const int2 s = samples[m];
     const float2 v = u[((s.y + C / 2) % C) * C + (s.x + C / 2) % C];

@@ -13,6 +13,7 @@
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
 #include "fft_reg.hpp"
 #include "internal.hpp"
+#include "passes.hpp"
 
 namespace mtd {
 
@@ -35,9 +36,10 @@ k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
   constexpr int NP = holo_pairs<N>();
   static_assert(F::LDS >= N, "transpose buffer must hold a full row");
   __shared__ float2 lds[NP * F::LDS];
-  const int a = blockIdx.y;
+  const int2 ab = xcd_block();
+  const int a = ab.x;
   const int p = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
-  const int y0 = 2 * (blockIdx.x * NP + p), y1 = y0 + 1;
+  const int y0 = 2 * (ab.y * NP + p), y1 = y0 + 1;
   const TIn* h0 = holos + ((size_t)a * N + y0) * N;
   const TIn* h1 = h0 + N;
   const double wy0 = win[y0], wy1 = win[y1];
@@ -87,7 +89,8 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
   __shared__ float2 lds[NC * F::LDS];
-  const int a = blockIdx.y, d0 = blockIdx.x * NC;
+  const int2 ab = xcd_block();
+  const int a = ab.x, d0 = ab.y * NC;
   const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
   const float2* src = t1 + (size_t)a * N * C + d0 + c;
   float2 v[F::L1], u[F::L2];

@@ -135,9 +135,10 @@ k_grad_rows(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
   using F = Fft2<D, false>;
   constexpr int NT = rows_nt<D>();
   __shared__ float2 lds[NT * F::LDS];
-  const int a = blockIdx.y;
+  const int2 ab = xcd_block();
+  const int a = ab.x;
   const int t = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
-  const int r = blockIdx.x * NT + t;
+  const int r = ab.y * NT + t;
   const float2* iu_a = iu + (size_t)a * C * C;
   const int ys = mirror_src<C>(r);
   auto pos = [ys](int i, int& y, int& x) { y = ys; x = mirror_src<C>(i); };
@@ -158,9 +159,10 @@ k_grad_cols(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
   using F = Fft2<D, false>;
   constexpr int NC = cols_nc<D>();
   __shared__ float2 lds[NC * F::LDS];
-  const int a = blockIdx.y;
+  const int2 ab = xcd_block();
+  const int a = ab.x;
   const int cl = threadIdx.x % NC, n2 = threadIdx.x / NC;
-  const int c = blockIdx.x * NC + cl;
+  const int c = ab.y * NC + cl;
   const float2* iu_a = iu + (size_t)a * C * C;
   const int xs = mirror_src<C>(c);
   auto pos = [xs](int i, int& y, int& x) { y = mirror_src<C>(i); x = xs; };

@@ -68,45 +68,54 @@ constexpr int GT = 16;  // tile side (columns per block = 256)
 // Windowed gather. A prepass lists, per 16x16 column tile and in angle order,
 // the angles whose pupil disc touches the tile, with a conservative height
 // range derived from the tile box's nearest and farthest pupil radii (h is
-// decreasing in r^2 and round is monotonic). The main kernel gives each
-// block one (tile, ZW-height window); angles whose range misses the window
-// are skipped with scalar tests, so each hit is tested in about
-// 1 + span/ZW windows instead of every window of the band.
+// decreasing in r^2 and round is monotonic), and each column's exact hit
+// height per listed angle as one byte. The main kernel gives each block one
+// (tile, ZW-height window); a wave tests 64 list entries against the window
+// at once, so each hit is visited in about 1 + span/ZW windows. Near the
+// lateral origin nearly every angle hits the same few voxels, and those
+// sums are serial by definition (angle order): the loads of kBatch hits are
+// issued together so that only the additions stay serial.
 // ---------------------------------------------------------------------------
 struct TileEntry {
   int a;
   int zpack;  // zlo | zhi << 16, absolute heights
+  int fxy;    // fxm0 & 0xffff | fym0 << 16 (signed 16-bit each)
+  int pad;
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Height of angle a's hit in column q, as an offset from the entry's lower
-// bound zl (255 = no hit). Exact: FP64 heights and round-half-away.
-__device__ __forceinline__ uint8_t hit_offset(const GatherArgs& g, const AngleGeo& ag, int qx,
-                                              int qy, int zl) {
-  const int fdx = qx + ag.fx, fdy = qy + ag.fy;
-  if (fdx < -g.nx || fdx >= g.nx || fdy < -g.nx || fdy >= g.nx) return 255;
-  const int r2 = fdx * fdx + fdy * fdy;
-  if (r2 > g.r2max) return 255;
-  const int z = (int)round_away(g.htab[r2] - ag.fz) + g.V / 2;
-  return (uint8_t)(z - zl);
-}
+constexpr int kBatch = 8;  // hits whose loads are issued together
 
+// Prepass, one block per 16x16 column tile. Angles are taken 256 at a time;
+// the active ones (pupil disc touches the tile) are appended in angle order
+// by a block scan, with a conservative height range from the tile box's
+// nearest and farthest pupil radii (h decreases with r^2, round is
+// monotonic). Right after each scan the new entries are staged in LDS and
+// every column computes its exact hit height for each of them, stored as a
+// byte offset from the entry's lower bound (255 = no hit): FP64 heights and
+// round-half-away, as the reference.
 __global__ void __launch_bounds__(256)
 k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo,
               uint8_t* __restrict__ zoff) {
   __shared__ int wsum[4];
   __shared__ int zr[2][4];
+  __shared__ int4 sent[256];   // a, zl, fx, fy of this scan's entries
+  __shared__ double sfz[256];
   const int tile = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int qx0 = (tile % ntx) * GT - g.V / 2, qy0 = (tile / ntx) * GT - g.V / 2;
+  const int qx = qx0 + tid % GT, qy = qy0 + tid / GT;  // this thread's column
+  TileEntry* lst = list + (size_t)tile * g.A;
+  uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
   int running = 0, zmin = 0x7fffffff, zmax = -1;
   for (int a0 = 0; a0 < g.A; a0 += 256) {
     const int a = a0 + tid;
     bool act = false;
     int zl = 0, zh = 0;
+    AngleGeo ag{};
     if (a < g.A) {
-      const AngleGeo ag = g.ageo[a];
+      ag = g.ageo[a];
       if (!ag.skip) {
         const int cxl = max(qx0 + ag.fx, -g.nx), cxh = min(qx0 + GT - 1 + ag.fx, g.nx - 1);
         const int cyl = max(qy0 + ag.fy, -g.nx), cyh = min(qy0 + GT - 1 + ag.fy, g.nx - 1);
@@ -129,15 +138,50 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
     const int before = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) wsum[wave] = __popcll(bal);
     __syncthreads();
-    int off = running;
+    int off = 0;
     for (int w = 0; w < wave; ++w) off += wsum[w];
+    const int n = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     if (act) {
-      list[(size_t)tile * g.A + off + before] = TileEntry{a, zl | (zh << 16)};
+      const int p = off + before;
+      lst[running + p] = TileEntry{a, zl | (zh << 16), (ag.fx & 0xffff) | (ag.fy << 16), 0};
+      sent[p] = make_int4(a, zl, ag.fx, ag.fy);
+      sfz[p] = ag.fz;
       zmin = min(zmin, zl);
       zmax = max(zmax, zh);
     }
-    running += wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
+    // exact hit heights of this thread's column for the n new entries
+    for (int i0 = 0; i0 < n; i0 += kBatch) {
+      double hb[kBatch];
+      int r2b[kBatch];
+#pragma unroll
+      for (int t = 0; t < kBatch; ++t) {
+        r2b[t] = -1;
+        hb[t] = 0.0;
+        if (i0 + t < n) {
+          const int4 e = sent[i0 + t];
+          const int fdx = qx + e.z, fdy = qy + e.w;
+          const int r2 = fdx * fdx + fdy * fdy;
+          if (fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max) {
+            r2b[t] = r2;
+            hb[t] = g.htab[r2];
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kBatch; ++t) {
+        if (i0 + t < n) {
+          uint8_t o = 255;
+          if (r2b[t] >= 0) {
+            const int z = (int)round_away(hb[t] - sfz[i0 + t]) + g.V / 2;
+            o = (uint8_t)(z - sent[i0 + t].y);
+          }
+          zo[(size_t)(running + i0 + t) * 256] = o;
+        }
+      }
+    }
+    running += n;
+    __syncthreads();  // sent/sfz/wsum are reused by the next scan
   }
   for (int o = 32; o > 0; o >>= 1) {
     zmin = min(zmin, __shfl_xor(zmin, o, 64));
@@ -149,16 +193,6 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
     const int lo = min(min(zr[0][0], zr[0][1]), min(zr[0][2], zr[0][3]));
     const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
     tinfo[tile] = make_int4(running, lo, hi, 0);
-  }
-  // per listed angle, each column's exact height offset (one byte)
-  __threadfence_block();
-  __syncthreads();
-  const int qx = qx0 + tid % GT, qy = qy0 + tid / GT;
-  const TileEntry* lst = list + (size_t)tile * g.A;
-  uint8_t* zo = zoff + (size_t)tile * g.A * 256;
-  for (int i = 0; i < running; ++i) {
-    const TileEntry e = lst[i];
-    zo[(size_t)i * 256 + tid] = hit_offset(g, g.ageo[e.a], qx, qy, e.zpack & 0xffff);
   }
 }
 
@@ -174,13 +208,22 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
   // here as plain operators (HIP's __fmul_rn/__fadd_rn are header functions
   // outside its scope and would still be contracted).
 #pragma clang fp contract(off)
-  __shared__ float2 acc[ZW][256];
-  __shared__ int cn[ZW][256];
-  // consecutive work items on one XCD (blocks are dealt round-robin to the 8 XCDs)
-  const int per = (items + 7) / 8;
-  const int item = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (item >= items) return;
-  const int tile = item / nwin, w = item % nwin;
+  // Blocks are dealt round-robin to the 8 XCDs. Tiles go round-robin too
+  // (the heavy tiles near the lateral origin are spread), and the windows
+  // of one tile stay on one XCD, in order (they read the same spectrum
+  // patches). A bijection when the tile count is a multiple of 8.
+  int tile, w;
+  {
+    const int b = blockIdx.x, tiles = items / nwin;
+    if (tiles % 8 == 0) {
+      const int n = b / 8;
+      tile = (n / nwin) * 8 + b % 8;
+      w = n % nwin;
+    } else {
+      tile = b / nwin;
+      w = b % nwin;
+    }
+  }
   const int tid = threadIdx.x;
   const int X = (tile % ntx) * GT + tid % GT, Y = (tile / ntx) * GT + tid / GT;
   const int qx = X - g.V / 2, qy = Y - g.V / 2;
@@ -200,19 +243,26 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
     }
     return;
   }
+  // this column's ZW partial sums live in registers: each hit is added to
+  // its height slot by a compile-time-indexed select
+  float2 acc[ZW];
+  int cn[ZW];
 #pragma unroll
-  for (int k = 0; k < ZW; ++k) { acc[k][tid] = make_float2(0.f, 0.f); cn[k][tid] = 0; }
+  for (int k = 0; k < ZW; ++k) { acc[k] = make_float2(0.f, 0.f); cn[k] = 0; }
   const TileEntry* lst = list + (size_t)tile * g.A;
   const uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
   // spectrum element read by this column: the unshifted frame holds it at
   // (qy mod C, qx mod C) for every angle; the +posSpec frame at fd + NXMAX
   const int frow = ((qy % g.C) + g.C) % g.C, fcol = ((qx % g.C) + g.C) % g.C;
-  // Each wave tests 64 entries against the window at once and then visits
-  // the overlapping ones lowest index first, which keeps the angle order.
+  const size_t fixed = (size_t)frow * g.C + fcol;
+  const size_t C2 = (size_t)g.C * g.C;
+  // Each wave tests 64 entries against the window at once, then takes the
+  // overlapping ones lowest index first (angle order) kBatch at a time: the
+  // loads of a batch are issued together, the sums applied in order.
   const int lane = tid & 63;
   for (int j0 = 0; j0 < info.x; j0 += 64) {
     const int j = j0 + lane;
-    TileEntry e{0, 0};
+    TileEntry e{0, 0, 0, 0};
     bool ov = false;
     if (j < info.x) {
       e = lst[j];
@@ -221,32 +271,57 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
     }
     unsigned long long m = __ballot(ov);  // wave-uniform
     while (m) {
-      const int bit = __ffsll((long long)m) - 1;
-      m &= m - 1;
-      const int i = j0 + bit;
-      const int ea = __shfl(e.a, bit, 64);
-      const int zl = __shfl(e.zpack, bit, 64) & 0xffff;
-      const int off = zo[(size_t)i * 256];
-      if (off == 255) continue;
-      const int k = zl + off - z0;
-      if (k < 0 || k >= ZW) continue;
-      const AngleGeo ag = g.ageo[ea];
-      const int fdx = qx + ag.fx, fdy = qy + ag.fy;
-      const size_t base = (size_t)ea * g.C * g.C;
-      const float2 sp = g.frame_pos ? g.spec[base + (size_t)(fdy + g.nx) * g.C + fdx + g.nx]
-                                    : g.spec[base + (size_t)frow * g.C + fcol];
-      const float b = g.beta[fdx * fdx + fdy * fdy];
-      const float pim = b * sp.y, pre = b * sp.x;  // value (-b Si, b Sr)
-      float2 s = acc[k][tid];
-      s.x = s.x + (-pim);
-      s.y = s.y + pre;
-      acc[k][tid] = s;
-      cn[k][tid] += 1;
+      int bitv[kBatch];
+#pragma unroll
+      for (int t = 0; t < kBatch; ++t) {
+        bitv[t] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+        m &= m - 1;  // m == 0 stays 0
+      }
+      int offv[kBatch], zlv[kBatch];
+      float2 spv[kBatch];
+      float bv[kBatch];
+#pragma unroll
+      for (int t = 0; t < kBatch; ++t) {
+        offv[t] = 255;
+        if (bitv[t] >= 0) {  // uniform
+          const int i = j0 + bitv[t];
+          const int ea = __builtin_amdgcn_readlane(e.a, bitv[t]);
+          zlv[t] = __builtin_amdgcn_readlane(e.zpack, bitv[t]) & 0xffff;
+          const int fxy = __builtin_amdgcn_readlane(e.fxy, bitv[t]);
+          const int fdx = qx + (int)(short)(fxy & 0xffff), fdy = qy + (fxy >> 16);
+          offv[t] = zo[(size_t)i * 256];
+          // addresses are clamped so that non-hits read in bounds; a hit has
+          // |fd| inside the crop and r2 <= r2max, where clamping is the identity
+          const int r2 = min(fdx * fdx + fdy * fdy, g.r2max);
+          const size_t si = g.frame_pos
+                                ? (size_t)(clampi(fdy, -g.nx, g.nx - 1) + g.nx) * g.C +
+                                      (clampi(fdx, -g.nx, g.nx - 1) + g.nx)
+                                : fixed;
+          spv[t] = g.spec[(size_t)ea * C2 + si];
+          bv[t] = g.beta[r2];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kBatch; ++t) {
+        const int k = offv[t] == 255 ? -1 : zlv[t] + offv[t] - z0;
+        const float pim = bv[t] * spv[t].y, pre = bv[t] * spv[t].x;  // (-b Si, b Sr)
+#pragma unroll
+        for (int kk = 0; kk < ZW; ++kk) {
+          if (k == kk) {
+            acc[kk].x = acc[kk].x + (-pim);
+            acc[kk].y = acc[kk].y + pre;
+            cn[kk] += 1;
+          }
+        }
+      }
     }
   }
-  for (int z = z0; z <= z1; ++z) {
-    const float2 s = acc[z - z0][tid];
-    const int c = cn[z - z0][tid];
+#pragma unroll
+  for (int k = 0; k < ZW; ++k) {
+    const int z = z0 + k;
+    if (z > z1) break;
+    const float2 s = acc[k];
+    const int c = cn[k];
     if (norm) {
       const float d = (float)(c == 0 ? 1 : c);
       float2 t = make_float2(s.x / d, s.y / d);
@@ -325,7 +400,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   }
   const int nwin = (V + ZW - 1) / ZW, items = tiles * nwin;
   Timed tm("ewald_gather", s);
-  hipLaunchKernelGGL((k_gather_win<ZW>), dim3(8 * ((items + 7) / 8)), dim3(256), 0, s, g,
+  hipLaunchKernelGGL((k_gather_win<ZW>), dim3(items), dim3(256), 0, s, g,
                      ntx, nwin, items, (const TileEntry*)list, (const int4*)tinfo,
                      (const uint8_t*)zoff, flags, band);
   return hipGetLastError();
@@ -334,7 +409,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
 size_t gather_scratch_bytes(int V, int A) {
   const size_t tiles = (size_t)(V / GT) * (V / GT);
   const size_t nA = (size_t)(A > 0 ? A : 1);
-  return tiles * nA * (sizeof(TileEntry) + 256) + tiles * sizeof(int4);
+  return tiles * nA * (sizeof(TileEntry) + 256) + tiles * sizeof(int4);  // 16 + 256 B per entry
 }
 
 // ---------------------------------------------------------------------------

@@ -30,6 +30,20 @@ constexpr int cols_nc() {
   return lines_per_block<L>();
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs in dispatch order (linear
+// id, x fastest). Renumber them so that XCD k runs the k-th eighth of the
+// line/column groups of every batch, in order: neighbouring column groups
+// share that XCD's L2 for the 128-byte lines they split, and every batch
+// (e.g. a z plane, some of which may be skipped) is spread over all XCDs.
+// A bijection when gridDim.x is a multiple of 8; the identity otherwise.
+__device__ __forceinline__ int2 xcd_block() {
+  const int gx = gridDim.x;
+  if (gx % 8) return make_int2(blockIdx.y, blockIdx.x);
+  const int id = blockIdx.x + blockIdx.y * gx;
+  const int per = gx / 8, n = id / 8;
+  return make_int2(n / per, (id % 8) * per + n % per);
+}
+
 // An operation may define skip(batch): the whole batch is then left alone
 // (neither read nor written) when it returns true.
 template <class Op, class = void>
@@ -49,10 +63,11 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
   using F = Fft2<L, INV>;
   constexpr int NT = rows_nt<L>();
   __shared__ float2 lds[NT * F::LDS];
-  const int a = blockIdx.y;
+  const int2 ab = xcd_block();
+  const int a = ab.x;
   if (skip_batch(op, a)) return;  // block-uniform, before the barrier
   const int r = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
-  const int line = blockIdx.x * NT + r;
+  const int line = ab.y * NT + r;
   op.begin(a);
   float2 v[F::L1], u[F::L2];
 #pragma unroll
@@ -69,7 +84,8 @@ __global__ void __launch_bounds__(256) k_cols(Op op) {
   using F = Fft2<L, INV>;
   constexpr int NC = cols_nc<L>();
   __shared__ float2 lds[NC * F::LDS];
-  const int a = blockIdx.y, c0 = blockIdx.x * NC;
+  const int2 ab = xcd_block();
+  const int a = ab.x, c0 = ab.y * NC;
   if (skip_batch(op, a)) return;  // block-uniform, before the barrier
   const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
   op.begin(a);
