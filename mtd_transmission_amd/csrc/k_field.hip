@@ -214,6 +214,19 @@ struct FieldMode {
   int c_aber;
 };
 
+// The fitted background sum_k c_k x^i y^j (compuBackgr2,
+// Correction_aberration.cpp:151-164; k runs i-outer over i + j <= 4) in
+// Horner form: sum_i x^i P_i(y), P_i(y) = sum_j c_(i,j) y^j. Along a row y is
+// fixed, so the P_i(y) are common to all its pixels.
+__device__ __forceinline__ double poly_eval(const double* __restrict__ c, double x, double y) {
+  const double p0 = (((c[4] * y + c[3]) * y + c[2]) * y + c[1]) * y + c[0];
+  const double p1 = ((c[8] * y + c[7]) * y + c[6]) * y + c[5];
+  const double p2 = (c[11] * y + c[10]) * y + c[9];
+  const double p3 = c[13] * y + c[12];
+  const double p4 = c[14];
+  return (((p4 * x + p3) * x + p2) * x + p1) * x + p0;
+}
+
 // phu_a: unwrapped phase of this angle (U frame, row length C) or nullptr
 __device__ __forceinline__ float2 form_pixel(float2 u, int X, int Y,
                                              const double* __restrict__ cph,
@@ -226,14 +239,8 @@ __device__ __forceinline__ float2 form_pixel(float2 u, int X, int Y,
   }
   const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
   const float amf = hypotf(u.x, u.y);
-  double mono[NCOEF];
-  monomials((double)X, (double)Y, mono);
-  double bgp = 0.0, bga = 0.0;
-#pragma unroll
-  for (int k = 0; k < NCOEF; ++k) {
-    bgp += cph[k] * mono[k];
-    bga += cam[k] * mono[k];
-  }
+  const double bgp = poly_eval(cph, (double)X, (double)Y);
+  const double bga = poly_eval(cam, (double)X, (double)Y);
   const double phic = (double)phf - bgp;
   double A = (double)amf / (bga + 0.00000001);
   if (A > 10) A = 1;

@@ -408,7 +408,32 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
   return MTD_OK;
 }
 
-static const size_t kChunkBudget = 128ull << 20;  // per-chunk hologram-stage scratch
+// Hologram stage: the N x C row-pass intermediate of a chunk stays within
+// the 256 MiB Infinity Cache. Field stage: about 1 GiB of per-angle buffers
+// per launch, so that its small kernels get hundreds of angles at once.
+static const size_t kChunkBudget = 128ull << 20;
+static const size_t kFieldBudget = 1ull << 30;
+
+// device bytes per angle of the field stage (crop, work, IU, and with
+// unwrapping its scratch and the phase)
+static size_t field_bytes(const mtd_geom* g) {
+  const size_t C = 2 * (size_t)g->nxmax;
+  size_t b = 3 * C * C * sizeof(float2);
+  if (g->c_aber && g->deroul == MTD_DEROUL_VOLKOV)
+    b += unwrap_scratch_bytes((int)C, 1) + C * C * sizeof(float);
+  return b;
+}
+
+// holograms [f0, f0+n) -> crops, in hologram-stage chunks
+static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
+                      int n, float2* crop) {
+  const int N = g->n, C = 2 * g->nxmax;
+  const int hc = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
+  for (int h0 = 0; h0 < n; h0 += hc)
+    MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
+                       crop + (size_t)h0 * C * C));
+  return MTD_OK;
+}
 
 }  // namespace mtd
 
@@ -564,7 +589,7 @@ int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A
   if (!crops || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int C = 2 * g->nxmax;
   if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
-  const int chunk = chunk_of((size_t)C * C * sizeof(float2) * 2, kChunkBudget, A);
+  const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(front_chunk(ctx, g, (const float2*)crops + (size_t)a0 * C * C, n, posspec + a0,
@@ -579,14 +604,14 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
   MTD_TRY(validate(ctx, g, A, NEED_HOLO));
   if (A == 0) return MTD_OK;
   if (!holos || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
-  const int N = g->n, C = 2 * g->nxmax;
+  const int C = 2 * g->nxmax;
   if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
-  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, A);
+  const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   if (!crop) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop));
     MTD_TRY(front_chunk(ctx, g, crop, n, posspec + a0, posspec + A + a0,
                         (float2*)fields + (size_t)a0 * C * C, nullptr));
   }
@@ -602,7 +627,7 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   const int C = 2 * g->nxmax;
   const float2* twC = twiddle(ctx, C);
   const double* winC = tukey_table(ctx, C, g->alpha_field);
-  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kChunkBudget, A);
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, A);
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
   if (!twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
@@ -662,7 +687,7 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
   const float2* twC = twiddle(ctx, C);
   const double* winC = tukey_table(ctx, C, g->alpha_field);
-  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kChunkBudget, std::max(A, 1));
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, std::max(A, 1));
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
   if (!F || !twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
@@ -710,7 +735,7 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_HOLO | NEED_VOL));
   if (!index_re || !index_im || (A > 0 && !holos)) return set_err(ctx, MTD_ERR_ARG, "null array");
-  const int N = g->n, C = 2 * g->nxmax, V = g->dim_final;
+  const int C = 2 * g->nxmax, V = g->dim_final;
   if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
   int32_t* pos = posspec;
   if (!pos) {
@@ -718,14 +743,14 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
     if (!pos) return set_err(ctx, MTD_ERR_HIP, "out of device memory (posspec)");
   }
   float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
-  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, std::max(A, 1));
+  const int chunk = chunk_of(field_bytes(g), kFieldBudget, std::max(A, 1));
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
   int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
   if (!F || !crop || !vol || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop));
     MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
                         F + (size_t)a0 * C * C));
   }
