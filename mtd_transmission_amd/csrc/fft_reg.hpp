@@ -6,7 +6,7 @@
 //           loaded straight from global memory: coalesced across threads),
 //           and does an L1-point DFT in registers;
 //   twiddle Y[n2][k1] *= W_L^(n2*k1) (table);
-//   exchange one padded LDS transpose;
+//   exchange one LDS transpose (real parts, then imaginary parts);
 //   step 2  thread k1 < L1 does the L2-point DFT over n2 in registers and
 //           holds X[k1 + L1*k2], k2 < L2 (stored straight to global memory).
 // The in-register DFTs are mixed-radix decimation in time, fully unrolled at
@@ -144,21 +144,58 @@ __host__ __device__ constexpr int fft2_l1(int L) { return L / fft2_l2(L); }
 // static LDS per workgroup stays within 64 KiB
 constexpr int kLdsBlockMax = 65536;
 
-// LDS floats2 needed per transform (padded transpose)
-__host__ __device__ constexpr int fft2_lds(int L) { return fft2_l1(L) * (fft2_l2(L) + 1); }
+// LDS floats per transform of the two exchange layouts (see Fft2)
+__host__ __device__ constexpr int fft2_lds_row(int L) { return fft2_l1(L) * (fft2_l2(L) + 1); }
+__host__ __device__ constexpr int fft2_lds_col(int L) { return L; }
+
+// Exchange storage of one transform, seen as an L1 x L2 matrix (at) or as a
+// line of L values (lin). Rows: a padded L1 x (L2+1) block per transform,
+// transforms one after the other (the lanes of a transform are adjacent).
+template <int L1, int L2>
+struct RowX {
+  float* p;
+  __device__ __forceinline__ float* at(int i, int j) const { return p + i * (L2 + 1) + j; }
+  __device__ __forceinline__ float* lin(int k) const { return p + k; }
+};
+// Columns: the NC transforms of a block interleaved (column fastest, as the
+// lanes are). A wave spans G = 64/NC consecutive matrix rows (or threads of a
+// transform); XOR-swizzling the column index with the row index mod G puts
+// those G rows on distinct groups of NC banks, for the transposing writes
+// and reads alike, while keeping only G distinct runtime addresses (the rest
+// are immediate offsets). No swizzle where G does not fit (correct, slower).
+template <int L2, int NC>
+constexpr int colx_group() {
+  return (NC <= 64 && 64 % NC == 0 && L2 % (64 / NC) == 0) ? 64 / NC : 1;
+}
+template <int L1, int L2, int NC>
+struct ColX {
+  static constexpr int G = colx_group<L2, NC>();
+  float* p;  // base + column
+  __device__ __forceinline__ float* at(int i, int j) const {
+    return p + (i * L2 + (j ^ (i & (G - 1)))) * NC;
+  }
+  __device__ __forceinline__ float* lin(int k) const { return p + k * NC; }
+};
 
 template <int L, bool INV>
 struct Fft2 {
   static constexpr int L1 = fft2_l1(L);
   static constexpr int L2 = fft2_l2(L);
   static constexpr int T = L2;  // threads per transform
-  static constexpr int LDS = fft2_lds(L);
+  static constexpr int LDS_ROW = fft2_lds_row(L);
+  static constexpr int LDS_COL = fft2_lds_col(L);
+  template <int NC>
+  using Col = ColX<L1, L2, NC>;
+  using Row = RowX<L1, L2>;
 
   // v[n1] = x[L2*n1 + n2] on entry (thread n2). On exit, threads n2 < L1
-  // hold u[k2] = X[n2 + L1*k2]. lds: LDS floats2 of this transform.
-  // Contains one barrier: every thread of the workgroup must call it.
-  __device__ __forceinline__ static void run(float2 (&v)[L1], float2 (&u)[L2], int n2,
-                                             float2* lds, const float2* __restrict__ tw) {
+  // hold u[k2] = X[n2 + L1*k2]. x: this transform's exchange storage. The
+  // transpose moves the real parts, then the imaginary parts, through a
+  // float buffer (half the LDS of a complex one). Contains three barriers:
+  // every thread of the workgroup must call it.
+  template <class X>
+  __device__ __forceinline__ static void run(float2 (&v)[L1], float2 (&u)[L2], int n2, X x,
+                                             const float2* __restrict__ tw) {
     if constexpr (L1 > 1) RegFFT<L1, INV>::run(v);
 #pragma unroll
     for (int k1 = 1; k1 < L1; ++k1) {
@@ -167,14 +204,49 @@ struct Fft2 {
       v[k1] = cmul(v[k1], w);
     }
 #pragma unroll
-    for (int k1 = 0; k1 < L1; ++k1) lds[k1 * (L2 + 1) + n2] = v[k1];
+    for (int k1 = 0; k1 < L1; ++k1) *x.at(k1, n2) = v[k1].x;
     __syncthreads();
     if (n2 < L1) {
 #pragma unroll
-      for (int n = 0; n < L2; ++n) u[n] = lds[n2 * (L2 + 1) + n];
+      for (int n = 0; n < L2; ++n) u[n].x = *x.at(n2, n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < L1; ++k1) *x.at(k1, n2) = v[k1].y;
+    __syncthreads();
+    if (n2 < L1) {
+#pragma unroll
+      for (int n = 0; n < L2; ++n) u[n].y = *x.at(n2, n);
       RegFFT<L2, INV>::run(u);
     }
   }
 };
+
+// Moves a line from the output layout of Fft2::run (threads k1 < L1 hold
+// u[k2] at position k1 + L1*k2) to its input layout (thread n2 holds v[n1]
+// at position L2*n1 + n2), one component at a time. Starts and ends with a
+// barrier, so the storage is free on entry and on exit.
+template <int L, class X>
+__device__ __forceinline__ void fft2_relayout(const float2 (&u)[Fft2<L, false>::L2],
+                                              float2 (&v)[Fft2<L, false>::L1], int n2, X x) {
+  constexpr int L1 = Fft2<L, false>::L1, L2 = Fft2<L, false>::L2;
+  __syncthreads();
+  if (n2 < L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < L2; ++k2) *x.lin(n2 + L1 * k2) = u[k2].x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < L1; ++n1) v[n1].x = *x.lin(L2 * n1 + n2);
+  __syncthreads();
+  if (n2 < L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < L2; ++k2) *x.lin(n2 + L1 * k2) = u[k2].y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < L1; ++n1) v[n1].y = *x.lin(L2 * n1 + n2);
+  __syncthreads();
+}
 
 }  // namespace mtd

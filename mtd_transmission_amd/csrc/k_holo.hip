@@ -17,12 +17,9 @@
 
 namespace mtd {
 
-// LDS per block kept near 36 KiB so that four blocks fit a CU
-constexpr int kHoloLdsBudget = 36864;
-
 template <int N>
-constexpr int holo_pairs() {  // row pairs per block
-  constexpr int by_lds = kHoloLdsBudget / (Fft2<N, false>::LDS * 8);
+constexpr int holo_pairs() {  // row pairs per block: <= 256 threads, <= 64 KiB
+  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS_ROW * 4);
   constexpr int cap = 256 / Fft2<N, false>::T;
   return divisor_le(N / 2, by_lds < cap ? (by_lds > 0 ? by_lds : 1) : cap);
 }
@@ -34,8 +31,8 @@ k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
             float2* __restrict__ t1) {
   using F = Fft2<N, false>;
   constexpr int NP = holo_pairs<N>();
-  static_assert(F::LDS >= N, "transpose buffer must hold a full row");
-  __shared__ float2 lds[NP * F::LDS];
+  static_assert(F::LDS_ROW >= N, "exchange storage must hold a full row");
+  __shared__ float lds[NP * F::LDS_ROW];
   const int2 ab = xcd_block();
   const int a = ab.x;
   const int p = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
@@ -54,41 +51,58 @@ k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
     if ((x + y0) & 1) g0 = -g0; else g1 = -g1;  // centring (-1)^(x+y)
     v[n1] = make_float2(g0, g1);
   }
-  float2* buf = lds + p * F::LDS;
-  F::run(v, u, n2, buf, tw);
-  __syncthreads();  // every thread has read its transposed inputs
-  if (n2 < F::L1) {
+  float* buf = lds + p * F::LDS_ROW;
+  F::run(v, u, n2, typename F::Row{buf}, tw);
+  // Separation of the two rows, for k = x0 + d, m = N - k:
+  //   X0 = Z[k] + conj(Z[m]) = (Zk.x + Zm.x, Zk.y - Zm.y)
+  //   X1 = -i (Z[k] - conj(Z[m])) = (Zk.y + Zm.y, Zm.x - Zk.x)
+  // (the 1/2 is folded into the column pass scale). The row goes through the
+  // float storage twice: real parts give X0.x and X1.y, imaginary parts
+  // X0.y and X1.x.
+  float* o0 = (float*)(t1 + ((size_t)a * N + y0) * C);
+  float* o1 = o0 + 2 * C;
 #pragma unroll
-    for (int k2 = 0; k2 < F::L2; ++k2) buf[n2 + F::L1 * k2] = u[k2];
-  }
-  __syncthreads();
-  float2* o0 = t1 + ((size_t)a * N + y0) * C;
-  float2* o1 = o0 + C;
-  for (int d = n2; d < C; d += F::T) {
-    const int k = x0 + d;
-    const int m = (N - k) % N;
-    const float2 zk = buf[k], zm = buf[m];
-    // X0 = Z[k] + conj(Z[N-k]), X1 = -i (Z[k] - conj(Z[N-k])); the 1/2 is
-    // folded into the column pass scale.
-    o0[d] = make_float2(zk.x + zm.x, zk.y - zm.y);
-    o1[d] = make_float2(zk.y + zm.y, zm.x - zk.x);
+  for (int part = 0; part < 2; ++part) {
+    __syncthreads();  // the storage is free
+    if (n2 < F::L1) {
+#pragma unroll
+      for (int k2 = 0; k2 < F::L2; ++k2) buf[n2 + F::L1 * k2] = part ? u[k2].y : u[k2].x;
+    }
+    __syncthreads();
+    for (int d = n2; d < C; d += F::T) {
+      const int k = x0 + d;
+      const int m = (N - k) % N;
+      const float zk = buf[k], zm = buf[m];
+      if (part == 0) {
+        o0[2 * d] = zk + zm;
+        o1[2 * d + 1] = zm - zk;
+      } else {
+        o0[2 * d + 1] = zk - zm;
+        o1[2 * d] = zk + zm;
+      }
+    }
   }
 }
 
 template <int N>
 constexpr int holo_cols_nc() {  // a power of two <= 16: divides every crop side
-  constexpr int by_thr = 256 / Fft2<N, false>::T;
-  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS * 8);
+  constexpr int by_thr = (N > 512 ? 256 : 1024) / Fft2<N, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS_COL * 4);
   return divisor_le(16, by_thr < by_lds ? by_thr : by_lds);
 }
 
 template <int N>
-__global__ void __launch_bounds__(256)
+constexpr int holo_cols_threads() {
+  return holo_cols_nc<N>() * Fft2<N, false>::T;
+}
+
+template <int N>
+__global__ void __launch_bounds__(holo_cols_threads<N>(), col_waves_per_simd<N>())
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
             int C, int x0, int y0, float scale, float2* __restrict__ crop) {
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
-  __shared__ float2 lds[NC * F::LDS];
+  __shared__ float lds[NC * F::LDS_COL];
   const int2 ab = xcd_block();
   const int a = ab.x, d0 = ab.y * NC;
   const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
@@ -96,7 +110,7 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = src[(size_t)(F::L2 * n1 + n2) * C];
-  F::run(v, u, n2, lds + c * F::LDS, tw);
+  F::run(v, u, n2, typename F::template Col<NC>{lds + c}, tw);
   if (n2 < F::L1) {
     float2* dst = crop + (size_t)a * C * C + d0 + c;
     const int kx = x0 + d0 + c;
@@ -143,7 +157,7 @@ static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
   if (C % NC) return hipErrorInvalidValue;
   const float scale = (float)(0.5 / ((double)N * (double)N));
   Timed tm("holo_cols", s);
-  hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(NC * Fft2<N, false>::T), 0, s,
+  hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(holo_cols_threads<N>()), 0, s,
                      t1, tw, C, x0, y0, scale, crop);
   return hipGetLastError();
 }

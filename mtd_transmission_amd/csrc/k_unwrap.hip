@@ -50,40 +50,35 @@ __device__ __forceinline__ float kfreq(int k) {
 
 // d/dx of one line held in the two-step layout: v (input layout) in, u
 // (threads n2 < L1 hold positions n2 + L1*k2) out. Forward FFT, times
-// 2 pi i k / D, LDS exchange back to the input layout, backward FFT.
-template <int D>
-__device__ __forceinline__ void line_derivative(float2* lds, const float2* __restrict__ tw, int n2,
+// 2 pi i k / D, back to the input layout through the exchange storage x,
+// backward FFT.
+template <int D, class X>
+__device__ __forceinline__ void line_derivative(X x, const float2* __restrict__ tw, int n2,
                                                 float2 (&v)[Fft2<D, false>::L1],
                                                 float2 (&u)[Fft2<D, false>::L2]) {
   using F = Fft2<D, false>;
   using B = Fft2<D, true>;
-  static_assert(F::LDS >= D, "exchange buffer must hold a full line");
   const float two_pi = (float)(2.0 * 3.14159265358979323846);
-  F::run(v, u, n2, lds, tw);
-  __syncthreads();
+  F::run(v, u, n2, x, tw);
   if (n2 < F::L1) {
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) {
-      const int k = n2 + F::L1 * k2;
-      const float f = two_pi * kfreq<D>(k) / (float)D;
-      lds[k] = make_float2(-f * u[k2].y, f * u[k2].x);
+      const float f = two_pi * kfreq<D>(n2 + F::L1 * k2) / (float)D;
+      u[k2] = make_float2(-f * u[k2].y, f * u[k2].x);
     }
   }
-  __syncthreads();
-#pragma unroll
-  for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = lds[F::L2 * n1 + n2];
-  __syncthreads();
-  B::run(v, u, n2, lds, tw);
+  fft2_relayout<D>(u, v, n2, x);
+  B::run(v, u, n2, x, tw);
 }
 
 // Gradient along one line of the extension: at the outputs held by threads
 // n2 < L1 (positions n2 + L1*k2), g = (Im(dZ / Z) - Re(dphi_s)) / (2 pi),
 // Z = exp(i phi_s). `pos(i)` maps line position i to the (y, x) of the U
-// image; `lds` holds one transform. The two derivatives run one after the
-// other so that only the real part of the first stays live.
-template <int D, class Pos>
+// image; x is this line's exchange storage. The two derivatives run one
+// after the other so that only the real part of the first stays live.
+template <int D, class Pos, class X>
 __device__ __forceinline__ void line_gradient(const float2* __restrict__ iu_a, Pos pos, int n2,
-                                              float2* lds, const float2* __restrict__ tw,
+                                              X x, const float2* __restrict__ tw,
                                               float (&g)[Fft2<D, false>::L2]) {
   using F = Fft2<D, false>;
   constexpr int C = D / 2;
@@ -93,32 +88,32 @@ __device__ __forceinline__ void line_gradient(const float2* __restrict__ iu_a, P
     float2 v[F::L1], u[F::L2];
 #pragma unroll
     for (int n1 = 0; n1 < F::L1; ++n1) {
-      int y, x;
-      pos(F::L2 * n1 + n2, y, x);
-      v[n1] = make_float2(wrapped_phase<C>(iu_a, y, x), 0.f);
+      int y, xx;
+      pos(F::L2 * n1 + n2, y, xx);
+      v[n1] = make_float2(wrapped_phase<C>(iu_a, y, xx), 0.f);
     }
-    line_derivative<D>(lds, tw, n2, v, u);
+    line_derivative<D>(x, tw, n2, v, u);
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) dphi[k2] = u[k2].x;
   }
-  __syncthreads();  // the exchange buffer is reused
+  __syncthreads();  // the exchange storage is reused
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) {
-    int y, x;
-    pos(F::L2 * n1 + n2, y, x);
+    int y, xx;
+    pos(F::L2 * n1 + n2, y, xx);
     float zs, zc;
-    sincosf(wrapped_phase<C>(iu_a, y, x), &zs, &zc);
+    sincosf(wrapped_phase<C>(iu_a, y, xx), &zs, &zc);
     v[n1] = make_float2(zc, zs);
   }
-  line_derivative<D>(lds, tw, n2, v, u);
+  line_derivative<D>(x, tw, n2, v, u);
   if (n2 < F::L1) {
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) {
-      int y, x;
-      pos(n2 + F::L1 * k2, y, x);
+      int y, xx;
+      pos(n2 + F::L1 * k2, y, xx);
       float zi, zr;
-      sincosf(wrapped_phase<C>(iu_a, y, x), &zi, &zr);
+      sincosf(wrapped_phase<C>(iu_a, y, xx), &zi, &zr);
       // -i (dZ / Z): its real part is Im(dZ / Z)
       const float2 dz = u[k2];
       const float im_q = (dz.y * zr - dz.x * zi) / (zr * zr + zi * zi);
@@ -134,7 +129,7 @@ k_grad_rows(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
   constexpr int D = 2 * C;
   using F = Fft2<D, false>;
   constexpr int NT = rows_nt<D>();
-  __shared__ float2 lds[NT * F::LDS];
+  __shared__ float lds[NT * F::LDS_ROW];
   const int2 ab = xcd_block();
   const int a = ab.x;
   const int t = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
@@ -143,12 +138,21 @@ k_grad_rows(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
   const int ys = mirror_src<C>(r);
   auto pos = [ys](int i, int& y, int& x) { y = ys; x = mirror_src<C>(i); };
   float g[F::L2];
-  line_gradient<D>(iu_a, pos, n2, lds + t * F::LDS, tw, g);
+  line_gradient<D>(iu_a, pos, n2, typename F::Row{lds + t * F::LDS_ROW}, tw, g);
   if (n2 < F::L1) {
     float* o = gx + ((size_t)a * D + r) * D;
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) o[n2 + F::L1 * k2] = g[k2];
   }
+}
+
+// columns per block of the gradient kernels: two transforms' registers
+// are live, so at most 256 threads
+template <int D>
+constexpr int grad_nc() {
+  constexpr int by_thr = 256 / Fft2<D, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<D, false>::LDS_COL * 4);
+  return divisor_le(D, by_thr < by_lds ? by_thr : by_lds);
 }
 
 // columns of the extension: gy[a][r][c]; NC adjacent columns per block
@@ -157,8 +161,8 @@ __global__ void __launch_bounds__(256)
 k_grad_cols(const float2* __restrict__ iu, const float2* __restrict__ tw, float* __restrict__ gy) {
   constexpr int D = 2 * C;
   using F = Fft2<D, false>;
-  constexpr int NC = cols_nc<D>();
-  __shared__ float2 lds[NC * F::LDS];
+  constexpr int NC = grad_nc<D>();
+  __shared__ float lds[NC * F::LDS_COL];
   const int2 ab = xcd_block();
   const int a = ab.x;
   const int cl = threadIdx.x % NC, n2 = threadIdx.x / NC;
@@ -167,7 +171,7 @@ k_grad_cols(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
   const int xs = mirror_src<C>(c);
   auto pos = [xs](int i, int& y, int& x) { y = mirror_src<C>(i); x = xs; };
   float g[F::L2];
-  line_gradient<D>(iu_a, pos, n2, lds + cl * F::LDS, tw, g);
+  line_gradient<D>(iu_a, pos, n2, typename F::template Col<NC>{lds + cl}, tw, g);
   if (n2 < F::L1) {
     float* o = gy + (size_t)a * D * D + c;
 #pragma unroll
@@ -260,7 +264,9 @@ static hipError_t unwrap_c(const float2* iu, int A, const float2* tw, float* gx,
   {
     Timed tm("unwrap_grad", s);
     hipLaunchKernelGGL((k_grad_rows<C>), rows_grid<D>(A), rows_block<D>(), 0, s, iu, tw, gx);
-    hipLaunchKernelGGL((k_grad_cols<C>), cols_grid<D>(A), cols_block<D>(), 0, s, iu, tw, gy);
+    constexpr int GNC = grad_nc<D>();
+    hipLaunchKernelGGL((k_grad_cols<C>), dim3(D / GNC, A), dim3(GNC * Fft2<D, false>::T), 0, s,
+                       iu, tw, gy);
   }
   {
     Timed tm("unwrap_fwd", s);

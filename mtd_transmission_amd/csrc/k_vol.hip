@@ -11,7 +11,7 @@
 // its hits in the reference's angle order: this is the sort-by-voxel of the
 // accumulation done analytically (the scatter map is invertible per angle),
 // and the sum is deterministic and equal to the sequential FP32 sum. Partial
-// columns live in LDS; every voxel of the V^3 spectrum and count volume is
+// columns live in registers; every voxel of the V^3 spectrum and count volume is
 // written exactly once (zeros included), so no clearing pass is needed.
 // Banded mode (the fused plain method): hits of all angles lie in a height
 // band about V/2 of half-width about R - sqrt(R^2 - NXMAX^2) (a third of the

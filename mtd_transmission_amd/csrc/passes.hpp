@@ -14,20 +14,35 @@
 
 namespace mtd {
 
-// transforms per block: at most 256 threads and 64 KiB of LDS per block
-template <int L>
-constexpr int lines_per_block() {
-  constexpr int by_thr = 256 / Fft2<L, false>::T;
-  constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS * 8);
-  return divisor_le(L, by_thr < by_lds ? by_thr : by_lds);
-}
+// Row passes: NT transforms per block (at most 256 threads, 64 KiB LDS).
 template <int L>
 constexpr int rows_nt() {
-  return lines_per_block<L>();
+  constexpr int by_thr = 256 / Fft2<L, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS_ROW * 4);
+  return divisor_le(L, by_thr < by_lds ? by_thr : by_lds);
 }
+// Column passes: NC adjacent columns per block, up to 32 (128-byte segments
+// even for float outputs), at most 1024 threads and 64 KiB LDS; lines
+// longer than 512 keep 8 columns (their register FFTs allow 3 waves per
+// SIMD, which 4-wave blocks fill).
 template <int L>
 constexpr int cols_nc() {
-  return lines_per_block<L>();
+  constexpr int by_thr = (L > 512 ? 256 : 1024) / Fft2<L, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS_COL * 4);
+  constexpr int m = by_thr < by_lds ? by_thr : by_lds;
+  return divisor_le(L, m < 32 ? m : 32);
+}
+template <int L>
+constexpr int cols_threads() {
+  return cols_nc<L>() * Fft2<L, false>::T;
+}
+// Column blocks are up to 1024 threads; ask the compiler for at least 4
+// waves per SIMD (at most 128 VGPRs) so that a CU keeps 16 waves in flight,
+// 3 for lines longer than 512 (a 32 + 32 point register FFT needs more).
+// (HIP's second __launch_bounds__ argument is amdgpu_waves_per_eu.)
+template <int L>
+constexpr int col_waves_per_simd() {
+  return L > 512 ? 3 : 4;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs in dispatch order (linear
@@ -62,7 +77,7 @@ template <int L, bool INV, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op) {
   using F = Fft2<L, INV>;
   constexpr int NT = rows_nt<L>();
-  __shared__ float2 lds[NT * F::LDS];
+  __shared__ float lds[NT * F::LDS_ROW];
   const int2 ab = xcd_block();
   const int a = ab.x;
   if (skip_batch(op, a)) return;  // block-uniform, before the barrier
@@ -72,7 +87,7 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = op.load(a, line, F::L2 * n1 + n2);
-  F::run(v, u, n2, lds + r * F::LDS, op.tw);
+  F::run(v, u, n2, typename F::Row{lds + r * F::LDS_ROW}, op.tw);
   if (n2 < F::L1) {
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) op.store(a, line, n2 + F::L1 * k2, u[k2]);
@@ -80,10 +95,10 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
 }
 
 template <int L, bool INV, class Op>
-__global__ void __launch_bounds__(256) k_cols(Op op) {
+__global__ void __launch_bounds__(cols_threads<L>(), col_waves_per_simd<L>()) k_cols(Op op) {
   using F = Fft2<L, INV>;
   constexpr int NC = cols_nc<L>();
-  __shared__ float2 lds[NC * F::LDS];
+  __shared__ float lds[NC * F::LDS_COL];
   const int2 ab = xcd_block();
   const int a = ab.x, c0 = ab.y * NC;
   if (skip_batch(op, a)) return;  // block-uniform, before the barrier
@@ -92,7 +107,7 @@ __global__ void __launch_bounds__(256) k_cols(Op op) {
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = op.load(a, c0 + c, F::L2 * n1 + n2);
-  F::run(v, u, n2, lds + c * F::LDS, op.tw);
+  F::run(v, u, n2, typename F::template Col<NC>{lds + c}, op.tw);
   if (n2 < F::L1) {
 #pragma unroll
     for (int k2 = 0; k2 < F::L2; ++k2) op.store(a, c0 + c, n2 + F::L1 * k2, u[k2]);
@@ -113,7 +128,7 @@ inline dim3 cols_grid(int A) {
 }
 template <int L>
 inline dim3 cols_block() {
-  return dim3(cols_nc<L>() * Fft2<L, false>::T);
+  return dim3(cols_threads<L>());
 }
 
 }  // namespace mtd
