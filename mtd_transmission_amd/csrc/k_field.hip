@@ -217,37 +217,45 @@ struct FieldMode {
 // The fitted background sum_k c_k x^i y^j (compuBackgr2,
 // Correction_aberration.cpp:151-164; k runs i-outer over i + j <= 4) in
 // Horner form: sum_i x^i P_i(y), P_i(y) = sum_j c_(i,j) y^j. Along a row y is
-// fixed, so the P_i(y) are common to all its pixels.
-__device__ __forceinline__ double poly_eval(const double* __restrict__ c, double x, double y) {
-  const double p0 = (((c[4] * y + c[3]) * y + c[2]) * y + c[1]) * y + c[0];
-  const double p1 = ((c[8] * y + c[7]) * y + c[6]) * y + c[5];
-  const double p2 = (c[11] * y + c[10]) * y + c[9];
-  const double p3 = c[13] * y + c[12];
-  const double p4 = c[14];
-  return (((p4 * x + p3) * x + p2) * x + p1) * x + p0;
+// fixed: the P_i(y) are computed once per row (RowPoly) and each pixel costs
+// four FP64 multiply-adds.
+struct RowPoly {
+  double p[5];
+};
+__device__ __forceinline__ RowPoly row_poly(const double* __restrict__ c, double y) {
+  RowPoly r;
+  r.p[0] = (((c[4] * y + c[3]) * y + c[2]) * y + c[1]) * y + c[0];
+  r.p[1] = ((c[8] * y + c[7]) * y + c[6]) * y + c[5];
+  r.p[2] = (c[11] * y + c[10]) * y + c[9];
+  r.p[3] = c[13] * y + c[12];
+  r.p[4] = c[14];
+  return r;
+}
+__device__ __forceinline__ double row_poly_at(const RowPoly& r, double x) {
+  return (((r.p[4] * x + r.p[3]) * x + r.p[2]) * x + r.p[1]) * x + r.p[0];
 }
 
-// phu_a: unwrapped phase of this angle (U frame, row length C) or nullptr
-__device__ __forceinline__ float2 form_pixel(float2 u, int X, int Y,
-                                             const double* __restrict__ cph,
-                                             const double* __restrict__ cam,
-                                             float2 spec_norm, int mode,
-                                             const float* __restrict__ phu_a, int C) {
-  if (!(mode & 2)) {  // C_ABER 0: U * conj(s)/|s|
+// Field of one pixel from its demodulated value u (main.cpp:289-316), given
+// the phase (wrapped atan2 of u, or unwrapped) and the two fitted
+// backgrounds at that pixel; C_ABER 0 (main.cpp:332-337): U * conj(s)/|s|.
+__device__ __forceinline__ float2 form_from(float2 u, float phf, double bgp, double bga,
+                                            float2 spec_norm, int mode) {
+  if (!(mode & 2)) {
     return make_float2(u.x * spec_norm.x - u.y * spec_norm.y,
                        u.x * spec_norm.y + u.y * spec_norm.x);
   }
-  const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
   const float amf = hypotf(u.x, u.y);
-  const double bgp = poly_eval(cph, (double)X, (double)Y);
-  const double bga = poly_eval(cam, (double)X, (double)Y);
   const double phic = (double)phf - bgp;
   double A = (double)amf / (bga + 0.00000001);
   if (A > 10) A = 1;
   if (A < -10) A = 1;
   const float pc = (float)phic;
   const float aa = (float)fabs(A);
-  if (mode & 1) return make_float2(aa * cosf(pc) - 1.0f, aa * sinf(pc));
+  if (mode & 1) {
+    float sp, cp;
+    sincosf(pc, &sp, &cp);
+    return make_float2(aa * cp - 1.0f, aa * sp);
+  }
   return make_float2(logf(aa), pc);
 }
 
@@ -274,8 +282,16 @@ k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
   const float2 sn = (mode & 2) ? make_float2(0.f, 0.f)
                                : spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
   const float2 u = iu[((size_t)a * C + (Y + C / 2) % C) * C + (X + C / 2) % C];
-  const float* phu_a = phu ? phu + (size_t)a * C * C : nullptr;
-  fields[(size_t)a * C * C + i] = form_pixel(u, X, Y, cph, cph + NCOEF, sn, mode, phu_a, C);
+  float2 f;
+  if (mode & 2) {
+    const float phf = phu ? phu[((size_t)a * C + Y) * C + X] : atan2f(u.y, u.x);
+    const double bgp = row_poly_at(row_poly(cph, (double)Y), (double)X);
+    const double bga = row_poly_at(row_poly(cph + NCOEF, (double)Y), (double)X);
+    f = form_from(u, phf, bgp, bga, sn, mode);
+  } else {
+    f = form_from(u, 0.f, 0.0, 0.0, sn, mode);
+  }
+  fields[(size_t)a * C * C + i] = f;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,19 +313,36 @@ struct FwdRowOp {
   float2 sn;
   const double* cph;
   const float* phu_a;
+  // per row (line): its U-frame index, window factor and background polys
+  int Y;
+  double wy;
+  RowPoly pp, pa;
   __device__ void begin(int a) {
     cph = coef + (size_t)a * 2 * NCOEF;
     phu_a = phu ? phu + (size_t)a * C * C : nullptr;
     if (from_iu && !(mode & 2))
       sn = spec_norm_of(crop + (size_t)a * C * C, C, posx[a], posy[a]);
   }
+  __device__ void line(int, int y) {
+    Y = (y + C / 2) % C;
+    wy = win[Y];
+    if (from_iu && (mode & 2)) {
+      pp = row_poly(cph, (double)Y);
+      pa = row_poly(cph + NCOEF, (double)Y);
+    }
+  }
   __device__ float2 load(int a, int y, int x) const {
-    const int X = (x + C / 2) % C, Y = (y + C / 2) % C;
-    const float w = (float)(win[X] * win[Y]);
+    const int X = (x + C / 2) % C;
+    const float w = (float)(win[X] * wy);
     float2 f;
     if (from_iu) {
-      f = form_pixel(src[((size_t)a * C + y) * C + x], X, Y, cph, cph + NCOEF, sn, mode, phu_a,
-                     C);
+      const float2 u = src[((size_t)a * C + y) * C + x];
+      if (mode & 2) {
+        const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
+        f = form_from(u, phf, row_poly_at(pp, (double)X), row_poly_at(pa, (double)X), sn, mode);
+      } else {
+        f = form_from(u, 0.f, 0.0, 0.0, sn, mode);
+      }
     } else {
       f = src[((size_t)a * C + Y) * C + X];
     }

@@ -11,7 +11,7 @@
 // its hits in the reference's angle order: this is the sort-by-voxel of the
 // accumulation done analytically (the scatter map is invertible per angle),
 // and the sum is deterministic and equal to the sequential FP32 sum. Partial
-// columns live in registers; every voxel of the V^3 spectrum and count volume is
+// columns live in LDS; every voxel of the V^3 spectrum and count volume is
 // written exactly once (zeros included), so no clearing pass is needed.
 // Banded mode (the fused plain method): hits of all angles lie in a height
 // band about V/2 of half-width about R - sqrt(R^2 - NXMAX^2) (a third of the
@@ -152,32 +152,24 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
     __syncthreads();
     // exact hit heights of this thread's column for the n new entries
     for (int i0 = 0; i0 < n; i0 += kBatch) {
+      // branch-free batch: the table loads of kBatch entries are in flight
+      // together (slots past n repeat the last entry and are not stored)
       double hb[kBatch];
-      int r2b[kBatch];
+      bool inp[kBatch];
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
-        r2b[t] = -1;
-        hb[t] = 0.0;
-        if (i0 + t < n) {
-          const int4 e = sent[i0 + t];
-          const int fdx = qx + e.z, fdy = qy + e.w;
-          const int r2 = fdx * fdx + fdy * fdy;
-          if (fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max) {
-            r2b[t] = r2;
-            hb[t] = g.htab[r2];
-          }
-        }
+        const int4 e = sent[min(i0 + t, n - 1)];
+        const int fdx = qx + e.z, fdy = qy + e.w;
+        const int r2 = fdx * fdx + fdy * fdy;
+        inp[t] = fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max;
+        hb[t] = g.htab[inp[t] ? r2 : 0];
       }
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
-        if (i0 + t < n) {
-          uint8_t o = 255;
-          if (r2b[t] >= 0) {
-            const int z = (int)round_away(hb[t] - sfz[i0 + t]) + g.V / 2;
-            o = (uint8_t)(z - sent[i0 + t].y);
-          }
-          zo[(size_t)(running + i0 + t) * 256] = o;
-        }
+        const int it = min(i0 + t, n - 1);
+        const int z = (int)round_away(hb[t] - sfz[it]) + g.V / 2;
+        const uint8_t o = inp[t] ? (uint8_t)(z - sent[it].y) : (uint8_t)255;
+        if (i0 + t < n) zo[(size_t)(running + i0 + t) * 256] = o;  // uniform
       }
     }
     running += n;
@@ -243,12 +235,12 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
     }
     return;
   }
-  // this column's ZW partial sums live in registers: each hit is added to
-  // its height slot by a compile-time-indexed select
-  float2 acc[ZW];
-  int cn[ZW];
+  // this column's ZW partial sums and hit counts (< 2^16: at most one hit
+  // per angle) live in LDS; each thread only touches its own column
+  __shared__ float2 acc[ZW][256];
+  __shared__ unsigned short cn[ZW][256];
 #pragma unroll
-  for (int k = 0; k < ZW; ++k) { acc[k] = make_float2(0.f, 0.f); cn[k] = 0; }
+  for (int k = 0; k < ZW; ++k) { acc[k][tid] = make_float2(0.f, 0.f); cn[k][tid] = 0; }
   const TileEntry* lst = list + (size_t)tile * g.A;
   const uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
   // spectrum element read by this column: the unshifted frame holds it at
@@ -271,10 +263,15 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
     }
     unsigned long long m = __ballot(ov);  // wave-uniform
     while (m) {
+      // Straight-line batch (no branch between the loads, so all of them are
+      // in flight together): unused slots repeat the first entry and are
+      // masked out afterwards.
       int bitv[kBatch];
+      bool valid[kBatch];
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
-        bitv[t] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+        valid[t] = m != 0;  // uniform; true for t = 0
+        bitv[t] = valid[t] ? __ffsll((unsigned long long)m) - 1 : bitv[0];
         m &= m - 1;  // m == 0 stays 0
       }
       int offv[kBatch], zlv[kBatch];
@@ -282,37 +279,39 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
       float bv[kBatch];
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
-        offv[t] = 255;
-        if (bitv[t] >= 0) {  // uniform
-          const int i = j0 + bitv[t];
-          const int ea = __builtin_amdgcn_readlane(e.a, bitv[t]);
-          zlv[t] = __builtin_amdgcn_readlane(e.zpack, bitv[t]) & 0xffff;
-          const int fxy = __builtin_amdgcn_readlane(e.fxy, bitv[t]);
-          const int fdx = qx + (int)(short)(fxy & 0xffff), fdy = qy + (fxy >> 16);
-          offv[t] = zo[(size_t)i * 256];
-          // addresses are clamped so that non-hits read in bounds; a hit has
-          // |fd| inside the crop and r2 <= r2max, where clamping is the identity
-          const int r2 = min(fdx * fdx + fdy * fdy, g.r2max);
-          const size_t si = g.frame_pos
-                                ? (size_t)(clampi(fdy, -g.nx, g.nx - 1) + g.nx) * g.C +
-                                      (clampi(fdx, -g.nx, g.nx - 1) + g.nx)
-                                : fixed;
-          spv[t] = g.spec[(size_t)ea * C2 + si];
-          bv[t] = g.beta[r2];
-        }
+        const int i = j0 + bitv[t];
+        const int ea = __builtin_amdgcn_readlane(e.a, bitv[t]);
+        zlv[t] = __builtin_amdgcn_readlane(e.zpack, bitv[t]) & 0xffff;
+        const int fxy = __builtin_amdgcn_readlane(e.fxy, bitv[t]);
+        const int fdx = qx + (int)(short)(fxy & 0xffff), fdy = qy + (fxy >> 16);
+        offv[t] = zo[(size_t)i * 256];
+        // addresses are clamped so that non-hits read in bounds; a hit has
+        // |fd| inside the crop and r2 <= r2max, where clamping is the identity
+        const int r2 = min(fdx * fdx + fdy * fdy, g.r2max);
+        const size_t sp = (size_t)(clampi(fdy, -g.nx, g.nx - 1) + g.nx) * g.C +
+                          (clampi(fdx, -g.nx, g.nx - 1) + g.nx);
+        spv[t] = g.spec[(size_t)ea * C2 + (g.frame_pos ? sp : fixed)];
+        bv[t] = g.beta[r2];
       }
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
-        const int k = offv[t] == 255 ? -1 : zlv[t] + offv[t] - z0;
-        const float pim = bv[t] * spv[t].y, pre = bv[t] * spv[t].x;  // (-b Si, b Sr)
+        if (!valid[t]) offv[t] = 255;
+      }
+      // Unconditional updates (a conditional one lets the compiler sink its
+      // loads behind the branch): a miss adds -0.0 to slot 0, which leaves
+      // every value bit-identical (x + -0.0 == x, also for x = +-0.0), and
+      // does not count.
 #pragma unroll
-        for (int kk = 0; kk < ZW; ++kk) {
-          if (k == kk) {
-            acc[kk].x = acc[kk].x + (-pim);
-            acc[kk].y = acc[kk].y + pre;
-            cn[kk] += 1;
-          }
-        }
+      for (int t = 0; t < kBatch; ++t) {
+        const int k = offv[t] == 255 ? -1 : zlv[t] + offv[t] - z0;
+        const bool hit = k >= 0 && k < ZW;
+        const int kc = hit ? k : 0;
+        const float pim = bv[t] * spv[t].y, pre = bv[t] * spv[t].x;  // (-b Si, b Sr)
+        float2 s = acc[kc][tid];
+        s.x = s.x + (hit ? -pim : -0.0f);
+        s.y = s.y + (hit ? pre : -0.0f);
+        acc[kc][tid] = s;
+        cn[kc][tid] = (unsigned short)(cn[kc][tid] + (hit ? 1 : 0));
       }
     }
   }
@@ -320,8 +319,8 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
   for (int k = 0; k < ZW; ++k) {
     const int z = z0 + k;
     if (z > z1) break;
-    const float2 s = acc[k];
-    const int c = cn[k];
+    const float2 s = acc[k][tid];
+    const int c = cn[k][tid];
     if (norm) {
       const float d = (float)(c == 0 ? 1 : c);
       float2 t = make_float2(s.x / d, s.y / d);

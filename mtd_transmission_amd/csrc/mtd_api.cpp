@@ -649,6 +649,8 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
                         const int32_t* posspec, int A, float* vol, int32_t* counts,
                         int flags = 0, int2* band = nullptr) {
+  // per-voxel hit counts are accumulated in 16 bits (one hit per angle)
+  if (A > 65535) return set_err(ctx, MTD_ERR_UNSUPPORTED, "more than 65535 angles per volume");
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));

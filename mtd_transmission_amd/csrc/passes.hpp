@@ -73,6 +73,13 @@ __device__ __forceinline__ bool skip_batch(const Op& op, int a) {
   return false;
 }
 
+// An operation may define line(batch, line), called once per transform
+// before its loads, to hoist per-line work out of load().
+template <class Op, class = void>
+struct has_line : std::false_type {};
+template <class Op>
+struct has_line<Op, std::void_t<decltype(std::declval<Op&>().line(0, 0))>> : std::true_type {};
+
 template <int L, bool INV, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op) {
   using F = Fft2<L, INV>;
@@ -84,6 +91,7 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
   const int r = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
   const int line = ab.y * NT + r;
   op.begin(a);
+  if constexpr (has_line<Op>::value) op.line(a, line);
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = op.load(a, line, F::L2 * n1 + n2);
