@@ -57,6 +57,7 @@ struct mtd_ctx {
   std::string err;
   std::map<int, float2*> twiddles;                       // per FFT length
   std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)
+  std::map<std::pair<int, uint32_t>, float*> tukey_signed;
   std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
   std::map<std::tuple<uint64_t, int, uint32_t>, float*> betatab;  // (R, r2max, k0) bits
   std::map<int, mtd::Buffer> ws;                          // scratch slots
@@ -107,14 +108,17 @@ int hip_check(mtd_ctx* ctx, hipError_t e, const char* what);
 void* ws_get(mtd_ctx* ctx, int slot, size_t bytes);
 const float2* twiddle(mtd_ctx* ctx, int L);
 const double* tukey_table(mtd_ctx* ctx, int n, double alpha);
+// (float)tukey1d[i] * (-1)^i
+const float* tukey_signed_table(mtd_ctx* ctx, int n, double alpha);
 void tukey1d_host(int n, double alpha, double* out);
 int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask);
 
 bool fft_len_supported(int L);
 
 // ---- kernel launchers (return hipError_t of the launch) -------------------
+// ws: float tukey1d(N, alpha_holo) with the centring sign (-1)^i folded in
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
-                            const double* win, const float2* tw, int x0,
+                            const float* ws, const float2* tw, int x0,
                             int C, float2* t1, hipStream_t s);
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
                             int C, int x0, int y0, float2* crop,

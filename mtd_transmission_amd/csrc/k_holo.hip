@@ -26,7 +26,7 @@ constexpr int holo_pairs() {  // row pairs per block: <= 256 threads, <= 64 KiB
 
 template <int N, typename TIn>
 __global__ void __launch_bounds__(256)
-k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
+k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
             const float2* __restrict__ tw, int x0, int C,
             float2* __restrict__ t1) {
   using F = Fft2<N, false>;
@@ -39,17 +39,15 @@ k_holo_rows(const TIn* __restrict__ holos, const double* __restrict__ win,
   const int y0 = 2 * (ab.y * NP + p), y1 = y0 + 1;
   const TIn* h0 = holos + ((size_t)a * N + y0) * N;
   const TIn* h1 = h0 + N;
-  const double wy0 = win[y0], wy1 = win[y1];
+  // window tuk(x) tuk(y) with the centring sign (-1)^(x+y): ws holds
+  // (-1)^i (float)tuk(i), so one product per sample carries both
+  const float wy0 = ws[y0], wy1 = ws[y1];
   float2 v[F::L1], u[F::L2];
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) {
     const int x = F::L2 * n1 + n2;
-    const double wx = win[x];
-    const float w0 = (float)(wx * wy0), w1 = (float)(wx * wy1);
-    float g0 = (float)h0[x] * w0;
-    float g1 = (float)h1[x] * w1;
-    if ((x + y0) & 1) g0 = -g0; else g1 = -g1;  // centring (-1)^(x+y)
-    v[n1] = make_float2(g0, g1);
+    const float wx = ws[x];
+    v[n1] = make_float2((float)h0[x] * (wx * wy0), (float)h1[x] * (wx * wy1));
   }
   float* buf = lds + p * F::LDS_ROW;
   F::run(v, u, n2, typename F::Row{buf}, tw);
@@ -127,7 +125,7 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
 }
 
 template <int N>
-static hipError_t rows_n(int dtype, const void* holos, int A, const double* win,
+static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
                          const float2* tw, int x0, int C, float2* t1,
                          hipStream_t s) {
   constexpr int NP = holo_pairs<N>();
@@ -165,7 +163,7 @@ static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
 #define MTD_HOLO_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
-                            const double* win, const float2* tw, int x0,
+                            const float* win, const float2* tw, int x0,
                             int C, float2* t1, hipStream_t s) {
   switch (N) {
 #define X(n) case n: return rows_n<n>(dtype, holos, A, win, tw, x0, C, t1, s);

@@ -149,6 +149,20 @@ const double* tukey_table(mtd_ctx* ctx, int n, double alpha) {
   return d;
 }
 
+const float* tukey_signed_table(mtd_ctx* ctx, int n, double alpha) {
+  const auto key = std::make_pair(n, bits_of((float)alpha));
+  auto it = ctx->tukey_signed.find(key);
+  if (it != ctx->tukey_signed.end()) return it->second;
+  std::vector<double> h(n);
+  tukey1d_host(n, alpha, h.data());
+  std::vector<float> f(n);
+  for (int i = 0; i < n; ++i) f[i] = (i & 1) ? -(float)h[i] : (float)h[i];
+  float* d = upload(f.data(), f.size());
+  if (!d) return nullptr;
+  ctx->tukey_signed[key] = d;
+  return d;
+}
+
 // ---------------------------------------------------------------------------
 // least-squares plan (Correction_aberration.cpp:86-149,186-217)
 // ---------------------------------------------------------------------------
@@ -347,7 +361,7 @@ static size_t dtype_bytes(int dtype) {
 static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
                       int dtype, int a0, int n, float2* crop_out) {
   const int N = g->n, C = 2 * g->nxmax;
-  const double* win = tukey_table(ctx, N, g->alpha_holo);
+  const float* win = tukey_signed_table(ctx, N, g->alpha_holo);
   const float2* twN = twiddle(ctx, N);
   float2* t1 = (float2*)ws_get(ctx, WS_T1, (size_t)n * N * C * sizeof(float2));
   if (!win || !twN || !t1) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram stage)");
@@ -471,6 +485,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
   for (auto& kv : ctx->tukey) (void)hipFree(kv.second);
+  for (auto& kv : ctx->tukey_signed) (void)hipFree(kv.second);
   for (auto& kv : ctx->htab) (void)hipFree(kv.second);
   for (auto& kv : ctx->betatab) (void)hipFree(kv.second);
   for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);

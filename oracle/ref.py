@@ -193,6 +193,16 @@ def tukey2d(n, alpha):
     return t[None, :] * t[:, None]
 
 
+def holo_window(n, alpha, prec="f64"):
+    """Window of the hologram stage: tukey2D in double (FP64 path); the FP32
+    path forms the same separable product from the float-rounded 1-D factors
+    (tuk1D[x] * tuk1D[y] in float32)."""
+    if prec == "f32":
+        t = tukey1d(n, alpha)[0].astype(np.float32)
+        return t[None, :] * t[:, None]
+    return tukey2d(n, alpha)
+
+
 # --------------------------------------------------------------------------
 # geometry (A1 derivations)
 # --------------------------------------------------------------------------
@@ -265,7 +275,7 @@ def holo_to_crop(holo, g: Geom, prec="f64", window=None):
     rt, ct = _real(prec), _cplx(prec)
     N, C = g.n, g.C
     if window is None:
-        window = tukey2d(N, g.alpha_holo)
+        window = holo_window(N, g.alpha_holo, prec)
     h = np.asarray(holo).astype(rt) * window.astype(rt)
     spec = np.fft.fft2(fftshift2(h).astype(ct))
     spec = (spec / rt(N * N)).astype(ct)
@@ -519,7 +529,7 @@ def preprocess(holos, g: Geom, mask=None, prec="f64", return_crops=False):
     all x then all y (main.cpp:250-251))."""
     C = g.C
     A = len(holos)
-    window = tukey2d(g.n, g.alpha_holo)
+    window = holo_window(g.n, g.alpha_holo, prec)
     plan = LsqPlan.build(load_mask(C, mask))
     fields = np.empty((A, C, C), dtype=_cplx(prec))
     crops = np.empty((A, C, C), dtype=_cplx(prec)) if return_crops else None
