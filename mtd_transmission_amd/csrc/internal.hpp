@@ -100,7 +100,7 @@ struct Timed {
 // scratch slots
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
-  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP
+  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -120,12 +120,18 @@ bool fft_len_supported(int L);
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
                             const float* ws, const float2* tw, int x0,
                             int C, float2* t1, hipStream_t s);
+// part != nullptr: also writes per-block argmax partials of the crops
+// (holo_argmax_parts(N, C) per angle) for launch_argmax_fin
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
-                            int C, int x0, int y0, float2* crop,
+                            int C, int x0, int y0, float2* crop, float2* part,
                             hipStream_t s);
+int holo_argmax_parts(int N, int C);
 
 hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
                          int* posy, float2* part, hipStream_t s);
+// second stage only, over S partials per angle
+hipError_t launch_argmax_fin(int C, const float2* part, int S, int A, int* posx, int* posy,
+                             hipStream_t s);
 size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
                         const int* posy, int A, const float2* tw,

@@ -11,6 +11,7 @@
 //   form       main.cpp:289-316 (Born / Rytov) or main.cpp:332-337 (C_ABER 0).
 //   spectrum   Reconstruction/main.cpp:207-217: window, shift by NXMAX,
 //              forward 2-D FFT * pixUBorn^2, optional shift by +posSpec.
+#include "argmax.hpp"
 #include "dft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
@@ -22,16 +23,9 @@ namespace mtd {
 // ---------------------------------------------------------------------------
 constexpr int ARGMAX_SEG = 16;  // blocks per crop in the first stage
 
-__device__ __forceinline__ bool better(float v2, int i2, float v1, int i1) {
-  return v2 > v1 || (v2 == v1 && i2 < i1);
-}
-
 template <int C>
 __global__ void __launch_bounds__(256)
 k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
-  // slot 0's |v|^2 is computed unfused, as the reference does (plain
-  // operators: the pragma is lexical)
-#pragma clang fp contract(off)
   const int a = blockIdx.y, seg = blockIdx.x;
   constexpr int LEN = (C * C + ARGMAX_SEG - 1) / ARGMAX_SEG;
   const float2* p = crop + (size_t)a * C * C;
@@ -39,41 +33,26 @@ k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
   int bi = 0x7fffffff;
   const int i0 = seg * LEN, i1 = min(i0 + LEN, C * C);
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
-    const float2 v = p[i];
-    const float m = (i == 0) ? v.x * v.x + v.y * v.y : hypotf(v.x, v.y);
+    const float m = argmax_key(p[i], i == 0);
     if (m > best) { best = m; bi = i; }
   }
-  __shared__ float sv[256];
-  __shared__ int si[256];
-  sv[threadIdx.x] = best;
-  si[threadIdx.x] = bi;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const float v2 = sv[threadIdx.x + s];
-      const int i2 = si[threadIdx.x + s];
-      if (better(v2, i2, sv[threadIdx.x], si[threadIdx.x])) {
-        sv[threadIdx.x] = v2;
-        si[threadIdx.x] = i2;
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[(size_t)a * ARGMAX_SEG + seg] = make_float2(sv[0], __int_as_float(si[0]));
+  __shared__ float red[8];
+  argmax_block(best, bi, red);
+  if (threadIdx.x == 0) part[(size_t)a * ARGMAX_SEG + seg] = make_float2(best, __int_as_float(bi));
 }
 
-// second stage: one thread per angle over its segments, in order
+// second stage: one thread per angle over its S partials
 template <int C>
-__global__ void k_argmax_fin(const float2* __restrict__ part, int A, int* __restrict__ posx,
+__global__ void k_argmax_fin(const float2* __restrict__ part, int S, int A, int* __restrict__ posx,
                              int* __restrict__ posy) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A) return;
   float best = -1.f;
   int bi = 0x7fffffff;
-  for (int s = 0; s < ARGMAX_SEG; ++s) {
-    const float2 p = part[(size_t)a * ARGMAX_SEG + s];
+  for (int s = 0; s < S; ++s) {
+    const float2 p = part[(size_t)a * S + s];
     const int i = __float_as_int(p.y);
-    if (better(p.x, i, best, bi)) { best = p.x; bi = i; }
+    if (argmax_better(p.x, i, best, bi)) { best = p.x; bi = i; }
   }
   posx[a] = bi % C;
   posy[a] = bi / C;
@@ -386,7 +365,17 @@ static hipError_t argmax_c(const float2* crop, int A, int* posx, int* posy,
     hipLaunchKernelGGL((k_argmax_part<C>), dim3(ARGMAX_SEG, A), dim3(256), 0, s, crop, part);
   }
   Timed tm("argmax_fin", s);
-  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, A, posx, posy);
+  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, ARGMAX_SEG, A,
+                     posx, posy);
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t argmax_fin_c(const float2* part, int S, int A, int* posx, int* posy,
+                               hipStream_t s) {
+  Timed tm("argmax_fin", s);
+  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, S, A, posx,
+                     posy);
   return hipGetLastError();
 }
 
@@ -455,6 +444,16 @@ hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
                          int* posy, float2* part, hipStream_t s) {
   switch (C) {
 #define X(n) case n: return argmax_c<n>(crop, A, posx, posy, part, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_argmax_fin(int C, const float2* part, int S, int A, int* posx, int* posy,
+                             hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return argmax_fin_c<n>(part, S, A, posx, posy, s);
     MTD_CROP_SIZES(X)
 #undef X
   }

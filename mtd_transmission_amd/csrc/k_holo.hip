@@ -11,6 +11,7 @@
 // written, to t1[a][y][d] (N x C per hologram).
 // Pass 2 (columns): each of the C columns of t1 is transformed along y and
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
+#include "argmax.hpp"
 #include "fft_reg.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
@@ -97,7 +98,8 @@ constexpr int holo_cols_threads() {
 template <int N>
 __global__ void __launch_bounds__(holo_cols_threads<N>(), col_waves_per_simd<N>())
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
-            int C, int x0, int y0, float scale, float2* __restrict__ crop) {
+            int C, int x0, int y0, float scale, float2* __restrict__ crop,
+            float2* __restrict__ part) {
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
   __shared__ float lds[NC * F::LDS_COL];
@@ -109,6 +111,10 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
 #pragma unroll
   for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = src[(size_t)(F::L2 * n1 + n2) * C];
   F::run(v, u, n2, typename F::template Col<NC>{lds + c}, tw);
+  // argmax partial of the crop values this block writes (coordSpec's
+  // first stage, fused: the crop is not read again for it)
+  float best = -1.f;
+  int bi = 0x7fffffff;
   if (n2 < F::L1) {
     float2* dst = crop + (size_t)a * C * C + d0 + c;
     const int kx = x0 + d0 + c;
@@ -118,9 +124,19 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
       const int e = ky - y0;
       if (e >= 0 && e < C) {
         const float s = ((kx + ky) & 1) ? -scale : scale;
-        dst[(size_t)e * C] = make_float2(u[k2].x * s, u[k2].y * s);
+        const float2 val = make_float2(u[k2].x * s, u[k2].y * s);
+        dst[(size_t)e * C] = val;
+        const int i = e * C + d0 + c;
+        const float m = argmax_key(val, i == 0);
+        if (argmax_better(m, i, best, bi)) { best = m; bi = i; }
       }
     }
+  }
+  if (part) {  // uniform
+    __shared__ float red[2 * (holo_cols_threads<N>() / 64)];
+    argmax_block(best, bi, red);
+    if (threadIdx.x == 0)
+      part[(size_t)a * (C / NC) + ab.y] = make_float2(best, __int_as_float(bi));
   }
 }
 
@@ -150,13 +166,13 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
 
 template <int N>
 static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
-                         int x0, int y0, float2* crop, hipStream_t s) {
+                         int x0, int y0, float2* crop, float2* part, hipStream_t s) {
   constexpr int NC = holo_cols_nc<N>();
   if (C % NC) return hipErrorInvalidValue;
   const float scale = (float)(0.5 / ((double)N * (double)N));
   Timed tm("holo_cols", s);
   hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(holo_cols_threads<N>()), 0, s,
-                     t1, tw, C, x0, y0, scale, crop);
+                     t1, tw, C, x0, y0, scale, crop, part);
   return hipGetLastError();
 }
 
@@ -174,14 +190,23 @@ hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
 }
 
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
-                            int C, int x0, int y0, float2* crop,
+                            int C, int x0, int y0, float2* crop, float2* part,
                             hipStream_t s) {
   switch (N) {
-#define X(n) case n: return cols_n<n>(t1, A, tw, C, x0, y0, crop, s);
+#define X(n) case n: return cols_n<n>(t1, A, tw, C, x0, y0, crop, part, s);
     MTD_HOLO_SIZES(X)
 #undef X
   }
   return hipErrorInvalidValue;
+}
+
+int holo_argmax_parts(int N, int C) {
+  switch (N) {
+#define X(n) case n: return C / holo_cols_nc<n>();
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return 0;
 }
 
 bool holo_len_supported(int N) {

@@ -358,8 +358,9 @@ static size_t dtype_bytes(int dtype) {
 }
 
 // holograms [a0, a0+n) -> crops
+// part_out (nullable): argmax partials of these crops, holo_argmax_parts per angle
 static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
-                      int dtype, int a0, int n, float2* crop_out) {
+                      int dtype, int a0, int n, float2* crop_out, float2* part_out = nullptr) {
   const int N = g->n, C = 2 * g->nxmax;
   const float* win = tukey_signed_table(ctx, N, g->alpha_holo);
   const float2* twN = twiddle(ctx, N);
@@ -369,7 +370,8 @@ static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
   MTD_TRY(hip_check(ctx, launch_holo_rows(N, dtype, src, n, win, twN, g->circle_cx - g->nxmax,
                                           C, t1, ctx->stream), "hologram row pass"));
   MTD_TRY(hip_check(ctx, launch_holo_cols(N, t1, n, twN, C, g->circle_cx - g->nxmax,
-                                          g->circle_cy - g->nxmax, crop_out, ctx->stream),
+                                          g->circle_cy - g->nxmax, crop_out, part_out,
+                                          ctx->stream),
                     "hologram column pass"));
   return MTD_OK;
 }
@@ -377,17 +379,25 @@ static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
 static int field_mode(const mtd_geom* g) { return (g->born ? 1 : 0) | (g->c_aber ? 2 : 0); }
 
 // crops [n] (angles a0..) -> fields (out_fields) or unshifted spectra F (out_F)
+// part_in (nullable): argmax partials already computed by the hologram
+// column pass, S per angle; otherwise the argmax reads the crops
 static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int n,
-                       int* posx, int* posy, float2* out_fields, float2* out_F) {
+                       int* posx, int* posy, float2* out_fields, float2* out_F,
+                       const float2* part_in = nullptr, int S = 0) {
   const int C = 2 * g->nxmax;
   const float2* twC = twiddle(ctx, C);
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)n * C * C * sizeof(float2));
   float2* iu = (float2*)ws_get(ctx, WS_IU, (size_t)n * C * C * sizeof(float2));
   double* coef = (double*)ws_get(ctx, WS_COEF, (size_t)n * 30 * sizeof(double));
-  float2* part = (float2*)ws_get(ctx, WS_ARGMAX, argmax_scratch_bytes(n));
-  if (!twC || !work || !iu || !coef || !part)
+  if (!twC || !work || !iu || !coef)
     return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
-  MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, part, ctx->stream), "argmax"));
+  if (part_in) {
+    MTD_TRY(hip_check(ctx, launch_argmax_fin(C, part_in, S, n, posx, posy, ctx->stream), "argmax"));
+  } else {
+    float2* part = (float2*)ws_get(ctx, WS_ARGMAX, argmax_scratch_bytes(n));
+    if (!part) return set_err(ctx, MTD_ERR_HIP, "out of device memory (argmax)");
+    MTD_TRY(hip_check(ctx, launch_argmax(C, crop, n, posx, posy, part, ctx->stream), "argmax"));
+  }
   MTD_TRY(hip_check(ctx, launch_demod(C, crop, posx, posy, n, twC, work, iu, ctx->stream), "demodulation"));
   const int mode = field_mode(g);
   // DEROUL 1 / VOLKOV 1: the phase is unwrapped before the fit; the
@@ -438,14 +448,15 @@ static size_t field_bytes(const mtd_geom* g) {
   return b;
 }
 
-// holograms [f0, f0+n) -> crops, in hologram-stage chunks
+// holograms [f0, f0+n) -> crops (and argmax partials, S per angle), in
+// hologram-stage chunks
 static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
-                      int n, float2* crop) {
+                      int n, float2* crop, float2* part, int S) {
   const int N = g->n, C = 2 * g->nxmax;
   const int hc = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
   for (int h0 = 0; h0 < n; h0 += hc)
     MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
-                       crop + (size_t)h0 * C * C));
+                       crop + (size_t)h0 * C * C, part + (size_t)h0 * S));
   return MTD_OK;
 }
 
@@ -622,13 +633,15 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
   const int C = 2 * g->nxmax;
   if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
+  const int S = holo_argmax_parts(g->n, C);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
-  if (!crop) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
+  float2* part = (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * S * sizeof(float2));
+  if (!crop || !part) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
     MTD_TRY(front_chunk(ctx, g, crop, n, posspec + a0, posspec + A + a0,
-                        (float2*)fields + (size_t)a0 * C * C, nullptr));
+                        (float2*)fields + (size_t)a0 * C * C, nullptr, part, S));
   }
   return MTD_OK;
 }
@@ -761,15 +774,18 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   }
   float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, std::max(A, 1));
+  const int S = holo_argmax_parts(g->n, C);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
+  float2* part = (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * S * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
   int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
-  if (!F || !crop || !vol || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!F || !crop || !part || !vol || !band)
+    return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
     MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
-                        F + (size_t)a0 * C * C));
+                        F + (size_t)a0 * C * C, part, S));
   }
   // The gather writes the count-normalised, centring-signed spectrum, and
   // only in the height band that holds hits; the inverse transform treats the
