@@ -154,12 +154,14 @@ size_t unwrap_scratch_bytes(int C, int A);
 // Per-angle forward spectrum. from_iu=0: src are fields (U frame);
 // from_iu=1: src is IU and the field formation runs inside the load.
 // Output frame: shift_pos=0 -> unshifted F, 1 -> shifted by +posSpec.
+// beta != nullptr (shift_pos=0 only): bins pre-multiplied by the Ewald
+// factor of their pupil pixel (gather flag 2).
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
                                int mode, const float* phu, const int* posx, const int* posy,
                                int A, const double* win, const float2* tw,
-                               float dx2, int shift_pos, float2* work,
-                               float2* out, hipStream_t s);
+                               float dx2, int shift_pos, const float* beta, int r2max,
+                               float2* work, float2* out, hipStream_t s);
 
 hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
                             double R2, const double* htab, int r2tab,
@@ -169,13 +171,14 @@ hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
 hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, int r2max,
                        int2* band, hipStream_t s);
 // band == nullptr: every voxel of the volume (and counts) is written;
-// otherwise only the height windows that meet the band are.
+// otherwise only the height windows that meet the band are, and [zlo, zhi]
+// is a host-side bound of the band (only windows meeting it are launched).
 hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
-                         hipStream_t s);
+                         int zlo, int zhi, hipStream_t s);
 size_t gather_scratch_bytes(int V, int A);
 // prenorm=1: vol already divided by counts and centring-signed (gather flag 1).
 // band != nullptr: heights outside the band are zero and are not read.

@@ -21,6 +21,8 @@
 // Finalisation (Reconstruction/main.cpp:243-352): divide by counts (0 -> 1),
 // backward 3-D FFT with both fftshift3D folded into (-1)^(x+y+z) sign flips,
 // times delta_f^3 and -1/(2 n0 kv^2), split into index and absorption.
+#include <algorithm>
+
 #include "dft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
@@ -190,11 +192,13 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
 
 // flags: bit0 = normalise (divide by counts, 0 -> 1) and apply the centring
 // sign (-1)^(x+y+z) of the 3-D inverse FFT; counts are then not written.
-template <int ZW>
+// PRE: the spectra are already multiplied by the Ewald factor beta of their
+// pupil pixel (the same FP32 product), so beta is not looked up here.
+template <int ZW, bool PRE>
 __global__ void __launch_bounds__(256)
-k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __restrict__ list,
-             const int4* __restrict__ tinfo, const uint8_t* __restrict__ zoff, int flags,
-             const int2* __restrict__ band) {
+k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
+             const TileEntry* __restrict__ list, const int4* __restrict__ tinfo,
+             const uint8_t* __restrict__ zoff, int flags, const int2* __restrict__ band) {
   // The sums must be the sequential FP32 sums of the reference: no fused
   // multiply-add. The pragma is lexical, so the products and sums are written
   // here as plain operators (HIP's __fmul_rn/__fadd_rn are header functions
@@ -203,17 +207,18 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
   // Blocks are dealt round-robin to the 8 XCDs. Tiles go round-robin too
   // (the heavy tiles near the lateral origin are spread), and the windows
   // of one tile stay on one XCD, in order (they read the same spectrum
-  // patches). A bijection when the tile count is a multiple of 8.
+  // patches). A bijection when the tile count is a multiple of 8. Windows
+  // w0 .. w0 + nwin - 1 are launched.
   int tile, w;
   {
     const int b = blockIdx.x, tiles = items / nwin;
     if (tiles % 8 == 0) {
       const int n = b / 8;
       tile = (n / nwin) * 8 + b % 8;
-      w = n % nwin;
+      w = w0 + n % nwin;
     } else {
       tile = b / nwin;
-      w = b % nwin;
+      w = w0 + b % nwin;
     }
   }
   const int tid = threadIdx.x;
@@ -291,7 +296,7 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
         const size_t sp = (size_t)(clampi(fdy, -g.nx, g.nx - 1) + g.nx) * g.C +
                           (clampi(fdx, -g.nx, g.nx - 1) + g.nx);
         spv[t] = g.spec[(size_t)ea * C2 + (g.frame_pos ? sp : fixed)];
-        bv[t] = g.beta[r2];
+        if constexpr (!PRE) bv[t] = g.beta[r2];
       }
 #pragma unroll
       for (int t = 0; t < kBatch; ++t) {
@@ -306,7 +311,9 @@ k_gather_win(GatherArgs g, int ntx, int nwin, int items, const TileEntry* __rest
         const int k = offv[t] == 255 ? -1 : zlv[t] + offv[t] - z0;
         const bool hit = k >= 0 && k < ZW;
         const int kc = hit ? k : 0;
-        const float pim = bv[t] * spv[t].y, pre = bv[t] * spv[t].x;  // (-b Si, b Sr)
+        // value (-b Si, b Sr)
+        const float pim = PRE ? spv[t].y : bv[t] * spv[t].y;
+        const float pre = PRE ? spv[t].x : bv[t] * spv[t].x;
         float2 s = acc[kc][tid];
         s.x = s.x + (hit ? -pim : -0.0f);
         s.y = s.y + (hit ? pre : -0.0f);
@@ -382,7 +389,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
-                         hipStream_t s) {
+                         int zlo, int zhi, hipStream_t s) {
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
@@ -397,11 +404,24 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
     Timed tm("ewald_tiles", s);
     hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
   }
-  const int nwin = (V + ZW - 1) / ZW, items = tiles * nwin;
+  // banded output: only the windows that can meet the band are launched
+  // (the exact band is only known on the device)
+  int w0 = 0, w1 = (V + ZW - 1) / ZW - 1;
+  if (band) {
+    w0 = std::max(zlo, 0) / ZW;
+    w1 = std::min(zhi, V - 1) / ZW;
+  }
+  const int nwin = w1 - w0 + 1, items = tiles * nwin;
   Timed tm("ewald_gather", s);
-  hipLaunchKernelGGL((k_gather_win<ZW>), dim3(items), dim3(256), 0, s, g,
-                     ntx, nwin, items, (const TileEntry*)list, (const int4*)tinfo,
-                     (const uint8_t*)zoff, flags, band);
+  if (flags & 2) {
+    hipLaunchKernelGGL((k_gather_win<ZW, true>), dim3(items), dim3(256), 0, s, g, ntx, w0, nwin,
+                       items, (const TileEntry*)list, (const int4*)tinfo, (const uint8_t*)zoff,
+                       flags, band);
+  } else {
+    hipLaunchKernelGGL((k_gather_win<ZW, false>), dim3(items), dim3(256), 0, s, g, ntx, w0,
+                       nwin, items, (const TileEntry*)list, (const int4*)tinfo,
+                       (const uint8_t*)zoff, flags, band);
+  }
   return hipGetLastError();
 }
 

@@ -425,8 +425,12 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
     const double* winC = tukey_table(ctx, C, g->alpha_field);
     if (!winC) return set_err(ctx, MTD_ERR_HIP, "out of device memory (window)");
     const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+    // the spectra go to the gather: pre-weighted by the Ewald factor
+    EwaldTables t;
+    MTD_TRY(ewald_tables(ctx, g, &t));
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, iu, 1, crop, coef, mode, phu, posx, posy, n, winC,
-                                               twC, dx2, 0, work, out_F, ctx->stream),
+                                               twC, dx2, 0, t.beta, t.r2max, work, out_F,
+                                               ctx->stream),
                       "forward spectrum"));
   }
   return MTD_OK;
@@ -663,15 +667,16 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
                                                nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
-                                               posspec + A + a0, n, winC, twC, dx2, 1, work,
-                                               (float2*)spectra + (size_t)a0 * C * C,
+                                               posspec + A + a0, n, winC, twC, dx2, 1, nullptr, 0,
+                                               work, (float2*)spectra + (size_t)a0 * C * C,
                                                ctx->stream),
                       "forward spectrum"));
   }
   return MTD_OK;
 }
 
-// flags: 1 = write the count-normalised, centring-signed spectrum (no counts).
+// flags: 1 = write the count-normalised, centring-signed spectrum (no counts);
+// 2 = spectra already multiplied by the Ewald factor of their pupil pixel.
 // band != nullptr: the hit band is computed into it and only the heights that
 // meet it are written.
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
@@ -690,10 +695,15 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
   if (band)
     MTD_TRY(hip_check(ctx, launch_band(ag, A, g->dim_final, t.htab, t.r2max, band, ctx->stream),
                       "hit band"));
+  // Host-side bound of the hit band: heights h(r2) - fz with h(r2) in
+  // [h(r2max), R] and fz = h(|fm|^2) in [0, R] for every non-skipped angle.
+  const int V = g->dim_final;
+  const int zlo = V / 2 + (int)std::round(std::sqrt(t.R2 - (double)t.r2max) - g->rayon);
+  const int zhi = V / 2 + (int)std::round(g->rayon);
   MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
-                                       2 * g->nxmax, g->nxmax, g->dim_final, t.htab, t.beta,
+                                       2 * g->nxmax, g->nxmax, V, t.htab, t.beta,
                                        t.r2max, (float2*)vol, counts, scratch, flags, band,
-                                       ctx->stream),
+                                       zlo, zhi, ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
 }
@@ -721,15 +731,19 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
   if (!F || !twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
                                                nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
-                                               posspec + A + a0, n, winC, twC, dx2, 0, work,
-                                               F + (size_t)a0 * C * C, ctx->stream),
+                                               posspec + A + a0, n, winC, twC, dx2, 0, t.beta,
+                                               t.r2max, work, F + (size_t)a0 * C * C,
+                                               ctx->stream),
                       "forward spectrum"));
   }
-  return scatter_impl(ctx, g, F, 0, posspec, A, vol, counts);
+  // spectra pre-weighted by the Ewald factor (gather flag 2)
+  return scatter_impl(ctx, g, F, 0, posspec, A, vol, counts, 2);
 }
 
 static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
@@ -790,7 +804,7 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   // The gather writes the count-normalised, centring-signed spectrum, and
   // only in the height band that holds hits; the inverse transform treats the
   // rest as the zeros it is and never reads it.
-  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1, band));
+  MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1 | 2, band));
   return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, index_re, index_im, nullptr);
 }
 
