@@ -60,6 +60,7 @@ struct mtd_ctx {
   std::map<std::pair<int, uint32_t>, float*> tukey_signed;
   std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
   std::map<std::tuple<uint64_t, int, uint32_t>, float*> betatab;  // (R, r2max, k0) bits
+  std::map<std::tuple<uint64_t, int, uint32_t, int>, float*> beta2tab;  // + NXMAX
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   mtd::LsqPlan lsq;
   // per-kernel event timing (mtd_ctx_set_timing)
@@ -154,13 +155,14 @@ size_t unwrap_scratch_bytes(int C, int A);
 // Per-angle forward spectrum. from_iu=0: src are fields (U frame);
 // from_iu=1: src is IU and the field formation runs inside the load.
 // Output frame: shift_pos=0 -> unshifted F, 1 -> shifted by +posSpec.
-// beta != nullptr (shift_pos=0 only): bins pre-multiplied by the Ewald
-// factor of their pupil pixel (gather flag 2).
+// beta2 != nullptr (shift_pos=0 only): bins pre-multiplied by the Ewald
+// factor of their pupil pixel, beta2 being the C x C pupil-frame table
+// (gather flag 2).
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
                                int mode, const float* phu, const int* posx, const int* posy,
                                int A, const double* win, const float2* tw,
-                               float dx2, int shift_pos, const float* beta, int r2max,
+                               float dx2, int shift_pos, const float* beta2,
                                float2* work, float2* out, hipStream_t s);
 
 hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,

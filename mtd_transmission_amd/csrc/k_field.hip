@@ -341,28 +341,27 @@ struct FwdColOp {
   const float2* tw;
   float dx2;
   int shift_pos;
-  // beta != nullptr (unshifted output only): each bin is also multiplied by
-  // the Ewald factor beta(|fd|^2) of its pupil pixel, as the accumulation
+  // beta2 != nullptr (unshifted output only): each bin is also multiplied
+  // by the Ewald factor beta(|fd|^2) of its pupil pixel, as the accumulation
   // would (retroPropag_Born, Reconstruction/src/fonctions.cpp:927: one FP32
   // product, so the weighted value is bit-identical to the one the gather
   // would form); bins outside the pupil, which no hit reads, become 0.
-  const float* beta;
-  int r2max, nx;
+  // beta2 is the C x C table of those factors in the pupil frame.
+  const float* beta2;
   int px, py, bx, by;
   __device__ void begin(int a) {
     if (shift_pos) { px = posx[a]; py = posy[a]; } else { px = 0; py = 0; }
-    if (beta) { bx = posx[a]; by = posy[a]; }
+    if (beta2) { bx = posx[a]; by = posy[a]; }
   }
   __device__ float2 load(int a, int col, int row) const {
     return in[((size_t)a * C + row) * C + col];
   }
   __device__ void store(int a, int col, int row, float2 v) const {
     float2 s = make_float2(v.x * dx2, v.y * dx2);
-    if (beta) {
-      // unshifted bin (row, col) holds pupil pixel fd = wrap((col, row) + fm0)
-      const int fdx = (col + bx) % C - nx, fdy = (row + by) % C - nx;
-      const int r2 = fdx * fdx + fdy * fdy;
-      const float b = r2 <= r2max ? beta[r2] : 0.f;
+    if (beta2) {
+      // unshifted bin (row, col) holds pupil pixel fd = wrap((col, row) + fm0),
+      // i.e. pupil-frame index ((row + posy) % C, (col + posx) % C)
+      const float b = beta2[((row + by) % C) * C + (col + bx) % C];
       s = make_float2(b * s.x, b * s.y);
     }
     out[((size_t)a * C + (row + py) % C) * C + (col + px) % C] = s;
@@ -436,8 +435,8 @@ template <int C>
 static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
                         const double* coef, int mode, const float* phu, const int* posx,
                         const int* posy, int A, const double* win,
-                        const float2* tw, float dx2, int shift_pos, const float* beta,
-                        int r2max, float2* work, float2* out, hipStream_t s) {
+                        const float2* tw, float dx2, int shift_pos, const float* beta2,
+                        float2* work, float2* out, hipStream_t s) {
   FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode, phu,
                 make_float2(0.f, 0.f), nullptr, nullptr};
   {
@@ -445,7 +444,7 @@ static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
     hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
                        rows_block<C>(), 0, s, r);
   }
-  FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, beta, r2max, C / 2, 0, 0, 0, 0};
+  FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, beta2, 0, 0, 0, 0};
   Timed tm("fwd_cols", s);
   hipLaunchKernelGGL((k_cols<C, false, FwdColOp<C>>), cols_grid<C>(A), cols_block<C>(), 0, s,
                      c);
@@ -512,13 +511,13 @@ hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
                                int mode, const float* phu, const int* posx, const int* posy,
                                int A, const double* win, const float2* tw,
-                               float dx2, int shift_pos, const float* beta, int r2max,
+                               float dx2, int shift_pos, const float* beta2,
                                float2* work, float2* out, hipStream_t s) {
   switch (C) {
 #define X(n)                                                                         \
   case n:                                                                            \
     return fwd_c<n>(src, from_iu, crop, coef, mode, phu, posx, posy, A, win, tw, dx2, \
-                    shift_pos, beta, r2max, work, out, s);
+                    shift_pos, beta2, work, out, s);
     MTD_CROP_SIZES(X)
 #undef X
   }

@@ -233,7 +233,8 @@ int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
 // ---------------------------------------------------------------------------
 struct EwaldTables {
   const double* htab;
-  const float* beta;
+  const float* beta;   // per r2 <= r2max
+  const float* beta2;  // C x C, pupil frame: beta at fd = (c - NXMAX, r - NXMAX), 0 outside
   int r2tab;
   int r2max;
   double R2;
@@ -288,8 +289,28 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
   } else {
     d_b = bt->second;
   }
+  // the same factors laid out in the pupil frame, for the forward column
+  // pass (coalesced reads instead of a per-bin lookup by r2)
+  const int C = 2 * nx;
+  const auto b2key = std::make_tuple(bits_of(R), r2max, bits_of(k0), nx);
+  float* d_b2;
+  auto b2t = ctx->beta2tab.find(b2key);
+  if (b2t == ctx->beta2tab.end()) {
+    std::vector<float> b2((size_t)C * C);
+    for (int r = 0; r < C; ++r)
+      for (int c = 0; c < C; ++c) {
+        const int fdx = c - nx, fdy = r - nx, r2 = fdx * fdx + fdy * fdy;
+        b2[(size_t)r * C + c] = r2 <= r2max ? b[r2] : 0.f;
+      }
+    d_b2 = upload(b2.data(), b2.size());
+    if (!d_b2) return set_err(ctx, MTD_ERR_HIP, "out of device memory (beta table)");
+    ctx->beta2tab[b2key] = d_b2;
+  } else {
+    d_b2 = b2t->second;
+  }
   t->htab = d_h;
   t->beta = d_b;
+  t->beta2 = d_b2;
   t->r2tab = r2tab;
   t->r2max = r2max;
   t->R2 = R2;
@@ -429,8 +450,7 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
     EwaldTables t;
     MTD_TRY(ewald_tables(ctx, g, &t));
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, iu, 1, crop, coef, mode, phu, posx, posy, n, winC,
-                                               twC, dx2, 0, t.beta, t.r2max, work, out_F,
-                                               ctx->stream),
+                                               twC, dx2, 0, t.beta2, work, out_F, ctx->stream),
                       "forward spectrum"));
   }
   return MTD_OK;
@@ -503,6 +523,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (auto& kv : ctx->tukey_signed) (void)hipFree(kv.second);
   for (auto& kv : ctx->htab) (void)hipFree(kv.second);
   for (auto& kv : ctx->betatab) (void)hipFree(kv.second);
+  for (auto& kv : ctx->beta2tab) (void)hipFree(kv.second);
   for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);
   (void)hipFree(ctx->lsq.d_samples);
   (void)hipFree(ctx->lsq.d_lu);
@@ -667,7 +688,7 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
                                                nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
-                                               posspec + A + a0, n, winC, twC, dx2, 1, nullptr, 0,
+                                               posspec + A + a0, n, winC, twC, dx2, 1, nullptr,
                                                work, (float2*)spectra + (size_t)a0 * C * C,
                                                ctx->stream),
                       "forward spectrum"));
@@ -737,9 +758,8 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
     const int n = std::min(chunk, A - a0);
     MTD_TRY(hip_check(ctx, launch_fwd_spectrum(C, (const float2*)fields + (size_t)a0 * C * C, 0,
                                                nullptr, nullptr, field_mode(g), nullptr, posspec + a0,
-                                               posspec + A + a0, n, winC, twC, dx2, 0, t.beta,
-                                               t.r2max, work, F + (size_t)a0 * C * C,
-                                               ctx->stream),
+                                               posspec + A + a0, n, winC, twC, dx2, 0, t.beta2,
+                                               work, F + (size_t)a0 * C * C, ctx->stream),
                       "forward spectrum"));
   }
   // spectra pre-weighted by the Ewald factor (gather flag 2)
