@@ -8,7 +8,10 @@
 //
 // Pass 1 (rows): two real rows are packed into one complex FFT of length N
 // (two-step register FFT); only the C bins kx in [x0, x0+C) are separated and
-// written, to t1[a][y][d] (N x C per hologram).
+// written, to t1 (N x C per hologram) in column tiles of W bins (W = 32 when
+// it divides C): t1[a][d / W][y][d % W]. A row pass stores 256-byte runs, and
+// the blocks of the column pass that share a tile read it as one contiguous
+// N x W slab.
 // Pass 2 (columns): each of the C columns of t1 is transformed along y and
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
 #include "argmax.hpp"
@@ -19,16 +22,38 @@
 namespace mtd {
 
 template <int N>
+constexpr int holo_cols_nc() {  // a power of two <= 16: divides every crop side
+  constexpr int by_thr = (N > 512 ? 256 : 1024) / Fft2<N, false>::T;
+  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS_COL * 4);
+  return divisor_le(16, by_thr < by_lds ? by_thr : by_lds);
+}
+
+// width of the column tiles of t1: the largest power of two from 32 down to
+// the column block width nc (a power of two dividing C) that divides C
+static int holo_tile_w(int C, int nc) {
+  for (int w = 32; w > nc; w /= 2)
+    if (C % w == 0) return w;
+  return nc;
+}
+
+template <int N>
 constexpr int holo_pairs() {  // row pairs per block: <= 256 threads, <= 64 KiB
   constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS_ROW * 4);
   constexpr int cap = 256 / Fft2<N, false>::T;
   return divisor_le(N / 2, by_lds < cap ? (by_lds > 0 ? by_lds : 1) : cap);
 }
 
+// waves per SIMD the row kernel is compiled for (blocks of at most 256
+// threads; four such blocks fit in the LDS of a CU)
+template <int N>
+constexpr int holo_rows_waves() {
+  return N == 768 ? 3 : N <= 1024 ? 4 : 2;
+}
+
 template <int N, typename TIn>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, holo_rows_waves<N>())
 k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
-            const float2* __restrict__ tw, int x0, int C,
+            const float2* __restrict__ tw, int x0, int C, int W,
             float2* __restrict__ t1) {
   using F = Fft2<N, false>;
   constexpr int NP = holo_pairs<N>();
@@ -58,8 +83,15 @@ k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
   // (the 1/2 is folded into the column pass scale). The row goes through the
   // float storage twice: real parts give X0.x and X1.y, imaginary parts
   // X0.y and X1.x.
-  float* o0 = (float*)(t1 + ((size_t)a * N + y0) * C);
-  float* o1 = o0 + 2 * C;
+  // The values of the first pass stay in registers (crops up to N/2 wide)
+  // so that each output is stored once, as a whole complex value.
+  // Bin d of row y0 sits at q0[tile(d)], of row y1 at q1[tile(d)].
+  float2* q0 = t1 + (size_t)a * N * C + (size_t)y0 * W;
+  float2* q1 = q0 + W;
+  auto tile = [W](int d) { return (d / W) * (N * W) + d % W; };
+  constexpr int DK = (N / 2 + F::T - 1) / F::T;
+  float r0[DK], r1[DK];
+  const bool keep = C <= N / 2;  // uniform
 #pragma unroll
   for (int part = 0; part < 2; ++part) {
     __syncthreads();  // the storage is free
@@ -68,26 +100,41 @@ k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
       for (int k2 = 0; k2 < F::L2; ++k2) buf[n2 + F::L1 * k2] = part ? u[k2].y : u[k2].x;
     }
     __syncthreads();
-    for (int d = n2; d < C; d += F::T) {
-      const int k = x0 + d;
-      const int m = (N - k) % N;
-      const float zk = buf[k], zm = buf[m];
-      if (part == 0) {
-        o0[2 * d] = zk + zm;
-        o1[2 * d + 1] = zm - zk;
-      } else {
-        o0[2 * d + 1] = zk - zm;
-        o1[2 * d] = zk + zm;
+    if (keep) {
+#pragma unroll
+      for (int j = 0; j < DK; ++j) {
+        const int d = n2 + F::T * j;
+        if (d < C) {
+          const int k = x0 + d;
+          const int m = (N - k) % N;
+          const float zk = buf[k], zm = buf[m];
+          if (part == 0) {
+            r0[j] = zk + zm;
+            r1[j] = zm - zk;
+          } else {
+            q0[tile(d)] = make_float2(r0[j], zk - zm);
+            q1[tile(d)] = make_float2(zk + zm, r1[j]);
+          }
+        }
+      }
+    } else {
+      float* o0 = (float*)q0;
+      float* o1 = (float*)q1;
+      for (int d = n2; d < C; d += F::T) {
+        const int k = x0 + d;
+        const int m = (N - k) % N;
+        const float zk = buf[k], zm = buf[m];
+        const int e = 2 * tile(d);
+        if (part == 0) {
+          o0[e] = zk + zm;
+          o1[e + 1] = zm - zk;
+        } else {
+          o0[e + 1] = zk - zm;
+          o1[e] = zk + zm;
+        }
       }
     }
   }
-}
-
-template <int N>
-constexpr int holo_cols_nc() {  // a power of two <= 16: divides every crop side
-  constexpr int by_thr = (N > 512 ? 256 : 1024) / Fft2<N, false>::T;
-  constexpr int by_lds = kLdsBlockMax / (Fft2<N, false>::LDS_COL * 4);
-  return divisor_le(16, by_thr < by_lds ? by_thr : by_lds);
 }
 
 template <int N>
@@ -95,10 +142,17 @@ constexpr int holo_cols_threads() {
   return holo_cols_nc<N>() * Fft2<N, false>::T;
 }
 
+// waves per SIMD the column kernel is compiled for: from N = 1024 its
+// register FFT plus the argmax take more than a third of the VGPRs
 template <int N>
-__global__ void __launch_bounds__(holo_cols_threads<N>(), col_waves_per_simd<N>())
+constexpr int holo_cols_waves() {
+  return N >= 1024 ? 2 : col_waves_per_simd<N>();
+}
+
+template <int N>
+__global__ void __launch_bounds__(holo_cols_threads<N>(), holo_cols_waves<N>())
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
-            int C, int x0, int y0, float scale, float2* __restrict__ crop,
+            int C, int W, int x0, int y0, float scale, float2* __restrict__ crop,
             float2* __restrict__ part) {
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
@@ -106,10 +160,15 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   const int2 ab = xcd_block();
   const int a = ab.x, d0 = ab.y * NC;
   const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
-  const float2* src = t1 + (size_t)a * N * C + d0 + c;
+  // this block's contiguous N x NC tile through a buffer view: one lane
+  // offset, uniform row offsets
+  const BufView src(t1 + (size_t)a * N * C + (size_t)(d0 / W) * N * W + d0 % W,
+                    (unsigned)((N * W - d0 % W) * sizeof(float2)));
+  const unsigned lane = (unsigned)((n2 * W + c) * sizeof(float2));
   float2 v[F::L1], u[F::L2];
 #pragma unroll
-  for (int n1 = 0; n1 < F::L1; ++n1) v[n1] = src[(size_t)(F::L2 * n1 + n2) * C];
+  for (int n1 = 0; n1 < F::L1; ++n1)
+    v[n1] = src.f2(lane, (unsigned)(F::L2 * n1 * W * sizeof(float2)));
   F::run(v, u, n2, typename F::template Col<NC>{lds + c}, tw);
   // argmax partial of the crop values this block writes (coordSpec's
   // first stage, fused: the crop is not read again for it)
@@ -145,20 +204,22 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
                          const float2* tw, int x0, int C, float2* t1,
                          hipStream_t s) {
   constexpr int NP = holo_pairs<N>();
+  if (C % holo_cols_nc<N>()) return hipErrorInvalidValue;  // whole column tiles
+  const int W = holo_tile_w(C, holo_cols_nc<N>());
   dim3 grid(N / (2 * NP), A), block(NP * Fft2<N, false>::T);
   Timed tm("holo_rows", s);
   switch (dtype) {
     case MTD_U8:
       hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
-                         (const uint8_t*)holos, win, tw, x0, C, t1);
+                         (const uint8_t*)holos, win, tw, x0, C, W, t1);
       break;
     case MTD_U16:
       hipLaunchKernelGGL((k_holo_rows<N, uint16_t>), grid, block, 0, s,
-                         (const uint16_t*)holos, win, tw, x0, C, t1);
+                         (const uint16_t*)holos, win, tw, x0, C, W, t1);
       break;
     default:
       hipLaunchKernelGGL((k_holo_rows<N, float>), grid, block, 0, s,
-                         (const float*)holos, win, tw, x0, C, t1);
+                         (const float*)holos, win, tw, x0, C, W, t1);
       break;
   }
   return hipGetLastError();
@@ -172,7 +233,7 @@ static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
   const float scale = (float)(0.5 / ((double)N * (double)N));
   Timed tm("holo_cols", s);
   hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(holo_cols_threads<N>()), 0, s,
-                     t1, tw, C, x0, y0, scale, crop, part);
+                     t1, tw, C, holo_tile_w(C, NC), x0, y0, scale, crop, part);
   return hipGetLastError();
 }
 

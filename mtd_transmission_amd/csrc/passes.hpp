@@ -14,6 +14,22 @@
 
 namespace mtd {
 
+// Raw buffer view of one device array (at most 2 GiB). A load takes a 32-bit
+// per-lane byte offset and a uniform byte offset held in a scalar register,
+// so a run of loads at uniform strides keeps one address register in place of
+// a 64-bit address per load. Offsets at or beyond `bytes` read as zero (the
+// buffer range check).
+struct BufView {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufView(const void* p, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                            0x00020000)) {}
+  __device__ __forceinline__ float2 f2(unsigned lane_bytes, unsigned uni_bytes) const {
+    return __builtin_bit_cast(float2,
+                              __builtin_amdgcn_raw_buffer_load_b64(r, lane_bytes, uni_bytes, 0));
+  }
+};
+
 // Row passes: NT transforms per block (at most 256 threads, 64 KiB LDS).
 template <int L>
 constexpr int rows_nt() {
