@@ -474,6 +474,7 @@ struct NormXOp {  // rows along x; a = z, line = y
 
 template <int V>
 struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
+  static constexpr int kColThreads = 512;  // complex in and out
   float2* buf;
   const float2* tw;
   const int2* band;
@@ -529,8 +530,8 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
   PlaneYOp<V> y{work, tw, band};
   {
     Timed tm("vol_y", s);
-    hipLaunchKernelGGL((k_cols<V, true, PlaneYOp<V>>), cols_grid<V>(V), cols_block<V>(), 0, s,
-                       y);
+    hipLaunchKernelGGL((k_cols<V, true, PlaneYOp<V>>), (cols_grid<V, PlaneYOp<V>>(V)),
+                       (cols_block<V, PlaneYOp<V>>()), 0, s, y);
   }
   ZOutOp<V> z{work, re, im, tw, df3, cte, gsign, band, 0, V - 1};
   Timed tm("vol_z", s);

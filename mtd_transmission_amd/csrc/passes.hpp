@@ -41,17 +41,31 @@ constexpr int rows_nt() {
 // even for float outputs), at most 1024 threads and 64 KiB LDS; lines
 // longer than 512 keep 8 columns (their register FFTs allow 3 waves per
 // SIMD, which 4-wave blocks fill).
-template <int L>
+// MAXT lowers the thread cap (see op_col_threads).
+template <int L, int MAXT = 1024>
 constexpr int cols_nc() {
-  constexpr int by_thr = (L > 512 ? 256 : 1024) / Fft2<L, false>::T;
+  constexpr int cap = L > 512 ? 256 : 1024;
+  constexpr int by_thr = (MAXT < cap ? MAXT : cap) / Fft2<L, false>::T;
   constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS_COL * 4);
   constexpr int m = by_thr < by_lds ? by_thr : by_lds;
   return divisor_le(L, m < 32 ? m : 32);
 }
-template <int L>
+template <int L, int MAXT = 1024>
 constexpr int cols_threads() {
-  return cols_nc<L>() * Fft2<L, false>::T;
+  return cols_nc<L, MAXT>() * Fft2<L, false>::T;
 }
+// An operation may cap the threads of its column blocks with a static
+// kColThreads: two half-size blocks per CU overlap one block's barriers with
+// the other's memory traffic, which pays where the stores are whole 128-byte
+// lines either way (complex outputs) but not for float outputs.
+template <class Op, class = void>
+struct op_col_threads {
+  static constexpr int value = 1024;
+};
+template <class Op>
+struct op_col_threads<Op, std::void_t<decltype(Op::kColThreads)>> {
+  static constexpr int value = Op::kColThreads;
+};
 // Column blocks are up to 1024 threads; ask the compiler for at least 4
 // waves per SIMD (at most 128 VGPRs) so that a CU keeps 16 waves in flight,
 // 3 for lines longer than 512 (a 32 + 32 point register FFT needs more).
@@ -119,9 +133,10 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
 }
 
 template <int L, bool INV, class Op>
-__global__ void __launch_bounds__(cols_threads<L>(), col_waves_per_simd<L>()) k_cols(Op op) {
+__global__ void __launch_bounds__((cols_threads<L, op_col_threads<Op>::value>()),
+                                  col_waves_per_simd<L>()) k_cols(Op op) {
   using F = Fft2<L, INV>;
-  constexpr int NC = cols_nc<L>();
+  constexpr int NC = cols_nc<L, op_col_threads<Op>::value>();
   __shared__ float lds[NC * F::LDS_COL];
   const int2 ab = xcd_block();
   const int a = ab.x, c0 = ab.y * NC;
@@ -146,13 +161,13 @@ template <int L>
 inline dim3 rows_block() {
   return dim3(rows_nt<L>() * Fft2<L, false>::T);
 }
-template <int L>
+template <int L, class Op = void>
 inline dim3 cols_grid(int A) {
-  return dim3(L / cols_nc<L>(), A);
+  return dim3(L / cols_nc<L, op_col_threads<Op>::value>(), A);
 }
-template <int L>
+template <int L, class Op = void>
 inline dim3 cols_block() {
-  return dim3(cols_threads<L>());
+  return dim3(cols_threads<L, op_col_threads<Op>::value>());
 }
 
 }  // namespace mtd
