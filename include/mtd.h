@@ -173,6 +173,30 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
                           int dtype, int A, const uint8_t* mask,
                           int32_t* posspec, float* index_re, float* index_im);
 
+/* ---- Born forward model (Simul_holo) and its adjoint -------------------- */
+/* The simulator's hologram-spectrum synthesis (Simul_holo/main.cpp:106-109,
+ * 162-171; calcHolo, Simul_holo/src/fonctions.cpp:109-140): TF3D =
+ * fftshift3D(FFT3D(fftshift3D(dn))) * Tp^3 (TF3Dcplx,
+ * Simul_holo/src/FFT_fonctions.cpp:301-318; Tp = 1/(V*delta_f)), then for
+ * every angle and pupil pixel fd (|fd|^2 < COEF_NA_OBJ_LIMIT*NXMAX^2)
+ * S[fd] = -i kv / (sqrt(R^2 - |fd|^2)/R) * TF3D[voxel], kv = 2*3.14159/LAMBDA,
+ * at retroPropag_Born's voxel (the same integer map as calcHolo's
+ * round(fd - fi)). dn_re/dn_im: index contrast n - n0, float V^3 (dn_im may
+ * be NULL). spectra: [A][C][C] c32 in the frame of mtd_field_to_spectrum,
+ * zero outside the pupil and for skipped angles. */
+int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re,
+                        const float* dn_im, const int32_t* posspec, int A,
+                        float* spectra);
+
+/* Adjoint of mtd_forward_spectra (no counterpart in the reference; the
+ * gradient of the adjoint-iterative reconstruction): out = A^H spectra, the
+ * sum over angles of i q S on each cap voxel (ascending angle order, as
+ * retroPropag_Born) followed by the centred, unnormalised backward 3-D FFT
+ * times Tp^3. out_re/out_im: float V^3. */
+int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                        const int32_t* posspec, int A, float* out_re,
+                        float* out_im);
+
 #ifdef __cplusplus
 }
 #endif

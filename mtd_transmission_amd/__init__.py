@@ -75,6 +75,8 @@ SIGNATURES = {
     "mtd_ewald_accumulate": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_finalize": (_I, [_P, _GP, _P, _P, _P, _P, _P]),
     "mtd_reconstruct_plain": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P, _P]),
+    "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
+    "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
 }
 
 
@@ -285,3 +287,20 @@ class Context:
                                                 _dtype_code(holos), holos.shape[0], mp,
                                                 _ptr(posspec), _ptr(index_re),
                                                 _ptr(index_im)), "reconstruct_plain")
+
+    # ---- Born forward model and adjoint ---------------------------------
+    def forward_spectra(self, g: Geom, dn_re, dn_im, posspec, spectra):
+        """spectra[A][C][C] c32 = A(dn) (Simul_holo's calcHolo over all angles)."""
+        self._stream()
+        A = posspec.numel() // 2
+        self._check(lib().mtd_forward_spectra(self.h, ctypes.byref(g), _ptr(dn_re),
+                                              _ptr(dn_im), _ptr(posspec), A,
+                                              _ptr(spectra)), "forward_spectra")
+
+    def adjoint_spectra(self, g: Geom, spectra, posspec, out_re, out_im):
+        """(out_re, out_im) = A^H spectra."""
+        self._stream()
+        A = posspec.numel() // 2
+        self._check(lib().mtd_adjoint_spectra(self.h, ctypes.byref(g), _ptr(spectra),
+                                              _ptr(posspec), A, _ptr(out_re),
+                                              _ptr(out_im)), "adjoint_spectra")

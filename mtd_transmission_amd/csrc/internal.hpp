@@ -61,6 +61,7 @@ struct mtd_ctx {
   std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
   std::map<std::tuple<uint64_t, int, uint32_t>, float*> betatab;  // (R, r2max, k0) bits
   std::map<std::tuple<uint64_t, int, uint32_t, int>, float*> beta2tab;  // + NXMAX
+  std::map<std::tuple<uint64_t, int, uint64_t, uint64_t>, float*> qtab;  // (R, r2max, lambda, n0)
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   mtd::LsqPlan lsq;
   // per-kernel event timing (mtd_ctx_set_timing)
@@ -182,6 +183,16 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
                          int zlo, int zhi, hipStream_t s);
 size_t gather_scratch_bytes(int V, int A);
+// Born forward model: work = centred 3-D spectrum of the contrast (re, im;
+// im may be nullptr) times scale (Tp^3 * gsign), stored only on the heights
+// of the device band.
+hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
+                                   float scale, const int2* band, const float2* tw,
+                                   hipStream_t s);
+// Per-angle cap samples -i q(|fd|^2) vol[voxel] into spec[A][C][C] (+posSpec frame).
+hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,
+                             const float* q, int r2max, int nx, int C, int V, float2* spec,
+                             hipStream_t s);
 // prenorm=1: vol already divided by counts and centring-signed (gather flag 1).
 // band != nullptr: heights outside the band are zero and are not read.
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,

@@ -192,6 +192,8 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
 
 // flags: bit0 = normalise (divide by counts, 0 -> 1) and apply the centring
 // sign (-1)^(x+y+z) of the 3-D inverse FFT; counts are then not written.
+// bit2 = the plain sums with that centring sign, no counts (the adjoint of
+// the Born forward model).
 // PRE: the spectra are already multiplied by the Ewald factor beta of their
 // pupil pixel (the same FP32 product), so beta is not looked up here.
 template <int ZW, bool PRE>
@@ -232,11 +234,11 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
     if (z1 < b.x || z0 > b.y) return;
   }
   const int4 info = tinfo[tile];
-  const bool norm = flags & 1;
+  const bool norm = flags & 1, sgn = flags & 5;
   if (info.x == 0 || z1 < info.y || z0 > info.z) {
     for (int z = z0; z <= z1; ++z) {
       g.vol[z * plane + col] = make_float2(0.f, 0.f);
-      if (!norm) g.cnt[z * plane + col] = 0;
+      if (!sgn) g.cnt[z * plane + col] = 0;
     }
     return;
   }
@@ -328,9 +330,12 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
     if (z > z1) break;
     const float2 s = acc[k][tid];
     const int c = cn[k][tid];
-    if (norm) {
-      const float d = (float)(c == 0 ? 1 : c);
-      float2 t = make_float2(s.x / d, s.y / d);
+    if (sgn) {
+      float2 t = s;
+      if (norm) {
+        const float d = (float)(c == 0 ? 1 : c);
+        t = make_float2(s.x / d, s.y / d);
+      }
       if ((X + Y + z) & 1) { t.x = -t.x; t.y = -t.y; }
       g.vol[z * plane + col] = t;
     } else {
@@ -539,6 +544,107 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Born forward model (Simul_holo/main.cpp:106-109 and calcHolo,
+// Simul_holo/src/fonctions.cpp:109-140): the centred 3-D spectrum of the
+// index contrast, TF3D = fftshift3D(FFT3D(fftshift3D(dn))) * Tp^3
+// (TF3Dcplx, Simul_holo/src/FFT_fonctions.cpp:301-318), sampled on each
+// angle's Ewald cap. Both shifts are (-1)^(x+y+z) sign flips (even V), the
+// output ones times (-1)^(3V/2) = gsign. Only the heights of the hit band are
+// stored, the only ones the caps read.
+// ---------------------------------------------------------------------------
+template <int V>
+struct ContrastXOp {  // rows along x of the contrast volume; a = z, line = y
+  const float* re;
+  const float* im;  // nullptr: real contrast
+  float2* out;
+  const float2* tw;
+  __device__ void begin(int) {}
+  __device__ float2 load(int z, int y, int x) const {
+    const size_t i = ((size_t)z * V + y) * V + x;
+    float2 t = make_float2(re[i], im ? im[i] : 0.f);
+    if ((x + y + z) & 1) { t.x = -t.x; t.y = -t.y; }
+    return t;
+  }
+  __device__ void store(int z, int y, int x, float2 v) const {
+    out[((size_t)z * V + y) * V + x] = v;
+  }
+};
+
+template <int V>
+struct SpectrumZOp {  // lines along z, in place; heights outside the band are not stored
+  static constexpr int kColThreads = 512;  // complex in and out
+  float2* buf;
+  const float2* tw;
+  float scale;  // Tp^3 * gsign
+  const int2* band;
+  int lo, hi;
+  __device__ void begin(int) {
+    const int2 b = *band;
+    lo = b.x;
+    hi = b.y;
+  }
+  __device__ float2 load(int y, int x, int z) const { return buf[((size_t)z * V + y) * V + x]; }
+  __device__ void store(int y, int x, int z, float2 v) const {
+    if (z < lo || z > hi) return;
+    const float s = ((x + y + z) & 1) ? -scale : scale;
+    buf[((size_t)z * V + y) * V + x] = make_float2(v.x * s, v.y * s);
+  }
+};
+
+template <int V>
+static hipError_t contrast_spectrum_v(const float* re, const float* im, float2* work,
+                                      float scale, const int2* band, const float2* tw,
+                                      hipStream_t s) {
+  ContrastXOp<V> x{re, im, work, tw};
+  {
+    Timed tm("fwd_vol_x", s);
+    hipLaunchKernelGGL((k_rows<V, false, ContrastXOp<V>>), rows_grid<V>(V), rows_block<V>(), 0,
+                       s, x);
+  }
+  PlaneYOp<V> y{work, tw, nullptr};
+  {
+    Timed tm("fwd_vol_y", s);
+    hipLaunchKernelGGL((k_cols<V, false, PlaneYOp<V>>), (cols_grid<V, PlaneYOp<V>>(V)),
+                       (cols_block<V, PlaneYOp<V>>()), 0, s, y);
+  }
+  SpectrumZOp<V> z{work, tw, scale, band, 0, V - 1};
+  Timed tm("fwd_vol_z", s);
+  hipLaunchKernelGGL((k_cols<V, false, SpectrumZOp<V>>), (cols_grid<V, SpectrumZOp<V>>(V)),
+                     (cols_block<V, SpectrumZOp<V>>()), 0, s, z);
+  return hipGetLastError();
+}
+
+// calcHolo's sampling, in the frame of the per-angle spectra (+posSpec frame:
+// bin (row, col) is pupil pixel fd = (col - NXMAX, row - NXMAX)): the voxel is
+// retroPropag_Born's (the simulator's fobj = round(fd - fi) in centred
+// coordinates is the same integer map), and the value is
+// -i * q(|fd|^2) * TF3D[voxel] with q = kv / (sqrt(R^2 - |fd|^2) / R). Bins
+// outside the pupil, of skipped angles, or whose height leaves the volume
+// are 0.
+__global__ void __launch_bounds__(256)
+k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
+             const double* __restrict__ htab, const float* __restrict__ q, int r2max, int nx,
+             int C, int V, float2* __restrict__ spec) {
+  const int a = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= C * C) return;
+  const int fdx = i % C - nx, fdy = i / C - nx;
+  const int r2 = fdx * fdx + fdy * fdy;
+  const AngleGeo ag = ageo[a];
+  float2 s = make_float2(0.f, 0.f);
+  if (!ag.skip && r2 <= r2max) {
+    const int z = (int)round_away(htab[r2] - ag.fz) + V / 2;
+    if (z >= 0 && z < V) {
+      const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+      const float2 t = vol[((size_t)z * V + Y) * V + X];
+      const float w = q[r2];
+      s = make_float2(w * t.y, -(w * t.x));
+    }
+  }
+  spec[(size_t)a * C * C + i] = s;
+}
+
 #define MTD_VOL_SIZES(X) \
   X(32) X(64) X(128) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(704) X(768) X(1024)
 
@@ -554,6 +660,28 @@ hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
 #undef X
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
+                                   float scale, const int2* band, const float2* tw,
+                                   hipStream_t s) {
+  switch (V) {
+#define X(n) \
+  case n:    \
+    return contrast_spectrum_v<n>(re, im, work, scale, band, tw, s);
+    MTD_VOL_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,
+                             const float* q, int r2max, int nx, int C, int V, float2* spec,
+                             hipStream_t s) {
+  Timed tm("cap_sample", s);
+  hipLaunchKernelGGL(k_cap_sample, dim3((C * C + 255) / 256, A), dim3(256), 0, s, vol, ageo,
+                     htab, q, r2max, nx, C, V, spec);
+  return hipGetLastError();
 }
 
 bool vol_len_supported(int V) {

@@ -235,6 +235,7 @@ struct EwaldTables {
   const double* htab;
   const float* beta;   // per r2 <= r2max
   const float* beta2;  // C x C, pupil frame: beta at fd = (c - NXMAX, r - NXMAX), 0 outside
+  const float* q;      // Born forward weights per r2 <= r2max (born_weights)
   int r2tab;
   int r2max;
   double R2;
@@ -311,6 +312,7 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
   t->htab = d_h;
   t->beta = d_b;
   t->beta2 = d_b2;
+  t->q = nullptr;
   t->r2tab = r2tab;
   t->r2max = r2max;
   t->R2 = R2;
@@ -524,6 +526,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (auto& kv : ctx->htab) (void)hipFree(kv.second);
   for (auto& kv : ctx->betatab) (void)hipFree(kv.second);
   for (auto& kv : ctx->beta2tab) (void)hipFree(kv.second);
+  for (auto& kv : ctx->qtab) (void)hipFree(kv.second);
   for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);
   (void)hipFree(ctx->lsq.d_samples);
   (void)hipFree(ctx->lsq.d_lu);
@@ -696,13 +699,45 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   return MTD_OK;
 }
 
+// Born forward-model weight of pupil radius r2 (calcHolo,
+// Simul_holo/src/fonctions.cpp:115-122,136, PI = 3.14159 there): the product
+// cteNormalisation * ctePot2UBorn * cteInd2Pot = -i kv (FP64), divided by
+// sdz = sqrt(R^2 - r2) / R; q(r2) = (float)(kv / sdz), the bin value being
+// -i q TF3D. The adjoint gather uses the same q: its built-in factor i gives
+// conj(-i q) = i q.
+static int born_weights(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
+  const double PI = 3.14159;
+  const double kv = 2 * PI / g->lambda, k0 = kv * g->n0;
+  // |(-1/(2 PI)) * (-i PI/k0) * (-2 kv^2 n0)|
+  const double c = (1.0 / (2 * PI)) * (PI / k0) * (2 * kv * kv * g->n0);
+  const double R = g->rayon;
+  const auto key = std::make_tuple(bits_of(R), t->r2max, bits_of(g->lambda), bits_of(g->n0));
+  auto it = ctx->qtab.find(key);
+  if (it != ctx->qtab.end()) {
+    t->q = it->second;
+    return MTD_OK;
+  }
+  std::vector<float> q(t->r2max + 1);
+  for (int r = 0; r <= t->r2max; ++r) {
+    const double sdz = std::sqrt(t->R2 - (double)r) / R;
+    q[r] = (float)(c / sdz);
+  }
+  float* d_q = upload(q.data(), q.size());
+  if (!d_q) return set_err(ctx, MTD_ERR_HIP, "out of device memory (forward weights)");
+  ctx->qtab[key] = d_q;
+  t->q = d_q;
+  return MTD_OK;
+}
+
 // flags: 1 = write the count-normalised, centring-signed spectrum (no counts);
-// 2 = spectra already multiplied by the Ewald factor of their pupil pixel.
+// 2 = spectra already multiplied by the Ewald factor of their pupil pixel;
+// 4 = the centring-signed plain sums (no counts).
 // band != nullptr: the hit band is computed into it and only the heights that
-// meet it are written.
+// meet it are written. weights: per-r2 factors in place of beta (the
+// adjoint of the Born forward model), or nullptr.
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
                         const int32_t* posspec, int A, float* vol, int32_t* counts,
-                        int flags = 0, int2* band = nullptr) {
+                        int flags = 0, int2* band = nullptr, const float* weights = nullptr) {
   // per-voxel hit counts are accumulated in 16 bits (one hit per angle)
   if (A > 65535) return set_err(ctx, MTD_ERR_UNSUPPORTED, "more than 65535 angles per volume");
   EwaldTables t;
@@ -722,8 +757,9 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
   const int zlo = V / 2 + (int)std::round(std::sqrt(t.R2 - (double)t.r2max) - g->rayon);
   const int zhi = V / 2 + (int)std::round(g->rayon);
   MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
-                                       2 * g->nxmax, g->nxmax, V, t.htab, t.beta,
-                                       t.r2max, (float2*)vol, counts, scratch, flags, band,
+                                       2 * g->nxmax, g->nxmax, V, t.htab,
+                                       weights ? weights : t.beta, t.r2max, (float2*)vol, counts,
+                                       scratch, flags, band,
                                        zlo, zhi, ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
@@ -766,9 +802,10 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   return scatter_impl(ctx, g, F, 0, posspec, A, vol, counts, 2);
 }
 
+// scale > 0: output factor in place of delta_f^3 * (-1/(2 n0 kv^2)).
 static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
                          const int32_t* counts, int prenorm, const int2* band, float* index_re,
-                         float* index_im, uint8_t* otf) {
+                         float* index_im, uint8_t* otf, double scale = 0) {
   const int V = g->dim_final;
   float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
   const float2* twV = twiddle(ctx, V);
@@ -776,8 +813,8 @@ static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
   // Reconstruction/main.cpp:346-347 (PI = M_PI there) and the delta_f^3 of
   // TF3Dcplx_INV (src/FFT_fonctions.cpp:399)
   const double kv = 2 * M_PI / g->lambda;
-  const double cte = -1.0 / (2 * g->n0 * kv * kv);
-  const float df3 = (float)std::pow(g->delta_f, 3);
+  const double cte = scale > 0 ? 1.0 : -1.0 / (2 * g->n0 * kv * kv);
+  const float df3 = scale > 0 ? (float)scale : (float)std::pow(g->delta_f, 3);
   const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
   return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, prenorm, band, work, df3,
                                         (float)cte, gsign, index_re, index_im, otf, twV,
@@ -791,6 +828,61 @@ int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_
   MTD_TRY(validate(ctx, g, 0, NEED_VOL));
   if (!vol || !counts || !index_re || !index_im) return set_err(ctx, MTD_ERR_ARG, "null array");
   return finalize_impl(ctx, g, vol, counts, 0, nullptr, index_re, index_im, otf);
+}
+
+// ---- Born forward model and its adjoint ----------------------------------
+// voxel size of the reconstruction, 1/(V delta_f): Simul_holo's Tp_Tomo
+static double tomo_pixel(const mtd_geom* g) { return 1.0 / (g->dim_final * g->delta_f); }
+
+int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re, const float* dn_im,
+                        const int32_t* posspec, int A, float* spectra) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (A == 0) return MTD_OK;
+  if (!dn_re || !posspec || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int V = g->dim_final, C = 2 * g->nxmax;
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  MTD_TRY(born_weights(ctx, g, &t));
+  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)A * sizeof(AngleGeo));
+  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+  float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
+  const float2* twV = twiddle(ctx, V);
+  if (!ag || !band || !work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
+                                          t.r2tab, ag, ctx->stream), "angle geometry"));
+  MTD_TRY(hip_check(ctx, launch_band(ag, A, V, t.htab, t.r2max, band, ctx->stream), "hit band"));
+  const double tp = tomo_pixel(g);
+  const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
+  MTD_TRY(hip_check(ctx, launch_contrast_spectrum(V, dn_re, dn_im, work,
+                                                  (float)(tp * tp * tp) * gsign, band, twV,
+                                                  ctx->stream),
+                    "contrast spectrum"));
+  return hip_check(ctx, launch_cap_sample(work, ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
+                                          (float2*)spectra, ctx->stream),
+                   "cap sampling");
+}
+
+int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                        const int32_t* posspec, int A, float* out_re, float* out_im) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (!out_re || !out_im || (A > 0 && (!spectra || !posspec)))
+    return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int V = g->dim_final;
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  MTD_TRY(born_weights(ctx, g, &t));
+  float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
+  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+  if (!vol || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  // sum over angles of i q S on each cap voxel (conj of the forward weight),
+  // centring-signed, then the unnormalised backward 3-D FFT times Tp^3
+  MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, (float*)vol, nullptr, 4,
+                       band, t.q));
+  const double tp = tomo_pixel(g);
+  return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, out_re, out_im, nullptr,
+                       tp * tp * tp);
 }
 
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,

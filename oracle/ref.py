@@ -673,6 +673,102 @@ def finalize(vol, cnt, g: Geom, prec="f64"):
     return idx, otf.reshape(V, V, V)
 
 
+# --------------------------------------------------------------------------
+# Born forward model (Simul_holo) and its adjoint
+# --------------------------------------------------------------------------
+# ``#define PI 3.14159`` at Simul_holo/src/fonctions.cpp:10 (calcHolo).
+PI_SIMUL = 3.14159
+
+
+def tomo_pixel(g: Geom):
+    """Voxel size 1/(V delta_f) (Simul_holo/src/manip.cpp:59: Tp_Tomo =
+    Tp_Uborn * 2 NXMAX / dim_final)."""
+    return 1.0 / (g.dim_final * g.delta_f)
+
+
+def born_weight(g: Geom):
+    """calcHolo's factor (Simul_holo/src/fonctions.cpp:115-122,136):
+    cteNormalisation * ctePot2UBorn * cteInd2Pot / sdz with sdz =
+    sqrt(R^2 - |fd|^2)/R, in FP64. Returns the complex factor per bin of the
+    C x C pupil frame (0 outside the pupil)."""
+    kv = 2 * PI_SIMUL / g.lam
+    k0 = kv * g.n0
+    cte = (-1.0 / (2 * PI_SIMUL)) * (-1j * PI_SIMUL / k0) * (-2 * kv * kv * g.n0)
+    _, _, pupil, h, _ = pupil_tables(g)
+    sdz = np.where(pupil, h / g.rayon, 1.0)
+    return np.where(pupil, cte / sdz, 0.0)
+
+
+def contrast_spectrum(dn, g: Geom, prec="f64"):
+    """TF_obj = fftshift3D(TF3Dcplx(fftshift3D(vol_obj))) with the forward
+    FFT times Tp^3 (Simul_holo/main.cpp:106-109, src/FFT_fonctions.cpp:301-318)."""
+    ct, rt = _cplx(prec), _real(prec)
+    V = g.dim_final
+    t = fftshift3(np.asarray(dn).reshape(V, V, V).astype(ct))
+    F = np.fft.fftn(t).astype(ct)
+    F = (F * rt(tomo_pixel(g) ** 3)).astype(ct)
+    return fftshift3(F)
+
+
+def _cap_voxels(pos, a, g: Geom):
+    """Voxel index of every pupil pixel of angle a (the map of retroPropag_Born,
+    Reconstruction/src/fonctions.cpp:918-926, which is calcHolo's
+    fobj = round(fd - fi) in centred coordinates, Simul_holo/src/fonctions.cpp:
+    124-136 with Point3D::coordI, src/Point3D.cpp:55-58). None for a skipped
+    angle; -1 where the height leaves the volume."""
+    A = len(pos) // 2
+    nx, V, R = g.nxmax, g.dim_final, g.rayon
+    half = V // 2
+    fdx, fdy, pupil, h, _ = pupil_tables(g)
+    fxm0, fym0 = int(pos[a]) - nx, int(pos[a + A]) - nx
+    fz2 = R * R - fxm0 * fxm0 - fym0 * fym0
+    if round_half_away(fz2) < 0:
+        return None
+    koz = round_half_away(h[pupil] - math.sqrt(fz2)).astype(np.int64) + half
+    vox = (fdx[pupil] - fxm0 + half) + (fdy[pupil] - fym0 + half) * V + koz * V * V
+    return np.where((koz >= 0) & (koz < V), vox, -1)
+
+
+def born_forward(dn, pos, g: Geom, prec="f64"):
+    """Simul_holo's per-angle hologram spectrum (calcHolo,
+    Simul_holo/src/fonctions.cpp:109-140, loop main.cpp:162-171) in the frame
+    of field_to_spectrum: S[a][row][col], fd = (col - NXMAX, row - NXMAX)."""
+    ct = _cplx(prec)
+    A = len(pos) // 2
+    C = g.C
+    TF = contrast_spectrum(dn, g, prec).ravel()
+    w = born_weight(g)
+    _, _, pupil, _, _ = pupil_tables(g)
+    S = np.zeros((A, C, C), dtype=ct)
+    for a in range(A):
+        vox = _cap_voxels(pos, a, g)
+        if vox is None:
+            continue
+        vals = np.where(vox >= 0, TF[np.maximum(vox, 0)], 0)
+        S[a][pupil] = (w[pupil] * vals).astype(ct)
+    return S
+
+
+def born_adjoint(S, pos, g: Geom):
+    """Adjoint of born_forward in FP64, written as its explicit transpose:
+    scatter conj(w) S onto the cap voxels, then the adjoint of the centred
+    FFT (shift, unnormalised backward FFT, shift) times Tp^3."""
+    A = len(pos) // 2
+    V = g.dim_final
+    w = born_weight(g)
+    _, _, pupil, _, _ = pupil_tables(g)
+    vol = np.zeros(V ** 3, dtype=np.complex128)
+    for a in range(A):
+        vox = _cap_voxels(pos, a, g)
+        if vox is None:
+            continue
+        ok = vox >= 0
+        np.add.at(vol, vox[ok], (np.conj(w[pupil]) * np.asarray(S[a])[pupil])[ok])
+    t = fftshift3(vol.reshape(V, V, V))
+    t = np.fft.ifftn(t, norm="forward") * tomo_pixel(g) ** 3
+    return fftshift3(t)
+
+
 def reconstruct(fields, pos, g: Geom, prec="f64"):
     """tomo_reconstruction's compute (Reconstruction/main.cpp:203-352) from
     the preprocessing outputs. Returns (index, vol, counts, otf)."""
