@@ -197,6 +197,15 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                         const int32_t* posspec, int A, float* out_re,
                         float* out_im);
 
+/* Data weights of the adjoint-iterative form (no counterpart in the
+ * reference): weights[A][C][C] float = 1 / (q^2 * hits of the sample's
+ * voxel), q = kv / (sqrt(R^2 - |fd|^2)/R) the forward model's factor, on the
+ * cap samples of mtd_forward_spectra (0 elsewhere). With them A^H W A is
+ * Tp^6 times the projection of the centred 3-D spectrum on the hit voxels,
+ * the normalisation the plain method applies by dividing by the counts. */
+int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec,
+                    int A, float* weights);
+
 #ifdef __cplusplus
 }
 #endif

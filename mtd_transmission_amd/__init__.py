@@ -77,6 +77,7 @@ SIGNATURES = {
     "mtd_reconstruct_plain": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P, _P]),
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
+    "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
 }
 
 
@@ -304,3 +305,10 @@ class Context:
         self._check(lib().mtd_adjoint_spectra(self.h, ctypes.byref(g), _ptr(spectra),
                                               _ptr(posspec), A, _ptr(out_re),
                                               _ptr(out_im)), "adjoint_spectra")
+
+    def cap_weights(self, g: Geom, posspec, weights):
+        """weights[A][C][C] float = 1 / (q^2 * hits of the sample's voxel)."""
+        self._stream()
+        A = posspec.numel() // 2
+        self._check(lib().mtd_cap_weights(self.h, ctypes.byref(g), _ptr(posspec), A,
+                                          _ptr(weights)), "cap_weights")

@@ -189,6 +189,10 @@ size_t gather_scratch_bytes(int V, int A);
 hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
                                    float scale, const int2* band, const float2* tw,
                                    hipStream_t s);
+// Per-sample data weights 1 / (q^2 * hits of the voxel) from the hit counts.
+hipError_t launch_cap_weights(const int32_t* cnt, const AngleGeo* ageo, int A,
+                              const double* htab, const float* q, int r2max, int nx, int C,
+                              int V, float* w, hipStream_t s);
 // Per-angle cap samples -i q(|fd|^2) vol[voxel] into spec[A][C][C] (+posSpec frame).
 hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,
                              const float* q, int r2max, int nx, int C, int V, float2* spec,

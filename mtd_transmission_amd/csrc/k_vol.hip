@@ -645,6 +645,34 @@ k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
   spec[(size_t)a * C * C + i] = s;
 }
 
+// Data weights of the adjoint-iterative form: w = 1 / (q(|fd|^2)^2 * hits of
+// the sample's voxel) on the cap samples, 0 elsewhere. With them A^H W A is
+// Tp^6 times the centred 3-D DFT's projection on the hit voxels: the least
+// squares are then as well conditioned as the plain method's count
+// normalisation.
+__global__ void __launch_bounds__(256)
+k_cap_weights(const int32_t* __restrict__ cnt, const AngleGeo* __restrict__ ageo,
+              const double* __restrict__ htab, const float* __restrict__ q, int r2max, int nx,
+              int C, int V, float* __restrict__ w) {
+  const int a = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= C * C) return;
+  const int fdx = i % C - nx, fdy = i / C - nx;
+  const int r2 = fdx * fdx + fdy * fdy;
+  const AngleGeo ag = ageo[a];
+  float out = 0.f;
+  if (!ag.skip && r2 <= r2max) {
+    const int z = (int)round_away(htab[r2] - ag.fz) + V / 2;
+    if (z >= 0 && z < V) {
+      const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+      const int c = cnt[((size_t)z * V + Y) * V + X];  // >= 1: this sample hits it
+      const float qq = q[r2];
+      out = 1.f / (qq * qq * (float)c);
+    }
+  }
+  w[(size_t)a * C * C + i] = out;
+}
+
 #define MTD_VOL_SIZES(X) \
   X(32) X(64) X(128) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(704) X(768) X(1024)
 
@@ -681,6 +709,15 @@ hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, con
   Timed tm("cap_sample", s);
   hipLaunchKernelGGL(k_cap_sample, dim3((C * C + 255) / 256, A), dim3(256), 0, s, vol, ageo,
                      htab, q, r2max, nx, C, V, spec);
+  return hipGetLastError();
+}
+
+hipError_t launch_cap_weights(const int32_t* cnt, const AngleGeo* ageo, int A,
+                              const double* htab, const float* q, int r2max, int nx, int C,
+                              int V, float* w, hipStream_t s) {
+  Timed tm("cap_weights", s);
+  hipLaunchKernelGGL(k_cap_weights, dim3((C * C + 255) / 256, A), dim3(256), 0, s, cnt, ageo,
+                     htab, q, r2max, nx, C, V, w);
   return hipGetLastError();
 }
 

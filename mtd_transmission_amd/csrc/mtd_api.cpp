@@ -885,6 +885,29 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                        tp * tp * tp);
 }
 
+int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int A,
+                    float* weights) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (A == 0) return MTD_OK;
+  if (!posspec || !weights) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int V = g->dim_final, C = 2 * g->nxmax;
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  MTD_TRY(born_weights(ctx, g, &t));
+  // hit counts from the accumulation itself (their values need no spectra:
+  // the gather reads the stage buffer and the sums are discarded)
+  float2* spec = (float2*)ws_get(ctx, WS_F, (size_t)A * C * C * sizeof(float2));
+  float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
+  int32_t* cnt = (int32_t*)ws_get(ctx, WS_CNT, (size_t)V * V * V * sizeof(int32_t));
+  if (!spec || !vol || !cnt) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  MTD_TRY(scatter_impl(ctx, g, spec, 0, posspec, A, (float*)vol, cnt));
+  const AngleGeo* ag = (const AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)A * sizeof(AngleGeo));
+  return hip_check(ctx, launch_cap_weights(cnt, ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
+                                           weights, ctx->stream),
+                   "cap weights");
+}
+
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                           const uint8_t* mask, int32_t* posspec, float* index_re,
                           float* index_im) {

@@ -12,6 +12,13 @@ bins, so every specular lands on a pixel of the crop.
 Units: metres for lengths; pixel (x, y) of an N x N hologram sits at
 ((x - N/2) * pix, (y - N/2) * pix); the phantom is centred on the field of
 view at z = 0.
+
+Phase sign: the object wave carries exp(-i phi) with phi = k0 * sum dn * L
+the projected delay. That is the reference's convention: its simulator's
+Born field of a thin positive-contrast object is -i kv * integral(dn dz)
+(calcHolo's factor -i kv / sdz, Simul_holo/src/fonctions.cpp:115-122,136,
+the exact inverse of retroPropag_Born), so -i phi, and its Born formation
+U_B = |A| cos(phi_c) - 1 + i |A| sin(phi_c) then needs phi_c = -phi.
 """
 from __future__ import annotations
 
@@ -146,7 +153,7 @@ def hologram(n, nxmax, circle_cx, circle_cy, pix, lam, n0, kx, ky, phantom,
     """Noise-free intensity |R + U|^2 for one illumination."""
     y, x = np.mgrid[0:n, 0:n]
     phi = projected_phase(n, pix, lam, n0, kx, ky, phantom)
-    u = np.exp(1j * (2 * np.pi * (kx * x + ky * y) / n + phi))
+    u = np.exp(1j * (2 * np.pi * (kx * x + ky * y) / n - phi))
     if lowpass:
         u = pupil_lowpass(u, nxmax)
     r = reference_wave(n, circle_cx, circle_cy, kx, ky, ref_amp)
@@ -270,7 +277,7 @@ def simulate_torch(n, nxmax, circle_cx, circle_cy, pix, lam, n0, kx, ky, phantom
         for e, L in zip(phantom, _torch_chords(P, d, phantom, torch)):
             phi += float(kvac * e.dn) * L
         ramp = 2 * math.pi * (kxs.view(-1, 1, 1).float() * xx + kys.view(-1, 1, 1).float() * yy) / n
-        u = torch.polar(torch.ones_like(phi), torch.remainder(ramp, 2 * math.pi) + phi)
+        u = torch.polar(torch.ones_like(phi), torch.remainder(ramp, 2 * math.pi) - phi)
         if lowpass:
             u = torch.fft.ifft2(torch.fft.fft2(u) * pupil)
         psi = (math.pi * ((kxs - fcx) + (kys - fcy))).float().view(-1, 1, 1)

@@ -769,6 +769,30 @@ def born_adjoint(S, pos, g: Geom):
     return fftshift3(t)
 
 
+def cap_weights(pos, g: Geom):
+    """Data weights of the adjoint-iterative form (no reference counterpart):
+    1 / (q^2 * hits of the sample's voxel) per cap sample, with q the float32
+    magnitude of calcHolo's factor and the hits of retroPropag_Born's map;
+    0 outside the pupil and for skipped angles. FP32 arithmetic as the GPU."""
+    A = len(pos) // 2
+    C, V = g.C, g.dim_final
+    _, cnt = ewald_scatter(np.zeros((A, C, C), np.complex64), pos, g, "f32")
+    q = np.abs(born_weight(g)).astype(np.float32)
+    _, _, pupil, _, _ = pupil_tables(g)
+    w = np.zeros((A, C, C), np.float32)
+    for a in range(A):
+        vox = _cap_voxels(pos, a, g)
+        if vox is None:
+            continue
+        ok = vox >= 0
+        qq = q[pupil][ok]
+        c = cnt[vox[ok]].astype(np.float32)
+        wa = np.zeros(pupil.sum(), np.float32)
+        wa[ok] = np.float32(1) / (qq * qq * c)
+        w[a][pupil] = wa
+    return w
+
+
 def reconstruct(fields, pos, g: Geom, prec="f64"):
     """tomo_reconstruction's compute (Reconstruction/main.cpp:203-352) from
     the preprocessing outputs. Returns (index, vol, counts, otf)."""
