@@ -187,6 +187,15 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
 int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re,
                         const float* dn_im, const int32_t* posspec, int A,
                         float* spectra);
+/* (posspec == NULL: the geometry plan of mtd_born_plan, below.) */
+
+/* Geometry plan for iterative callers (no counterpart in the reference):
+ * copies posspec[2A] and builds the per-angle cap geometry, the hit band and
+ * the accumulation's per-tile angle lists once. mtd_forward_spectra,
+ * mtd_adjoint_spectra and mtd_cap_weights called with posspec == NULL then
+ * use it (same geometry and A, else MTD_ERR_ARG). One plan per context; the
+ * next call replaces it. */
+int mtd_born_plan(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int A);
 
 /* Adjoint of mtd_forward_spectra (no counterpart in the reference; the
  * gradient of the adjoint-iterative reconstruction): out = A^H spectra, the
@@ -205,6 +214,16 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
  * the normalisation the plain method applies by dividing by the counts. */
 int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec,
                     int A, float* weights);
+
+/* One fused step of the adjoint-iterative solver (no counterpart in the
+ * reference): float V^3 volumes, x' = max(z - step*(data_scale*grad_data +
+ * lam*grad TV_eps(z)), 0) (no max when positivity == 0), z_next = x' +
+ * beta*(x' - x), x <- x'. TV_eps is the isotropic total variation with
+ * forward differences (zero across the far faces), smoothed by eps. z and
+ * z_next must be distinct buffers. */
+int mtd_tv_fista_step(mtd_ctx* ctx, int V, const float* z, const float* grad_data,
+                      float* x, float* z_next, float data_scale, float lam,
+                      float eps, float step, float beta, int positivity);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,9 @@ SIGNATURES = {
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
+    "mtd_born_plan": (_I, [_P, _GP, _P, _I]),
+    "mtd_tv_fista_step": (_I, [_P, _I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_float, _I]),
 }
 
 
@@ -290,25 +293,48 @@ class Context:
                                                 _ptr(index_im)), "reconstruct_plain")
 
     # ---- Born forward model and adjoint ---------------------------------
-    def forward_spectra(self, g: Geom, dn_re, dn_im, posspec, spectra):
-        """spectra[A][C][C] c32 = A(dn) (Simul_holo's calcHolo over all angles)."""
+    def born_plan(self, g: Geom, posspec):
+        """Geometry plan: later forward/adjoint/weights calls with
+        posspec=None reuse its angle geometry, band and tile lists."""
         self._stream()
-        A = posspec.numel() // 2
+        self._check(lib().mtd_born_plan(self.h, ctypes.byref(g), _ptr(posspec),
+                                        posspec.numel() // 2), "born_plan")
+
+    @staticmethod
+    def _angles(posspec, arr):
+        return posspec.numel() // 2 if posspec is not None else arr.shape[0]
+
+    def forward_spectra(self, g: Geom, dn_re, dn_im, posspec, spectra):
+        """spectra[A][C][C] c32 = A(dn) (Simul_holo's calcHolo over all angles);
+        posspec=None: the geometry plan."""
+        self._stream()
+        A = self._angles(posspec, spectra)
         self._check(lib().mtd_forward_spectra(self.h, ctypes.byref(g), _ptr(dn_re),
                                               _ptr(dn_im), _ptr(posspec), A,
                                               _ptr(spectra)), "forward_spectra")
 
     def adjoint_spectra(self, g: Geom, spectra, posspec, out_re, out_im):
-        """(out_re, out_im) = A^H spectra."""
+        """(out_re, out_im) = A^H spectra; posspec=None: the geometry plan."""
         self._stream()
-        A = posspec.numel() // 2
+        A = self._angles(posspec, spectra)
         self._check(lib().mtd_adjoint_spectra(self.h, ctypes.byref(g), _ptr(spectra),
                                               _ptr(posspec), A, _ptr(out_re),
                                               _ptr(out_im)), "adjoint_spectra")
 
     def cap_weights(self, g: Geom, posspec, weights):
-        """weights[A][C][C] float = 1 / (q^2 * hits of the sample's voxel)."""
+        """weights[A][C][C] float = 1 / (q^2 * hits of the sample's voxel);
+        posspec=None: the geometry plan."""
         self._stream()
-        A = posspec.numel() // 2
+        A = self._angles(posspec, weights)
         self._check(lib().mtd_cap_weights(self.h, ctypes.byref(g), _ptr(posspec), A,
                                           _ptr(weights)), "cap_weights")
+
+    def tv_fista_step(self, z, grad_data, x, z_next, data_scale, lam, eps, step, beta,
+                      positivity=True):
+        """x' = P(z - step (data_scale grad_data + lam grad TV_eps(z))),
+        z_next = x' + beta (x' - x), x <- x' (float V^3 volumes)."""
+        self._stream()
+        V = z.shape[-1]
+        self._check(lib().mtd_tv_fista_step(self.h, V, _ptr(z), _ptr(grad_data), _ptr(x),
+                                            _ptr(z_next), data_scale, lam, eps, step, beta,
+                                            int(bool(positivity))), "tv_fista_step")

@@ -38,28 +38,27 @@ class BornSpectra(torch.autograd.Function):
     conjugate-Wirtinger gradient, 2 (S - y) for ||S - y||^2)."""
 
     @staticmethod
-    def forward(ctx, dn, mtd, g, posspec):
-        A = posspec.numel() // 2
+    def forward(ctx, dn, mtd, g, posspec, A=None):
+        # posspec None: the context's geometry plan (Context.born_plan) of A angles
+        A = posspec.numel() // 2 if posspec is not None else A
         C = g.C
         S = torch.empty((A, C, C), dtype=torch.complex64, device=dn.device)
         mtd.forward_spectra(g, dn.detach().contiguous(), None, posspec, S)
-        ctx.mtd, ctx.g = mtd, g
-        ctx.save_for_backward(posspec)
+        ctx.mtd, ctx.g, ctx.posspec = mtd, g, posspec
         return S
 
     @staticmethod
     def backward(ctx, gS):
-        (posspec,) = ctx.saved_tensors
         V = ctx.g.dim_final
         re = torch.empty((V, V, V), dtype=torch.float32, device=gS.device)
         im = torch.empty_like(re)
-        ctx.mtd.adjoint_spectra(ctx.g, gS.contiguous(), posspec, re, im)
-        return re, None, None, None
+        ctx.mtd.adjoint_spectra(ctx.g, gS.contiguous(), ctx.posspec, re, im)
+        return re, None, None, None, None
 
 
-def born_spectra(dn, mtd, g, posspec):
-    """Differentiable A(dn)."""
-    return BornSpectra.apply(dn, mtd, g, posspec)
+def born_spectra(dn, mtd, g, posspec, A=None):
+    """Differentiable A(dn); posspec None: the geometry plan of A angles."""
+    return BornSpectra.apply(dn, mtd, g, posspec, A)
 
 
 def tv(dn, eps):
@@ -85,13 +84,16 @@ def normal_norm(mtd, g, posspec, weights=None, iters=20, seed=0):
     are taken in float64: float32 squares of its outputs underflow. With the
     cap weights, kappa = Tp^6 V^3 exactly (the centred DFT's norm squared)."""
     V = g.dim_final
+    A = posspec.numel() // 2
+    mtd.born_plan(g, posspec)
+    posspec = None
     gen = torch.Generator(device="cuda").manual_seed(seed)
     v = torch.randn((V, V, V), device="cuda", generator=gen)
     v /= v.norm()
     lam = 0.0
     for _ in range(iters):
         v = v.requires_grad_(True)
-        S = born_spectra(v, mtd, g, posspec)
+        S = born_spectra(v, mtd, g, posspec, A)
         (w,) = torch.autograd.grad(data_term(S, 0, weights), v)
         w = w.double()
         lam = float(w.norm())
@@ -109,7 +111,7 @@ class TVResult:
 
 
 def reconstruct_tv(mtd, g, posspec, y, n_iter=100, lam=1e-3, eps=1e-3, dn0=None,
-                   positivity=True, kappa=None, weights=None, record=False):
+                   positivity=True, kappa=None, weights=None, record=False, monitor=None):
     """FISTA on 1/2 ||A dn - y||_W^2 / kappa + lam TV_eps(dn), dn >= 0.
 
     y: complex64 [A][C][C] measured spectra (field_to_spectrum frame).
@@ -117,32 +119,47 @@ def reconstruct_tv(mtd, g, posspec, y, n_iter=100, lam=1e-3, eps=1e-3, dn0=None,
     dn0: start volume (e.g. the plain reconstruction's real part), zeros
     otherwise. The step is 1/L with L = 1 + 12 lam / eps (the data term's
     Hessian has norm 1 after the kappa scaling; 12 bounds ||grad||^2 in
-    3-D). Returns a TVResult."""
+    3-D). Each iteration takes the data gradient by autograd through
+    BornSpectra (GPU forward + adjoint, the output gradient W (S - y) formed
+    directly), then one fused library step (TV gradient, gradient step,
+    projection, momentum). record: the objective at
+    each iterate z. monitor(k, x): called after iteration k (1-based) with
+    the current estimate. Returns a TVResult."""
     V = g.dim_final
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if kappa is None:
         kappa = normal_norm(mtd, g, posspec, weights)
+    # one geometry plan for all the forward / adjoint calls of the loop
+    A = posspec.numel() // 2
+    mtd.born_plan(g, posspec)
     step = 1.0 / (1.0 + 12.0 * lam / eps)
     x = torch.zeros((V, V, V), device="cuda") if dn0 is None else dn0.detach().clone()
     if positivity:
         x.clamp_(min=0.0)
     z = x.clone()
+    zn = torch.empty_like(x)
     t = 1.0
     losses = []
-    for _ in range(n_iter):
-        z = z.requires_grad_(True)
-        S = born_spectra(z, mtd, g, posspec)
-        loss = data_term(S, y, weights) / kappa + lam * tv(z, eps)
-        (grad,) = torch.autograd.grad(loss, z)
-        if record:
-            losses.append(float(loss.detach()))
+    for k in range(1, n_iter + 1):
+        zg = z.detach().requires_grad_(True)
+        S = born_spectra(zg, mtd, g, None, A)
+        # d/dS* of 1/2 sum W |S - y|^2 is W (S - y): handed to autograd as the
+        # output gradient, so the backward is BornSpectra's adjoint alone
         with torch.no_grad():
-            xn = z - step * grad
-            if positivity:
-                xn.clamp_(min=0.0)
-            tn = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
-            z = xn + ((t - 1.0) / tn) * (xn - x)
-            x, t = xn, tn
+            gS = S - y
+            if weights is not None:
+                gS *= weights
+            if record:
+                ld = data_term(S, y, weights)
+                losses.append(float(ld) / kappa + lam * float(tv(z, eps)))
+        (gd,) = torch.autograd.grad(S, zg, grad_outputs=gS)
+        tn = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
+        mtd.tv_fista_step(z, gd, x, zn, 1.0 / kappa, lam, eps, step, (t - 1.0) / tn,
+                          positivity)
+        z, zn = zn, z
+        t = tn
+        if monitor is not None:
+            monitor(k, x)
     torch.cuda.synchronize()
-    return TVResult(x.detach(), losses, kappa, n_iter, time.perf_counter() - t0)
+    return TVResult(x, losses, kappa, n_iter, time.perf_counter() - t0)

@@ -64,6 +64,11 @@ struct mtd_ctx {
   std::map<std::tuple<uint64_t, int, uint64_t, uint64_t>, float*> qtab;  // (R, r2max, lambda, n0)
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   mtd::LsqPlan lsq;
+  // geometry plan of the forward model / adjoint (mtd_born_plan): speculars,
+  // angle geometry, hit band and the gather's tile lists, in WS_PLAN* slots
+  bool plan_valid = false;
+  int plan_A = 0;
+  mtd_geom plan_g{};
   // per-kernel event timing (mtd_ctx_set_timing)
   bool timing = false;
   std::vector<mtd::EvPair> pending;
@@ -102,7 +107,8 @@ struct Timed {
 // scratch slots
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
-  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART
+  WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART,
+  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -181,7 +187,10 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, hipStream_t s);
+                         int zlo, int zhi, bool prepass, hipStream_t s);
+// the gather's prepass alone (per-tile angle lists into scratch)
+hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
+                                 const double* htab, int r2max, void* scratch, hipStream_t s);
 size_t gather_scratch_bytes(int V, int A);
 // Born forward model: work = centred 3-D spectrum of the contrast (re, im;
 // im may be nullptr) times scale (Tp^3 * gsign), stored only on the heights
@@ -189,6 +198,10 @@ size_t gather_scratch_bytes(int V, int A);
 hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
                                    float scale, const int2* band, const float2* tw,
                                    hipStream_t s);
+// One fused FISTA step with the smoothed TV gradient (k_tv.hip).
+hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
+                           float sd, float lam, float eps, float step, float beta,
+                           int positivity, hipStream_t s);
 // Per-sample data weights 1 / (q^2 * hits of the voxel) from the hit counts.
 hipError_t launch_cap_weights(const int32_t* cnt, const AngleGeo* ageo, int A,
                               const double* htab, const float* q, int r2max, int nx, int C,

@@ -1,0 +1,85 @@
+// One step of the adjoint-iterative form's solver, fused into a single pass
+// over the volume: the gradient of the smoothed isotropic total variation
+//   TV(z) = sum_i sqrt(dx_i^2 + dy_i^2 + dz_i^2 + eps^2) - eps,
+//   d*_i = z[i + e*] - z[i] (0 at the far face of each axis),
+// the gradient step with the data gradient, the projection on z >= 0 and the
+// FISTA momentum:
+//   x' = P(z - step * (sd * gd + lam * gTV(z))),  z' = x' + beta * (x' - x).
+// dTV/dz_i collects the terms of voxel i and of its three lower neighbours:
+//   -(dx_i + dy_i + dz_i)/s_i + dx_{i-ex}/s_{i-ex} + dy_{i-ey}/s_{i-ey}
+//   + dz_{i-ez}/s_{i-ez},  s_j = sqrt(dx_j^2 + dy_j^2 + dz_j^2 + eps^2).
+// z and z' must not overlap (neighbours of z are read); x is updated in place.
+#include "internal.hpp"
+
+namespace mtd {
+
+namespace {
+
+struct Vol {
+  const float* z;
+  int V;
+  __device__ __forceinline__ float at(int x, int y, int k) const {
+    return z[((size_t)k * V + y) * V + x];
+  }
+  // forward differences of voxel (x, y, k) and the reciprocal of their
+  // smoothed norm (one hardware reciprocal square root, about 1 ulp)
+  __device__ __forceinline__ void diffs(int x, int y, int k, float c, float eps2, float& dx,
+                                        float& dy, float& dz, float& rs) const {
+    dx = x + 1 < V ? at(x + 1, y, k) - c : 0.f;
+    dy = y + 1 < V ? at(x, y + 1, k) - c : 0.f;
+    dz = k + 1 < V ? at(x, y, k + 1) - c : 0.f;
+    rs = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz + eps2);
+  }
+};
+
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __restrict__ x,
+           float* __restrict__ zn, int V, float sd, float lam, float eps, float step,
+           float beta, int positivity) {
+  // grid (ceil(V/256), V, V): x from the thread, y and z from the block
+  const int xi = blockIdx.x * 256 + threadIdx.x, yi = blockIdx.y, ki = blockIdx.z;
+  if (xi >= V) return;
+  const size_t i = ((size_t)ki * V + yi) * V + xi;
+  const Vol vz{z, V};
+  const float eps2 = eps * eps;
+  const float c = z[i];
+  float dx, dy, dz, rs;
+  vz.diffs(xi, yi, ki, c, eps2, dx, dy, dz, rs);
+  float gtv = -(dx + dy + dz) * rs;
+  if (xi > 0) {
+    const float cl = vz.at(xi - 1, yi, ki);
+    float ax, ay, az, ar;
+    vz.diffs(xi - 1, yi, ki, cl, eps2, ax, ay, az, ar);
+    gtv += ax * ar;
+  }
+  if (yi > 0) {
+    const float cl = vz.at(xi, yi - 1, ki);
+    float ax, ay, az, ar;
+    vz.diffs(xi, yi - 1, ki, cl, eps2, ax, ay, az, ar);
+    gtv += ay * ar;
+  }
+  if (ki > 0) {
+    const float cl = vz.at(xi, yi, ki - 1);
+    float ax, ay, az, ar;
+    vz.diffs(xi, yi, ki - 1, cl, eps2, ax, ay, az, ar);
+    gtv += az * ar;
+  }
+  float xn = c - step * (sd * gd[i] + lam * gtv);
+  if (positivity) xn = fmaxf(xn, 0.f);
+  const float xo = x[i];
+  x[i] = xn;
+  zn[i] = xn + beta * (xn - xo);
+}
+
+hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
+                           float sd, float lam, float eps, float step, float beta,
+                           int positivity, hipStream_t s) {
+  Timed tm("tv_fista", s);
+  hipLaunchKernelGGL(k_tv_fista, dim3((V + 255) / 256, V, V), dim3(256), 0, s, z, gd, x, zn, V,
+                     sd, lam, eps, step, beta, positivity);
+  return hipGetLastError();
+}
+
+}  // namespace mtd

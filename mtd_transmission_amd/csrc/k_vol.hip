@@ -394,7 +394,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, hipStream_t s) {
+                         int zlo, int zhi, bool prepass, hipStream_t s) {
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
@@ -405,7 +405,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   TileEntry* list = (TileEntry*)scratch;
   uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
   int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
-  {
+  if (prepass) {
     Timed tm("ewald_tiles", s);
     hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
   }
@@ -427,6 +427,19 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                        nwin, items, (const TileEntry*)list, (const int4*)tinfo,
                        (const uint8_t*)zoff, flags, band);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
+                                 const double* htab, int r2max, void* scratch, hipStream_t s) {
+  GatherArgs g{nullptr, 0, ageo, A, C, nxmax, V, htab, nullptr, r2max, nullptr, nullptr};
+  const int ntx = V / GT, tiles = ntx * ntx;
+  const size_t nA = (size_t)(A > 0 ? A : 1);
+  TileEntry* list = (TileEntry*)scratch;
+  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
+  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
+  Timed tm("ewald_tiles", s);
+  hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
   return hipGetLastError();
 }
 

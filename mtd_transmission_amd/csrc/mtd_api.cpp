@@ -735,22 +735,54 @@ static int born_weights(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
 // band != nullptr: the hit band is computed into it and only the heights that
 // meet it are written. weights: per-r2 factors in place of beta (the
 // adjoint of the Born forward model), or nullptr.
+// The geometry of one set of speculars, computed per call or taken from the
+// context's plan (mtd_born_plan): posspec, angle table, hit band, tile lists.
+struct GeoBufs {
+  const int32_t* pos = nullptr;
+  const AngleGeo* ag = nullptr;
+  int2* band = nullptr;
+  void* scratch = nullptr;  // the gather's tile lists, built
+};
+
+static int plan_bufs(mtd_ctx* ctx, const mtd_geom* g, int A, GeoBufs* b) {
+  if (!ctx->plan_valid || A != ctx->plan_A || std::memcmp(g, &ctx->plan_g, sizeof(mtd_geom)))
+    return set_err(ctx, MTD_ERR_ARG,
+                   "posspec NULL needs a geometry plan (mtd_born_plan) of the same geometry "
+                   "and angle count");
+  b->pos = (const int32_t*)ctx->ws[WS_PLANPOS].ptr;
+  b->ag = (const AngleGeo*)ctx->ws[WS_PLANAGEO].ptr;
+  b->band = (int2*)ctx->ws[WS_PLANBAND].ptr;
+  b->scratch = ctx->ws[WS_PLANGATHER].ptr;
+  return MTD_OK;
+}
+
 static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int frame_pos,
                         const int32_t* posspec, int A, float* vol, int32_t* counts,
-                        int flags = 0, int2* band = nullptr, const float* weights = nullptr) {
+                        int flags = 0, int2* band = nullptr, const float* weights = nullptr,
+                        const GeoBufs* plan = nullptr) {
   // per-voxel hit counts are accumulated in 16 bits (one hit per angle)
   if (A > 65535) return set_err(ctx, MTD_ERR_UNSUPPORTED, "more than 65535 angles per volume");
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
-  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
-  void* scratch = ws_get(ctx, WS_GATHER, gather_scratch_bytes(g->dim_final, A));
-  if (!ag || !scratch) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
-  if (A > 0)
-    MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
-                                            t.r2tab, ag, ctx->stream), "angle geometry"));
-  if (band)
-    MTD_TRY(hip_check(ctx, launch_band(ag, A, g->dim_final, t.htab, t.r2max, band, ctx->stream),
-                      "hit band"));
+  const AngleGeo* ag;
+  void* scratch;
+  if (plan) {  // geometry, band and tile lists already built
+    ag = plan->ag;
+    scratch = plan->scratch;
+    posspec = plan->pos;
+    if (band && band != plan->band) return set_err(ctx, MTD_ERR_ARG, "band of another plan");
+  } else {
+    AngleGeo* agw = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
+    scratch = ws_get(ctx, WS_GATHER, gather_scratch_bytes(g->dim_final, A));
+    if (!agw || !scratch) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
+    if (A > 0)
+      MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
+                                              t.r2tab, agw, ctx->stream), "angle geometry"));
+    if (band)
+      MTD_TRY(hip_check(ctx, launch_band(agw, A, g->dim_final, t.htab, t.r2max, band,
+                                         ctx->stream), "hit band"));
+    ag = agw;
+  }
   // Host-side bound of the hit band: heights h(r2) - fz with h(r2) in
   // [h(r2max), R] and fz = h(|fm|^2) in [0, R] for every non-skipped angle.
   const int V = g->dim_final;
@@ -760,7 +792,7 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
                                        2 * g->nxmax, g->nxmax, V, t.htab,
                                        weights ? weights : t.beta, t.r2max, (float2*)vol, counts,
                                        scratch, flags, band,
-                                       zlo, zhi, ctx->stream),
+                                       zlo, zhi, plan == nullptr, ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
 }
@@ -834,31 +866,83 @@ int mtd_finalize(mtd_ctx* ctx, const mtd_geom* g, const float* vol, const int32_
 // voxel size of the reconstruction, 1/(V delta_f): Simul_holo's Tp_Tomo
 static double tomo_pixel(const mtd_geom* g) { return 1.0 / (g->dim_final * g->delta_f); }
 
+// Geometry of a forward / adjoint / weights call: the context's plan when
+// posspec is NULL, else computed now into the per-call slots (band and angle
+// table; the tile lists are built by the gather itself).
+static int call_geometry(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int A,
+                         const EwaldTables& t, bool with_band, GeoBufs* b, bool* planned) {
+  *planned = posspec == nullptr;
+  if (*planned) return plan_bufs(ctx, g, A, b);
+  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)std::max(A, 1) * sizeof(AngleGeo));
+  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+  if (!ag || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory (angle table)");
+  MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
+                                          t.r2tab, ag, ctx->stream), "angle geometry"));
+  if (with_band)
+    MTD_TRY(hip_check(ctx, launch_band(ag, A, g->dim_final, t.htab, t.r2max, band, ctx->stream),
+                      "hit band"));
+  b->pos = posspec;
+  b->ag = ag;
+  b->band = band;
+  b->scratch = nullptr;
+  return MTD_OK;
+}
+
+int mtd_born_plan(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int A) {
+  CtxScope scope(ctx);
+  ctx->plan_valid = false;
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (A <= 0) return set_err(ctx, MTD_ERR_ARG, "a plan needs at least one angle");
+  if (A > 65535) return set_err(ctx, MTD_ERR_UNSUPPORTED, "more than 65535 angles per volume");
+  if (!posspec) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int V = g->dim_final;
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  MTD_TRY(born_weights(ctx, g, &t));
+  int32_t* pos = (int32_t*)ws_get(ctx, WS_PLANPOS, (size_t)2 * A * sizeof(int32_t));
+  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_PLANAGEO, (size_t)A * sizeof(AngleGeo));
+  int2* band = (int2*)ws_get(ctx, WS_PLANBAND, sizeof(int2));
+  void* scratch = ws_get(ctx, WS_PLANGATHER, gather_scratch_bytes(V, A));
+  if (!pos || !ag || !band || !scratch)
+    return set_err(ctx, MTD_ERR_HIP, "out of device memory (geometry plan)");
+  MTD_TRY(hip_check(ctx, hipMemcpyAsync(pos, posspec, (size_t)2 * A * sizeof(int32_t),
+                                        hipMemcpyDeviceToDevice, ctx->stream),
+                    "plan speculars"));
+  MTD_TRY(hip_check(ctx, launch_angle_geo(pos, pos + A, A, g->nxmax, t.R2, t.htab, t.r2tab, ag,
+                                          ctx->stream), "angle geometry"));
+  MTD_TRY(hip_check(ctx, launch_band(ag, A, V, t.htab, t.r2max, band, ctx->stream), "hit band"));
+  MTD_TRY(hip_check(ctx, launch_gather_prepass(ag, A, 2 * g->nxmax, g->nxmax, V, t.htab,
+                                               t.r2max, scratch, ctx->stream),
+                    "gather prepass"));
+  ctx->plan_g = *g;
+  ctx->plan_A = A;
+  ctx->plan_valid = true;
+  return MTD_OK;
+}
+
 int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re, const float* dn_im,
                         const int32_t* posspec, int A, float* spectra) {
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
   if (A == 0) return MTD_OK;
-  if (!dn_re || !posspec || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (!dn_re || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int V = g->dim_final, C = 2 * g->nxmax;
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   MTD_TRY(born_weights(ctx, g, &t));
-  AngleGeo* ag = (AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)A * sizeof(AngleGeo));
-  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+  GeoBufs geo;
+  bool planned;
+  MTD_TRY(call_geometry(ctx, g, posspec, A, t, true, &geo, &planned));
   float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
   const float2* twV = twiddle(ctx, V);
-  if (!ag || !band || !work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
-  MTD_TRY(hip_check(ctx, launch_angle_geo(posspec, posspec + A, A, g->nxmax, t.R2, t.htab,
-                                          t.r2tab, ag, ctx->stream), "angle geometry"));
-  MTD_TRY(hip_check(ctx, launch_band(ag, A, V, t.htab, t.r2max, band, ctx->stream), "hit band"));
+  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const double tp = tomo_pixel(g);
   const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
   MTD_TRY(hip_check(ctx, launch_contrast_spectrum(V, dn_re, dn_im, work,
-                                                  (float)(tp * tp * tp) * gsign, band, twV,
+                                                  (float)(tp * tp * tp) * gsign, geo.band, twV,
                                                   ctx->stream),
                     "contrast spectrum"));
-  return hip_check(ctx, launch_cap_sample(work, ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
+  return hip_check(ctx, launch_cap_sample(work, geo.ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
                                           (float2*)spectra, ctx->stream),
                    "cap sampling");
 }
@@ -867,19 +951,30 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                         const int32_t* posspec, int A, float* out_re, float* out_im) {
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
-  if (!out_re || !out_im || (A > 0 && (!spectra || !posspec)))
-    return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (!out_re || !out_im || (A > 0 && !spectra)) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (A > 0 && !posspec && !ctx->plan_valid)
+    return set_err(ctx, MTD_ERR_ARG, "null array (posspec, and no geometry plan)");
   const int V = g->dim_final;
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   MTD_TRY(born_weights(ctx, g, &t));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
-  int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
-  if (!vol || !band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!vol) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   // sum over angles of i q S on each cap voxel (conj of the forward weight),
   // centring-signed, then the unnormalised backward 3-D FFT times Tp^3
-  MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, (float*)vol, nullptr, 4,
-                       band, t.q));
+  int2* band;
+  if (A > 0 && !posspec) {
+    GeoBufs geo;
+    MTD_TRY(plan_bufs(ctx, g, A, &geo));
+    band = geo.band;
+    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, nullptr, A, (float*)vol, nullptr, 4,
+                         band, t.q, &geo));
+  } else {
+    band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
+    if (!band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, (float*)vol, nullptr, 4,
+                         band, t.q));
+  }
   const double tp = tomo_pixel(g);
   return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, out_re, out_im, nullptr,
                        tp * tp * tp);
@@ -890,22 +985,41 @@ int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
   if (A == 0) return MTD_OK;
-  if (!posspec || !weights) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (!weights) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int V = g->dim_final, C = 2 * g->nxmax;
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
   MTD_TRY(born_weights(ctx, g, &t));
+  GeoBufs geo;
+  bool planned = posspec == nullptr;
+  if (planned) MTD_TRY(plan_bufs(ctx, g, A, &geo));
   // hit counts from the accumulation itself (their values need no spectra:
   // the gather reads the stage buffer and the sums are discarded)
   float2* spec = (float2*)ws_get(ctx, WS_F, (size_t)A * C * C * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
   int32_t* cnt = (int32_t*)ws_get(ctx, WS_CNT, (size_t)V * V * V * sizeof(int32_t));
   if (!spec || !vol || !cnt) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
-  MTD_TRY(scatter_impl(ctx, g, spec, 0, posspec, A, (float*)vol, cnt));
-  const AngleGeo* ag = (const AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)A * sizeof(AngleGeo));
+  MTD_TRY(scatter_impl(ctx, g, spec, 0, posspec, A, (float*)vol, cnt, 0, nullptr, nullptr,
+                       planned ? &geo : nullptr));
+  const AngleGeo* ag =
+      planned ? geo.ag : (const AngleGeo*)ws_get(ctx, WS_AGEO, (size_t)A * sizeof(AngleGeo));
   return hip_check(ctx, launch_cap_weights(cnt, ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
                                            weights, ctx->stream),
                    "cap weights");
+}
+
+int mtd_tv_fista_step(mtd_ctx* ctx, int V, const float* z, const float* grad_data, float* x,
+                      float* z_next, float data_scale, float lam, float eps, float step,
+                      float beta, int positivity) {
+  CtxScope scope(ctx);
+  if (!ctx) return MTD_ERR_ARG;
+  if (V <= 0 || V > 2048) return set_err(ctx, MTD_ERR_ARG, "volume side out of range");
+  if (!z || !grad_data || !x || !z_next) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (z == z_next) return set_err(ctx, MTD_ERR_ARG, "z and z_next must be distinct buffers");
+  if (!(eps > 0.f) || !(step > 0.f)) return set_err(ctx, MTD_ERR_ARG, "eps and step must be > 0");
+  return hip_check(ctx, launch_tv_fista(V, z, grad_data, x, z_next, data_scale, lam, eps, step,
+                                        beta, positivity, ctx->stream),
+                   "TV step");
 }
 
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,

@@ -309,3 +309,24 @@ def simulate_torch(n, nxmax, circle_cx, circle_cy, pix, lam, n0, kx, ky, phantom
     grown[:2, :] = grown[-2:, :] = True
     grown[:, :2] = grown[:, -2:] = True
     return out, np.where(grown, 0, 255).astype(np.uint8)
+
+
+def voxelize_torch(phantom, V, voxel, device="cuda"):
+    """Index contrast of the phantom on the reconstruction grid (V^3 float32
+    torch tensor, [z][y][x]): voxel (x, y, z) sits at ((x - V/2) voxel,
+    (y - V/2) voxel, (z - V/2) voxel), the frame of the reconstructed index
+    volume (its lateral centre is the hologram centre, its height centre the
+    z = 0 plane the fields are recorded in). Overlapping ellipsoids add, as
+    in the projected phase. Sampled at voxel centres."""
+    import torch
+    ar = (torch.arange(V, device=device, dtype=torch.float64) - V // 2) * voxel
+    out = torch.zeros((V, V, V), device=device, dtype=torch.float32)
+    for z0 in range(0, V, 32):  # slabs bound the temporary memory
+        z1 = min(V, z0 + 32)
+        Z, Y, X = torch.meshgrid(ar[z0:z1], ar, ar, indexing="ij")
+        P = torch.stack([X, Y, Z], -1)
+        for e in phantom:
+            M = torch.tensor(e.quad, device=device)
+            q = P - torch.tensor(e.centre, device=device)
+            out[z0:z1] += (((q @ M) * q).sum(-1) <= 1.0).float() * float(e.dn)
+    return out

@@ -180,3 +180,36 @@ def test_adjoint_of_no_angles_is_zero(gpu_ctx):
     gpu_ctx.adjoint_spectra(gl, _empty((0, 32, 32), torch.complex64), pos, re, im)
     torch.cuda.synchronize()
     assert not re.any() and not im.any()
+
+
+def test_geometry_plan_reuse(gpu_ctx):
+    """Forward, adjoint and weights through a geometry plan (posspec None)
+    are bit-identical to the per-call geometry; a plan of another angle
+    count is refused."""
+    import torch
+    import mtd_transmission_amd as m
+    nx, V, A = 16, 64, 24
+    g, gl = _geoms(nx, V)
+    pos = _t(_ring_pos(nx, A).astype(np.int32))
+    dn, _ = _phantom(V, 8)
+    tdn = _t(dn)
+    C = g.C
+    S0 = _empty((A, C, C), torch.complex64)
+    gpu_ctx.forward_spectra(gl, tdn, None, pos, S0)
+    r0, i0 = _empty((V, V, V), torch.float32), _empty((V, V, V), torch.float32)
+    gpu_ctx.adjoint_spectra(gl, S0, pos, r0, i0)
+    w0 = _empty((A, C, C), torch.float32)
+    gpu_ctx.cap_weights(gl, pos, w0)
+    gpu_ctx.born_plan(gl, pos)
+    S1 = _empty((A, C, C), torch.complex64)
+    gpu_ctx.forward_spectra(gl, tdn, None, None, S1)
+    r1, i1 = _empty((V, V, V), torch.float32), _empty((V, V, V), torch.float32)
+    gpu_ctx.adjoint_spectra(gl, S0, None, r1, i1)
+    w1 = _empty((A, C, C), torch.float32)
+    gpu_ctx.cap_weights(gl, None, w1)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.view_as_real(S0), torch.view_as_real(S1))
+    assert torch.equal(r0, r1) and torch.equal(i0, i1)
+    assert torch.equal(w0, w1)
+    with pytest.raises(m.MTDError):
+        gpu_ctx.forward_spectra(gl, tdn, None, None, _empty((A - 1, C, C), torch.complex64))
