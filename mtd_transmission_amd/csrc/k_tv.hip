@@ -38,8 +38,17 @@ __global__ void __launch_bounds__(256)
 k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __restrict__ x,
            float* __restrict__ zn, int V, float sd, float lam, float eps, float step,
            float beta, int positivity) {
-  // grid (ceil(V/256), V, V): x from the thread, y and z from the block
-  const int xi = blockIdx.x * 256 + threadIdx.x, yi = blockIdx.y, ki = blockIdx.z;
+  // 1-D grid of V*V rows x cx = ceil(V/256) segments. Blocks are dealt
+  // round-robin to the 8 XCDs; renumbered so that XCD k sweeps the k-th
+  // eighth of the rows in order, the rows y +- 1 and planes k +- 1 the
+  // stencil reads were just read by the same XCD (its L2).
+  const int cx = (V + 255) / 256;
+  const unsigned nb = gridDim.x;
+  unsigned r = blockIdx.x;
+  if (nb % 8 == 0) r = (r % 8) * (nb / 8) + r / 8;
+  const int seg = (int)(r % cx);
+  const unsigned row = r / cx;
+  const int xi = seg * 256 + threadIdx.x, yi = (int)(row % V), ki = (int)(row / V);
   if (xi >= V) return;
   const size_t i = ((size_t)ki * V + yi) * V + xi;
   const Vol vz{z, V};
@@ -77,8 +86,9 @@ hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, flo
                            float sd, float lam, float eps, float step, float beta,
                            int positivity, hipStream_t s) {
   Timed tm("tv_fista", s);
-  hipLaunchKernelGGL(k_tv_fista, dim3((V + 255) / 256, V, V), dim3(256), 0, s, z, gd, x, zn, V,
-                     sd, lam, eps, step, beta, positivity);
+  const unsigned nb = (unsigned)((V + 255) / 256) * (unsigned)V * (unsigned)V;
+  hipLaunchKernelGGL(k_tv_fista, dim3(nb), dim3(256), 0, s, z, gd, x, zn, V, sd, lam, eps, step,
+                     beta, positivity);
   return hipGetLastError();
 }
 

@@ -1,12 +1,14 @@
 // tomo_reconstruction: drop-in for Tomo_prog/Reconstruction (installed as
 // tomo_reconstruction, INSTALL.sh:24). Same argv and configuration files;
 // reads parametres.raw, tab_posSpec.raw and UBornfinal_{Re,Im}CxCxA.raw from
-// CHEMIN_RESULT and writes indice.tif / absorption.tif (float32 pages),
+// CHEMIN_RESULT and writes indice.tif / absorption.tif (float32 pages; in
+// polar mode indice_cplx_hdf5.hdf instead),
 // OTF3D.tif (EXPORT_OTF), centre_reconstruction.tif and slice.tiff. The
 // compute runs on the GPU through mtd_ewald_accumulate and mtd_finalize.
 #include <cmath>
 #include <fstream>
 
+#include "h5_volume.hpp"
 #include "tomo_common.hpp"
 
 using namespace tomo;
@@ -155,10 +157,21 @@ int main(int argc, char** argv) {
   bool ok = true;
   if (export_otf)
     ok &= mtd::write_tiff_f32(chemin_result + "/OTF3D.tif", otf.data(), V, V, V, pix_tomo, 4);
-  if (args.polar)
-    std::cout << "mode polar: volume written as TIFF (HDF5 output is not built)\n";
-  ok &= mtd::write_tiff_f32(chemin_result + "/indice.tif", ire.data(), V, V, V, pix_tomo, 4);
-  ok &= mtd::write_tiff_f32(chemin_result + "/absorption.tif", iim.data(), V, V, V, pix_tomo, 4);
+  // polar mode: HDF5 instead of the two TIFFs (main.cpp:360-367)
+  bool h5_done = false;
+  if (args.polar) {
+    const std::string h5path = chemin_result + "/indice_cplx_hdf5.hdf";
+    std::cout << "sauvegarde en hdf dans" << h5path << "\n";
+    h5_done = write_complex_volume_h5(h5path, ire.data(), iim.data(), V, V, V);
+    if (h5_done)
+      std::cout << "sauvegarde HDF!\n";
+    else
+      std::cout << "mode polar: libhdf5 not loadable, volume written as TIFF instead\n";
+  }
+  if (!h5_done) {
+    ok &= mtd::write_tiff_f32(chemin_result + "/indice.tif", ire.data(), V, V, V, pix_tomo, 4);
+    ok &= mtd::write_tiff_f32(chemin_result + "/absorption.tif", iim.data(), V, V, V, pix_tomo, 4);
+  }
   // extractSliceZ(indice, "x", 236) (main.cpp:375-376, fonctions.cpp:18-52)
   const int sx = 236;
   if (sx < V) {

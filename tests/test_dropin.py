@@ -122,3 +122,60 @@ def test_programs_round_trip(tmp_path, deroul, born):
     assert ire.shape == (V, V, V)
     assert rel_l2(ire + 1j * iim, idx32) < 1e-5
     assert np.array_equal(read_tiff_f32(res / "OTF3D.tif").astype(bool), otf32)
+
+
+H5DUMP = "/opt/conda/bin/h5dump"
+
+
+def _h5_dataset(path, name, tmp_path):
+    out = tmp_path / (name.strip("/").replace("/", "_") + ".bin")
+    r = subprocess.run([H5DUMP, "-d", name, "-b", "LE", "-o", str(out), str(path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return np.fromfile(out, "<f4")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump not in this image")
+def test_programs_polar_hdf5(tmp_path):
+    """-polar -etat /LC/polar0/: holograms from CHEMIN_RACINE/demosaic/<etat>,
+    outputs and the reconstruction's configs in .../UBorn/, the index volume
+    as indice_cplx_hdf5.hdf (/indice_complexe/Re, Im; Z, Y, X float32)."""
+    case = make_case("small")
+    print("drop-in polar case=small", flush=True)
+    env, res = setup_home(tmp_path, case)
+    home = tmp_path / "home"
+    racine = tmp_path / "racine"
+    etat = "/LC/polar0/"
+    acq = racine / "demosaic" / "LC" / "polar0"
+    ub = acq / "UBorn"
+    ub.mkdir(parents=True)
+    cfg = tmp_path / "config"
+    (home / ".config" / "gui_tomo_polar.conf").write_text(
+        f"CHEMIN_RESULT {res}\nCHEMIN_ACQUIS {tmp_path / 'acquis'}\nCHEMIN_CONFIG {cfg}\n"
+        f"CHEMIN_RACINE {racine}\n")
+    for k, h in enumerate(case.acq.holos):
+        write_pgm16(acq / ("i%03d.pgm" % k), h)
+    write_pgm8(racine / "Image_mask.pgm", case.acq.mask)
+    for f in ("recon.txt", "config_manip.txt"):
+        (ub / f).write_text((cfg / f).read_text())
+    r = subprocess.run([PRE, "-polar", "-etat", etat], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    A, C, V = len(case.acq.holos), case.g.C, case.g.dim_final
+    assert (ub / f"UBornfinal_Re{C}x{C}x{A}.raw").exists()
+    r = subprocess.run([REC, "-polar", "-etat", etat], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    h5 = ub / "indice_cplx_hdf5.hdf"
+    assert h5.exists() and not (ub / "indice.tif").exists()
+    hdr = subprocess.run([H5DUMP, "-H", str(h5)], capture_output=True, text=True, timeout=60)
+    assert f"( {V}, {V}, {V} )" in hdr.stdout and "H5T_IEEE_F32LE" in hdr.stdout
+    re = _h5_dataset(h5, "/indice_complexe/Re", tmp_path).reshape(V, V, V)
+    im = _h5_dataset(h5, "/indice_complexe/Im", tmp_path).reshape(V, V, V)
+    fre = np.fromfile(ub / f"UBornfinal_Re{C}x{C}x{A}.raw", "<f8")
+    fim = np.fromfile(ub / f"UBornfinal_Im{C}x{C}x{A}.raw", "<f8")
+    fields = (fre + 1j * fim).reshape(A, C, C).astype(np.complex64)
+    pos = np.fromfile(ub / "tab_posSpec.raw", "<f8").astype(np.int64)
+    idx32, _, _, _ = ref.reconstruct(fields, pos, case.g, "f32")
+    assert rel_l2(re + 1j * im, idx32) < 1e-5
