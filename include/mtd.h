@@ -189,6 +189,15 @@ int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re,
                         float* spectra);
 /* (posspec == NULL: the geometry plan of mtd_born_plan, below.) */
 
+/* The simulator's per-angle field synthesis (Simul_holo/main.cpp:174-180:
+ * decal2DCplxGen by -spec + dim/2, TF2Dcplx_INV x Delta_f_Uborn^2,
+ * fftshift2D before SAVCplx): spectra[A][C][C] c32 in the frame of
+ * mtd_forward_spectra (specular at posspec) -> fields[A][C][C] c32, the
+ * UBornfinal layout. The inverse of mtd_field_to_spectrum without its
+ * window. */
+int mtd_spectra_to_fields(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                          const int32_t* posspec, int A, float* fields);
+
 /* Geometry plan for iterative callers (no counterpart in the reference):
  * copies posspec[2A] and builds the per-angle cap geometry, the hit band and
  * the accumulation's per-tile angle lists once. mtd_forward_spectra,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
     "mtd_born_plan": (_I, [_P, _GP, _P, _I]),
+    "mtd_spectra_to_fields": (_I, [_P, _GP, _P, _P, _I, _P]),
     "mtd_tv_fista_step": (_I, [_P, _I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
                                ctypes.c_float, ctypes.c_float, ctypes.c_float, _I]),
 }
@@ -293,6 +294,13 @@ class Context:
                                                 _ptr(index_im)), "reconstruct_plain")
 
     # ---- Born forward model and adjoint ---------------------------------
+    def spectra_to_fields(self, g: Geom, spectra, posspec, fields):
+        """The simulator's field synthesis: fields[A][C][C] (UBornfinal layout)."""
+        self._stream()
+        self._check(lib().mtd_spectra_to_fields(self.h, ctypes.byref(g), _ptr(spectra),
+                                                _ptr(posspec), spectra.shape[0], _ptr(fields)),
+                    "spectra_to_fields")
+
     def born_plan(self, g: Geom, posspec):
         """Geometry plan: later forward/adjoint/weights calls with
         posspec=None reuse its angle geometry, band and tile lists."""

@@ -144,6 +144,11 @@ size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
                         const int* posy, int A, const float2* tw,
                         float2* work, float2* iu, hipStream_t s);
+// The simulator's field synthesis: shift by -posSpec, unnormalised inverse
+// FFT, shift by C/2, times scale (delta_f^2): fields[A][C][C].
+hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, const int* posy,
+                                    int A, const float2* tw, float2* work, float scale,
+                                    float2* fields, hipStream_t s);
 // phu: unwrapped phase [A][C][C] in the U frame, or nullptr for the wrapped
 // atan2 of IU (DEROUL 0)
 hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,

@@ -92,6 +92,24 @@ struct ColInPlaceOp {  // column pass reading and writing one stack
   }
 };
 
+// Column pass of the simulator's field synthesis: the IU frame moved to the
+// centred field frame (fftshift2D, a shift by C/2) and scaled by delta_f^2.
+template <int C>
+struct ColCenterScaleOp {
+  const float2* in;
+  float2* out;
+  const float2* tw;
+  float scale;
+  __device__ void begin(int) {}
+  __device__ float2 load(int a, int col, int row) const {
+    return in[((size_t)a * C + row) * C + col];
+  }
+  __device__ void store(int a, int col, int row, float2 x) const {
+    out[((size_t)a * C + (row + C / 2) % C) * C + (col + C / 2) % C] =
+        make_float2(x.x * scale, x.y * scale);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // least squares: per angle, FP64 right-hand sides A^T b for the phase and
 // amplitude fits over the sample set, then the LU solve.
@@ -468,6 +486,34 @@ hipError_t launch_argmax_fin(int C, const float2* part, int S, int A, int* posx,
                              hipStream_t s) {
   switch (C) {
 #define X(n) case n: return argmax_fin_c<n>(part, S, A, posx, posy, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int C>
+static hipError_t synth_fields_c(const float2* spec, const int* posx, const int* posy, int A,
+                                 const float2* tw, float2* work, float scale, float2* fields,
+                                 hipStream_t s) {
+  DemodRowOp<C> r{spec, posx, posy, work, tw, 0, 0};
+  {
+    Timed tm("synth_rows", s);
+    hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A), rows_block<C>(), 0,
+                       s, r);
+  }
+  ColCenterScaleOp<C> c{work, fields, tw, scale};
+  Timed tm("synth_cols", s);
+  hipLaunchKernelGGL((k_cols<C, true, ColCenterScaleOp<C>>), cols_grid<C>(A), cols_block<C>(),
+                     0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, const int* posy,
+                                    int A, const float2* tw, float2* work, float scale,
+                                    float2* fields, hipStream_t s) {
+  switch (C) {
+#define X(n) case n: return synth_fields_c<n>(spec, posx, posy, A, tw, work, scale, fields, s);
     MTD_CROP_SIZES(X)
 #undef X
   }

@@ -1008,6 +1008,30 @@ int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int
                    "cap weights");
 }
 
+int mtd_spectra_to_fields(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                          const int32_t* posspec, int A, float* fields) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, 0));
+  if (A == 0) return MTD_OK;
+  if (!spectra || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int C = 2 * g->nxmax;
+  const float2* twC = twiddle(ctx, C);
+  const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, A);
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
+  if (!twC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  // TF2Dcplx_INV's factor delta_f^2 (Simul_holo/src/FFT_fonctions.cpp:280-299)
+  const float scale = (float)(g->delta_f * g->delta_f);
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(hip_check(ctx, launch_spectra_to_fields(C, (const float2*)spectra + (size_t)a0 * C * C,
+                                                    posspec + a0, posspec + A + a0, n, twC, work,
+                                                    scale, (float2*)fields + (size_t)a0 * C * C,
+                                                    ctx->stream),
+                      "field synthesis"));
+  }
+  return MTD_OK;
+}
+
 int mtd_tv_fista_step(mtd_ctx* ctx, int V, const float* z, const float* grad_data, float* x,
                       float* z_next, float data_scale, float lam, float eps, float step,
                       float beta, int positivity) {

@@ -680,6 +680,18 @@ def finalize(vol, cnt, g: Geom, prec="f64"):
 PI_SIMUL = 3.14159
 
 
+def spectra_to_fields(S, pos, g: Geom):
+    """The simulator's field synthesis (Simul_holo/main.cpp:174-180):
+    decal2DCplxGen moving the specular to bin 0, TF2Dcplx_INV (FFTW backward
+    x Delta_f^2, Simul_holo/src/FFT_fonctions.cpp:280-299), fftshift2D. FP64."""
+    A = len(pos) // 2
+    out = np.empty((A,) + S.shape[1:], np.complex128)
+    for a in range(A):
+        t = decal2d(np.asarray(S[a], np.complex128), -int(pos[a]), -int(pos[a + A]))
+        out[a] = fftshift2(np.fft.ifft2(t, norm="forward") * g.delta_f ** 2)
+    return out
+
+
 def tomo_pixel(g: Geom):
     """Voxel size 1/(V delta_f) (Simul_holo/src/manip.cpp:59: Tp_Tomo =
     Tp_Uborn * 2 NXMAX / dim_final)."""

@@ -179,3 +179,80 @@ def test_programs_polar_hdf5(tmp_path):
     pos = np.fromfile(ub / "tab_posSpec.raw", "<f8").astype(np.int64)
     idx32, _, _, _ = ref.reconstruct(fields, pos, case.g, "f32")
     assert rel_l2(re + 1j * im, idx32) < 1e-5
+
+
+SIM = os.path.join(m.BIN_DIR, "tomo_simul_holo")
+
+
+@pytest.mark.gpu
+def test_simulator_then_reconstruction(tmp_path):
+    """tomo_simul_holo as Simul_holo (block object, 4-axis flower scan, fixed
+    2048 hologram side) writes the fields, speculars and parameters that
+    tomo_reconstruction then reads; fields against the oracle's forward
+    model + field synthesis, index volume against the oracle's
+    reconstruction of the written fields."""
+    import math
+    print("simulator -> reconstruction", flush=True)
+    home, res, cfg = (tmp_path / d for d in ("home", "result", "config"))
+    for d in (home / ".config", res, cfg):
+        d.mkdir(parents=True)
+    A, nx, N = 24, 16, 2048
+    tpcam = 16.3 * LAM / (NA * N)      # NXMAX = (uint16)(NA N Tp / lambda) = 16
+    (home / ".config" / "gui_tomo_polar.conf").write_text(
+        f"CHEMIN_RESULT {res}\nCHEMIN_CONFIG {cfg}\n")
+    (cfg / "config_manip.txt").write_text(
+        f"NB_HOLO {A}\nN0 {N0}\nNA_OBJ {NA}\nNA_COND {NA}\nLAMBDA {LAM!r}\nF_TUBE 1\nF_OBJ 1\n"
+        f"RF 1\nTPCAM {tpcam!r}\n")
+    env = dict(os.environ, HOME=str(home))
+    r = subprocess.run([SIM], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    C, V = 2 * nx, 4 * nx
+    f32 = lambda v: float(np.float32(v))
+    rayon = float(ref.round_half_away(nx / math.sin(math.asin(f32(NA) / f32(N0)))))
+    param = np.fromfile(res / "parametres.raw", "<f8")
+    assert list(param[:4]) == [nx, A, rayon, N]
+    # the flower pattern, from OTF::bFleur's formula
+    th = 2 * math.pi / A * np.arange(A)
+    sx = ref.round_half_away((nx - 1) * np.cos(4 * th) * np.cos(th))
+    sy = ref.round_half_away((nx - 1) * np.cos(4 * th) * np.sin(th))
+    pos = np.concatenate([sx + C // 2, -sy + C // 2]).astype(np.int64)
+    assert np.array_equal(np.fromfile(res / "tab_posSpec.raw", "<f8"), pos.astype(float))
+    # the block of genere_barre on the reconstruction grid
+    g = ref.derive_geom(N, nx, V, N // 2, N // 2, NA, N0, LAM, tpcam, 1, 1, 1)
+    g.rayon = rayon
+    tp = ref.tomo_pixel(g)
+    lo = [int(ref.round_half_away(v / tp)) for v in (-2.5e-6, -2.5e-6, -2.5e-6)]
+    hi = [int(ref.round_half_away(v / tp)) for v in (2.5e-6, 2.5e-6, 2.5e-6)]
+    obj = np.zeros((V, V, V))
+    obj[lo[2] + V // 2:hi[2] + V // 2, lo[1] + V // 2:hi[1] + V // 2,
+        lo[0] + V // 2:hi[0] + V // 2] = f32(1.395 - f32(N0))
+    want = ref.spectra_to_fields(ref.born_forward(obj, pos, g, "f64"), pos, g)
+    dim = f"{C}x{C}x{A}"
+    got = (np.fromfile(res / f"UBornfinal_Re{dim}.raw", "<f8")
+           + 1j * np.fromfile(res / f"UBornfinal_Im{dim}.raw", "<f8")).reshape(A, C, C)
+    e = rel_l2(got, want)
+    print(f"  simulated fields rel-L2 vs f64 {e:.3e}", flush=True)
+    assert e < 1e-5
+    assert (res.parent / (res.name + "centres_simul.tif")).exists()
+    # tomo_reconstruction on the simulator's output
+    cfg2 = tmp_path / "config2"
+    cfg2.mkdir()
+    (home / ".config" / "gui_tomo.conf").write_text(
+        f"CHEMIN_RESULT {res}\nCHEMIN_ACQUIS {tmp_path}\nCHEMIN_CONFIG {cfg2}\n")
+    (cfg2 / "config_manip.txt").write_text(
+        f"NA {NA}\nN0 {N0}\nLAMBDA {LAM!r}\nF_TUBE 1\nF_OBJ 1\nTPCAM {tpcam!r}\nRF 1\n"
+        f"NXMAX {nx}\nCIRCLE_CX {N // 2}\nCIRCLE_CY {N // 2}\nNB_HOLO {A}\n")
+    (cfg2 / "recon.txt").write_text(
+        f"CCD_ROIX {N}\nDEROUL 0\nVOLKOV 1\nC_ABER 1\nBORN 1\nNB_THREADS 4\nPREMIER_ANGLE 0\n"
+        f"DIM_FINAL {V}\nEXPORT_OTF 0\n")
+    r = subprocess.run([REC], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    gr = ref.derive_geom(N, nx, V, N // 2, N // 2, NA, N0, LAM, tpcam, 1, 1, 1)
+    idx32, _, _, _ = ref.reconstruct(got.astype(np.complex64), pos, gr, "f32")
+    ire = read_tiff_f32(res / "indice.tif")
+    iim = read_tiff_f32(res / "absorption.tif")
+    assert rel_l2(ire + 1j * iim, idx32) < 1e-5
+    # and the reconstruction sees the block
+    c = float((ire * obj).sum() / (np.linalg.norm(ire) * np.linalg.norm(obj)))
+    print(f"  reconstructed index vs block: correlation {c:.3f}", flush=True)
+    assert c > 0.5
