@@ -29,6 +29,8 @@ WORKLOADS = {
     "cfg2": dict(n=1024, nxmax=128, v=512, A=600, seed=2024),
     # differentiation form: plain vs adjoint-TV index RMSE against the phantom
     "diff": dict(n=768, nxmax=64, v=256, A=400, seed=2025),
+    # large combined configuration (plain method only): --workload large
+    "large": dict(n=2048, nxmax=256, v=1024, A=1000, seed=2026),
 }
 # adjoint-TV settings of the quality comparison (objective in index units:
 # 1/2 ||A dn - y||_W^2 / ||A^H W A|| + lam * TV_eps(dn)); the RMSE curve is
@@ -351,7 +353,7 @@ def main():
     achieved = bpl / avg_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.workload == "cfg2":  # measured on that workload
         traffic = json.load(open(tpath)).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -359,7 +361,7 @@ def main():
                 "ms_per_launch": round(avg_s * 1e3, 5)}
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample > 0 and args.workload == "cfg2":
         r = cpu_baseline(holos, mask, args.cpu_sample)
         per_holo = (r["pre"] + r["rec"]) / r["S"]
         cpu_ms_volume = (per_holo * A + r["fin"]) * 1e3

@@ -38,13 +38,14 @@ constexpr int rows_nt() {
   return divisor_le(L, by_thr < by_lds ? by_thr : by_lds);
 }
 // Column passes: NC adjacent columns per block, up to 32 (128-byte segments
-// even for float outputs), at most 1024 threads and 64 KiB LDS; lines
-// longer than 512 keep 8 columns (their register FFTs allow 3 waves per
-// SIMD, which 4-wave blocks fill).
+// even for float outputs), at most 1024 threads and 64 KiB LDS; lines of
+// 513-1023 keep 256 threads (their register FFTs allow 3 waves per SIMD,
+// which 4-wave blocks fill), lines from 1024 take 512 (2 waves per SIMD:
+// one 8-wave block per CU either way, and twice the row segment).
 // MAXT lowers the thread cap (see op_col_threads).
 template <int L, int MAXT = 1024>
 constexpr int cols_nc() {
-  constexpr int cap = L > 512 ? 256 : 1024;
+  constexpr int cap = L >= 1024 ? 512 : L > 512 ? 256 : 1024;
   constexpr int by_thr = (MAXT < cap ? MAXT : cap) / Fft2<L, false>::T;
   constexpr int by_lds = kLdsBlockMax / (Fft2<L, false>::LDS_COL * 4);
   constexpr int m = by_thr < by_lds ? by_thr : by_lds;
@@ -68,11 +69,12 @@ struct op_col_threads<Op, std::void_t<decltype(Op::kColThreads)>> {
 };
 // Column blocks are up to 1024 threads; ask the compiler for at least 4
 // waves per SIMD (at most 128 VGPRs) so that a CU keeps 16 waves in flight,
-// 3 for lines longer than 512 (a 32 + 32 point register FFT needs more).
+// 3 for lines longer than 512 and 2 from 1024 (a 32 + 32 point register FFT
+// needs about 170 VGPRs: 3 waves would spill).
 // (HIP's second __launch_bounds__ argument is amdgpu_waves_per_eu.)
 template <int L>
 constexpr int col_waves_per_simd() {
-  return L > 512 ? 3 : 4;
+  return L >= 1024 ? 2 : L > 512 ? 3 : 4;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs in dispatch order (linear

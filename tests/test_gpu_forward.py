@@ -114,11 +114,12 @@ def test_adjoint_spectra(gpu_ctx, name):
     assert e64 < TOL
 
 
-@pytest.mark.parametrize("nx,V,A", [(16, 64, 24), (64, 256, 400)])
+@pytest.mark.parametrize("nx,V,A", [(16, 64, 24), (64, 256, 400), (256, 1024, 32)])
 def test_gpu_pair_is_adjoint(gpu_ctx, nx, V, A):
     """<A x, y> = <x, A^H y> with the GPU forward and adjoint (complex x);
     the second size is the differentiation workload (400 angles, 128^2
-    spectra, 256^3 volume)."""
+    spectra, 256^3 volume), the third the large configuration's 512^2
+    spectra and 1024^3 volume (its forward and inverse 1024-point passes)."""
     import torch
     _say(f"adjointness nx={nx} V={V} A={A}")
     g, gl = _geoms(nx, V)
@@ -135,9 +136,12 @@ def test_gpu_pair_is_adjoint(gpu_ctx, nx, V, A):
     gpu_ctx.adjoint_spectra(gl, y, pos, ar, ai)
     torch.cuda.synchronize()
     lhs = torch.vdot(y.ravel().to(torch.complex128), Ax.ravel().to(torch.complex128)).item()
-    aH = torch.complex(ar, ai).to(torch.complex128)
-    x = torch.complex(xr, xi).to(torch.complex128)
-    rhs = torch.vdot(aH.ravel(), x.ravel()).item()
+    # <x, A^H y> = sum conj(A^H y) x, accumulated in float64 plane by plane
+    rhs = 0j
+    for z in range(V):
+        ar_, ai_ = ar[z].double(), ai[z].double()
+        xr_, xi_ = xr[z].double(), xi[z].double()
+        rhs += complex(float((ar_ * xr_ + ai_ * xi_).sum()), float((ar_ * xi_ - ai_ * xr_).sum()))
     _say(f"  <Ax,y> {lhs:.6e}  <x,A^H y> {rhs:.6e}")
     assert abs(lhs - rhs) <= TOL * abs(lhs)
 
