@@ -312,21 +312,23 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     print(f"[bench] {args.steps} steps, {ms:.3f} ms/volume", file=sys.stderr, flush=True)
 
-    # PCIe-inclusive rate: the holograms start in pinned host memory and are
-    # copied to HBM inside each timed step (the executables' situation)
+    # PCIe-inclusive rate: the holograms start in pinned host memory and the
+    # library streams them to HBM inside each timed step, each chunk's copy
+    # overlapped with the previous chunk's transform (the executables' case)
     host = torch.empty(holos.shape, dtype=holos.dtype, pin_memory=True)
     host.copy_(holos)
+    ctx.reconstruct_plain(g, host, mask, re, im, pos)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        holos.copy_(host, non_blocking=True)
-        ctx.reconstruct_plain(g, holos, mask, re, im, pos)
+        ctx.reconstruct_plain(g, host, mask, re, im, pos)
     torch.cuda.synchronize()
     ms_pcie = (time.perf_counter() - t0) * 1e3 / args.steps
     del host
     pcie = {"value": round(A / (ms_pcie / 1e3), 2), "unit": "holograms/s",
             "ms_per_step": round(ms_pcie, 4),
-            "note": "pinned host holograms copied to HBM inside the step"}
+            "note": "pinned host holograms streamed to HBM by the library inside the step "
+                    "(copies overlapped with the hologram stage)"}
 
     # roofline of the dominant kernel
     P = int(sum(1 for y in range(-nx, nx) for x in range(-nx, nx) if x * x + y * y < nx * nx))

@@ -16,6 +16,11 @@
  *  - Array arguments are DEVICE pointers (hipMalloc / torch CUDA tensors)
  *    unless the comment says "host". The caller owns them; the context owns
  *    its scratch workspaces, twiddle tables and fit plans.
+ *  - Hologram stacks (`holos` of mtd_holo_to_crop, mtd_preprocess,
+ *    mtd_reconstruct_plain) may also be HOST memory: pinned (hipHostMalloc)
+ *    or pageable. They are then copied chunk by chunk on a second stream of
+ *    the context, each copy overlapped with the transform of the previous
+ *    chunk (pageable memory is staged by the runtime and overlaps less).
  *  - Calls are asynchronous on the context's stream (mtd_ctx_set_stream);
  *    mtd_ctx_sync() waits. One context per host thread.
  *  - Complex arrays are interleaved float pairs (re, im) -- "c32".

@@ -162,6 +162,16 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _ptr_holos(t):
+    """Hologram stacks may be GPU tensors or host (pinned or pageable) CPU
+    tensors: the library streams host stacks to the GPU itself."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise MTDError("hologram arrays must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def _dtype_code(t):
     import torch
     if t.dtype == torch.uint8:
@@ -242,7 +252,7 @@ class Context:
     # ---- stages ---------------------------------------------------------
     def holo_to_crop(self, g: Geom, holos, crops):
         self._stream()
-        self._check(lib().mtd_holo_to_crop(self.h, ctypes.byref(g), _ptr(holos),
+        self._check(lib().mtd_holo_to_crop(self.h, ctypes.byref(g), _ptr_holos(holos),
                                            _dtype_code(holos), holos.shape[0],
                                            _ptr(crops)), "holo_to_crop")
 
@@ -256,7 +266,7 @@ class Context:
     def preprocess(self, g: Geom, holos, mask, posspec, fields):
         self._stream()
         keep, mp = _mask_ptr(mask, g.C)
-        self._check(lib().mtd_preprocess(self.h, ctypes.byref(g), _ptr(holos),
+        self._check(lib().mtd_preprocess(self.h, ctypes.byref(g), _ptr_holos(holos),
                                          _dtype_code(holos), holos.shape[0], mp,
                                          _ptr(posspec), _ptr(fields)), "preprocess")
 
@@ -288,7 +298,7 @@ class Context:
     def reconstruct_plain(self, g: Geom, holos, mask, index_re, index_im, posspec=None):
         self._stream()
         keep, mp = _mask_ptr(mask, g.C)
-        self._check(lib().mtd_reconstruct_plain(self.h, ctypes.byref(g), _ptr(holos),
+        self._check(lib().mtd_reconstruct_plain(self.h, ctypes.byref(g), _ptr_holos(holos),
                                                 _dtype_code(holos), holos.shape[0], mp,
                                                 _ptr(posspec), _ptr(index_re),
                                                 _ptr(index_im)), "reconstruct_plain")

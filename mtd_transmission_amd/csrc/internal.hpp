@@ -54,6 +54,10 @@ struct EvPair {
 struct mtd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // host-to-device hologram streaming: copy stream and per-slot events
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_ready[2] = {nullptr, nullptr};
+  hipEvent_t ev_free[2] = {nullptr, nullptr};
   std::string err;
   std::map<int, float2*> twiddles;                       // per FFT length
   std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)
@@ -108,7 +112,7 @@ struct Timed {
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
   WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART,
-  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER
+  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER, WS_HSTAGE
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);

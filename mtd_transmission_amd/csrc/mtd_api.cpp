@@ -476,13 +476,77 @@ static size_t field_bytes(const mtd_geom* g) {
 
 // holograms [f0, f0+n) -> crops (and argmax partials, S per angle), in
 // hologram-stage chunks
+// Host memory (pinned, or pageable: the runtime then stages each copy) as
+// reported by the HIP runtime; device and managed memory are read in place.
+static bool is_host_ptr(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();  // unregistered pointer: pageable host memory
+    return true;
+  }
+  return at.type == hipMemoryTypeHost || at.type == hipMemoryTypeUnregistered;
+}
+
+// The context's copy stream and its slot events, created on first use
+// (default priority, non-blocking with respect to the null stream).
+static int copy_engine(mtd_ctx* ctx) {
+  if (ctx->copy_stream) return MTD_OK;
+  MTD_TRY(hip_check(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking),
+                    "copy stream"));
+  for (int k = 0; k < 2; ++k) {
+    MTD_TRY(hip_check(ctx, hipEventCreateWithFlags(&ctx->ev_ready[k], hipEventDisableTiming),
+                      "copy event"));
+    MTD_TRY(hip_check(ctx, hipEventCreateWithFlags(&ctx->ev_free[k], hipEventDisableTiming),
+                      "copy event"));
+  }
+  return MTD_OK;
+}
+
+// Hologram stage over angles f0 .. f0+n-1 in chunks of at most
+// kChunkBudget of row-pass intermediate. Host holograms are streamed: chunk
+// i+1 is copied into one of two device slots on the copy stream while chunk
+// i is transformed; the copy into a slot waits for the row pass that last
+// read it, the row pass waits for its copy.
 static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
                       int n, float2* crop, float2* part, int S) {
   const int N = g->n, C = 2 * g->nxmax;
   const int hc = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
-  for (int h0 = 0; h0 < n; h0 += hc)
-    MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
-                       crop + (size_t)h0 * C * C, part + (size_t)h0 * S));
+  auto part_at = [&](int h0) { return part ? part + (size_t)h0 * S : nullptr; };
+  if (!is_host_ptr(holos)) {
+    for (int h0 = 0; h0 < n; h0 += hc)
+      MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
+                         crop + (size_t)h0 * C * C, part_at(h0)));
+    return MTD_OK;
+  }
+  MTD_TRY(copy_engine(ctx));
+  const size_t hb = (size_t)N * N * dtype_bytes(dtype);
+  char* slots = (char*)ws_get(ctx, WS_HSTAGE, 2 * (size_t)hc * hb);
+  if (!slots) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram staging)");
+  const int nchunks = (n + hc - 1) / hc;
+  bool used[2] = {false, false};
+  auto prefetch = [&](int i) -> int {
+    const int k = i % 2, h0 = i * hc, m = std::min(hc, n - h0);
+    if (used[k])
+      MTD_TRY(hip_check(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->ev_free[k], 0),
+                        "copy wait"));
+    MTD_TRY(hip_check(ctx, hipMemcpyAsync(slots + (size_t)k * hc * hb,
+                                          (const char*)holos + (size_t)(f0 + h0) * hb,
+                                          (size_t)m * hb, hipMemcpyHostToDevice,
+                                          ctx->copy_stream),
+                      "hologram upload"));
+    MTD_TRY(hip_check(ctx, hipEventRecord(ctx->ev_ready[k], ctx->copy_stream), "copy event"));
+    used[k] = true;
+    return MTD_OK;
+  };
+  MTD_TRY(prefetch(0));
+  for (int i = 0; i < nchunks; ++i) {
+    if (i + 1 < nchunks) MTD_TRY(prefetch(i + 1));
+    const int k = i % 2, h0 = i * hc, m = std::min(hc, n - h0);
+    MTD_TRY(hip_check(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_ready[k], 0), "compute wait"));
+    MTD_TRY(holo_chunk(ctx, g, slots + (size_t)k * hc * hb, dtype, 0, m,
+                       crop + (size_t)h0 * C * C, part_at(h0)));
+    MTD_TRY(hip_check(ctx, hipEventRecord(ctx->ev_free[k], ctx->stream), "slot event"));
+  }
   return MTD_OK;
 }
 
@@ -528,6 +592,14 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (auto& kv : ctx->beta2tab) (void)hipFree(kv.second);
   for (auto& kv : ctx->qtab) (void)hipFree(kv.second);
   for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);
+  if (ctx->copy_stream) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    for (int k = 0; k < 2; ++k) {
+      (void)hipEventDestroy(ctx->ev_ready[k]);
+      (void)hipEventDestroy(ctx->ev_free[k]);
+    }
+    (void)hipStreamDestroy(ctx->copy_stream);
+  }
   (void)hipFree(ctx->lsq.d_samples);
   (void)hipFree(ctx->lsq.d_lu);
   (void)hipFree(ctx->lsq.d_perm);
@@ -626,13 +698,7 @@ int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dty
   MTD_TRY(validate(ctx, g, A, NEED_HOLO));
   if (A == 0) return MTD_OK;
   if (!holos || !crops) return set_err(ctx, MTD_ERR_ARG, "null array");
-  const int N = g->n, C = 2 * g->nxmax;
-  const int chunk = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, A);
-  for (int a0 = 0; a0 < A; a0 += chunk) {
-    const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_chunk(ctx, g, holos, dtype, a0, n, (float2*)crops + (size_t)a0 * C * C));
-  }
-  return MTD_OK;
+  return holo_range(ctx, g, holos, dtype, 0, A, (float2*)crops, nullptr, 0);
 }
 
 int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A,

@@ -4,6 +4,7 @@
 // outputs in CHEMIN_RESULT: UBornfinal_{Re,Im}CxCxA.raw (f64), parametres.raw,
 // tab_posSpec.raw, centres.tif, SAV_param_manip.txt. The compute runs on the
 // GPU through mtd_preprocess.
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 
@@ -69,8 +70,12 @@ int main(int argc, char** argv) {
   // hologram loop (main.cpp:130-165): missing files are skipped and the
   // angles compacted
   const long nb_angle = nb_holo - premier > 0 ? nb_holo - premier : 0;
-  std::vector<uint16_t> holos;
-  holos.reserve((size_t)nb_angle * N * N);
+  // pinned host stack: the library streams it to the GPU, overlapping the
+  // copies with the hologram transforms
+  uint16_t* holos = nullptr;
+  if (nb_angle > 0)
+    hip_or_die(hipHostMalloc((void**)&holos, (size_t)nb_angle * N * N * sizeof(uint16_t), 0),
+               "hipHostMalloc");
   int A = 0;
   Clock t_load;
   for (long k = 0; k < nb_angle; ++k) {
@@ -90,7 +95,8 @@ int main(int argc, char** argv) {
       return 1;
     }
     for (int y = 0; y < N; ++y)
-      holos.insert(holos.end(), px.begin() + (size_t)y * w, px.begin() + (size_t)y * w + N);
+      std::copy(px.begin() + (size_t)y * w, px.begin() + (size_t)y * w + N,
+                holos + ((size_t)A * N + y) * N);
     ++A;
   }
   std::cout << "NbAngle=" << nb_angle << " trouves=" << A << "\n";
@@ -126,20 +132,18 @@ int main(int argc, char** argv) {
   std::vector<int32_t> pos((size_t)2 * A);
   Clock t_gpu;
   if (A > 0) {
-    void *d_h = nullptr, *d_f = nullptr, *d_p = nullptr;
-    hip_or_die(hipMalloc(&d_h, holos.size() * 2), "hipMalloc");
+    void *d_f = nullptr, *d_p = nullptr;
     hip_or_die(hipMalloc(&d_f, fields.size() * 4), "hipMalloc");
     hip_or_die(hipMalloc(&d_p, pos.size() * 4), "hipMalloc");
-    hip_or_die(hipMemcpy(d_h, holos.data(), holos.size() * 2, hipMemcpyHostToDevice), "upload");
-    mtd_or_die(ctx, mtd_preprocess(ctx, &g, d_h, MTD_U16, A, maskp, (int32_t*)d_p, (float*)d_f),
+    mtd_or_die(ctx, mtd_preprocess(ctx, &g, holos, MTD_U16, A, maskp, (int32_t*)d_p, (float*)d_f),
                "preprocess");
     mtd_or_die(ctx, mtd_ctx_sync(ctx), "sync");
     hip_or_die(hipMemcpy(fields.data(), d_f, fields.size() * 4, hipMemcpyDeviceToHost), "download");
     hip_or_die(hipMemcpy(pos.data(), d_p, pos.size() * 4, hipMemcpyDeviceToHost), "download");
-    (void)hipFree(d_h);
     (void)hipFree(d_f);
     (void)hipFree(d_p);
   }
+  if (holos) (void)hipHostFree(holos);
   std::cout << "Temps pour FFT holo+découpe Spectre+part2 (GPU)= " << t_gpu.s() << "\n";
 
   // outputs (main.cpp:171-173,325-326,351-358)
