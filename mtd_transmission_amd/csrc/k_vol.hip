@@ -88,6 +88,7 @@ struct TileEntry {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 constexpr int kBatch = 8;  // hits whose loads are issued together
+constexpr int kPrepBatch = 8;  // prepass: height-table loads issued together
 
 // Prepass, one block per 16x16 column tile. Angles are taken 256 at a time;
 // the active ones (pupil disc touches the tile) are appended in angle order
@@ -111,6 +112,7 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
   TileEntry* lst = list + (size_t)tile * g.A;
   uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
   int running = 0, zmin = 0x7fffffff, zmax = -1;
+  const double R2 = g.htab[0] * g.htab[0];  // h(0) = R, an integer: exact
   for (int a0 = 0; a0 < g.A; a0 += 256) {
     const int a = a0 + tid;
     bool act = false;
@@ -153,21 +155,24 @@ k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restr
     }
     __syncthreads();
     // exact hit heights of this thread's column for the n new entries
-    for (int i0 = 0; i0 < n; i0 += kBatch) {
-      // branch-free batch: the table loads of kBatch entries are in flight
+    for (int i0 = 0; i0 < n; i0 += kPrepBatch) {
+      // branch-free batch: the table loads of kPrepBatch entries are in flight
       // together (slots past n repeat the last entry and are not stored)
-      double hb[kBatch];
-      bool inp[kBatch];
+      double hb[kPrepBatch];
+      bool inp[kPrepBatch];
 #pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
+      for (int t = 0; t < kPrepBatch; ++t) {
         const int4 e = sent[min(i0 + t, n - 1)];
         const int fdx = qx + e.z, fdy = qy + e.w;
         const int r2 = fdx * fdx + fdy * fdy;
         inp[t] = fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max;
-        hb[t] = g.htab[inp[t] ? r2 : 0];
+        // h(r2) = sqrt(R^2 - r2) computed here: IEEE sqrt of the same exact
+        // double is the host table's value bit for bit, without a 64-way
+        // scattered table load per entry
+        hb[t] = sqrt(R2 - (double)(inp[t] ? r2 : 0));
       }
 #pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
+      for (int t = 0; t < kPrepBatch; ++t) {
         const int it = min(i0 + t, n - 1);
         const int z = (int)round_away(hb[t] - sfz[it]) + g.V / 2;
         const uint8_t o = inp[t] ? (uint8_t)(z - sent[it].y) : (uint8_t)255;
