@@ -88,110 +88,141 @@ struct TileEntry {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 constexpr int kBatch = 8;  // hits whose loads are issued together
-constexpr int kPrepBatch = 8;  // prepass: height-table loads issued together
+constexpr int kPrepBatch = 8;  // prepass: entries whose heights are computed together
+constexpr int kTileChunk = 64;  // prepass: angles per block
 
-// Prepass, one block per 16x16 column tile. Angles are taken 256 at a time;
-// the active ones (pupil disc touches the tile) are appended in angle order
-// by a block scan, with a conservative height range from the tile box's
-// nearest and farthest pupil radii (h decreases with r^2, round is
-// monotonic). Right after each scan the new entries are staged in LDS and
-// every column computes its exact hit height for each of them, stored as a
-// byte offset from the entry's lower bound (255 = no hit): FP64 heights and
-// round-half-away, as the reference.
+// h(r2) = sqrt(R^2 - r2) for 0 <= r2 <= R^2: IEEE sqrt of the same exact
+// double, so the host height table's value bit for bit
+__device__ __forceinline__ double height(double R2, int r2) { return sqrt(R2 - (double)r2); }
+
+// Does angle a's pupil disc touch the 16x16 column tile at (qx0, qy0)? If so,
+// the nearest and farthest pupil radii^2 of the tile box (the farthest
+// clamped to the pupil edge).
+__device__ __forceinline__ bool tile_touch(const GatherArgs& g, const AngleGeo& ag, int qx0,
+                                           int qy0, int& dmin, int& dmax) {
+  if (ag.skip) return false;
+  const int cxl = max(qx0 + ag.fx, -g.nx), cxh = min(qx0 + GT - 1 + ag.fx, g.nx - 1);
+  const int cyl = max(qy0 + ag.fy, -g.nx), cyh = min(qy0 + GT - 1 + ag.fy, g.nx - 1);
+  if (cxl > cxh || cyl > cyh) return false;
+  const int mx = cxl > 0 ? cxl : (cxh < 0 ? -cxh : 0);
+  const int my = cyl > 0 ? cyl : (cyh < 0 ? -cyh : 0);
+  const int Mx = max(abs(cxl), abs(cxh)), My = max(abs(cyl), abs(cyh));
+  dmin = mx * mx + my * my;
+  dmax = min(Mx * Mx + My * My, g.r2max);
+  return dmin <= g.r2max;
+}
+
+__device__ __forceinline__ int block_sum256(int v, int* wsum) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();  // wsum is reused
+  return t;
+}
+
+// Prepass: the angles whose pupil disc touches each 16x16 column tile, in
+// angle order, with a conservative height range from the tile box's nearest
+// and farthest pupil radii (h decreases with r^2, round is monotonic), and
+// every column's exact hit height per listed angle as a byte offset from the
+// entry's lower bound (255 = no hit): FP64 heights and round-half-away, as
+// the reference. One block per (tile, chunk of kTileChunk angles), so that
+// the tiles near the lateral origin, which nearly every angle touches, are
+// spread over many blocks: a block first counts the tile's entries of the
+// earlier chunks (its offset in the tile's list; the chunk-0 block counts all
+// of them and writes the tile's total and height range), then lists its own
+// chunk and computes the heights of those entries, one column per thread.
 __global__ void __launch_bounds__(256)
-k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo,
-              uint8_t* __restrict__ zoff) {
+k_tile_angles(GatherArgs g, int ntx, int nchunk, TileEntry* __restrict__ list,
+              int4* __restrict__ tinfo, uint8_t* __restrict__ zoff) {
   __shared__ int wsum[4];
   __shared__ int zr[2][4];
-  __shared__ int4 sent[256];   // a, zl, fx, fy of this scan's entries
-  __shared__ double sfz[256];
-  const int tile = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int nown;
+  __shared__ int4 sent[kTileChunk];  // a, zl, fx, fy of this chunk's entries
+  __shared__ double sfz[kTileChunk];
+  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int qx0 = (tile % ntx) * GT - g.V / 2, qy0 = (tile / ntx) * GT - g.V / 2;
   const int qx = qx0 + tid % GT, qy = qy0 + tid / GT;  // this thread's column
-  TileEntry* lst = list + (size_t)tile * g.A;
-  uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
-  int running = 0, zmin = 0x7fffffff, zmax = -1;
   const double R2 = g.htab[0] * g.htab[0];  // h(0) = R, an integer: exact
-  for (int a0 = 0; a0 < g.A; a0 += 256) {
-    const int a = a0 + tid;
+  const int a_own = chunk * kTileChunk;
+  // entries of the earlier chunks; chunk 0: all entries and the height range
+  const int lim = chunk == 0 ? g.A : a_own;
+  int cnt = 0, zmin = 0x7fffffff, zmax = -1;
+  for (int a = tid; a < lim; a += 256) {
+    const AngleGeo ag = g.ageo[a];
+    int dmin, dmax;
+    if (tile_touch(g, ag, qx0, qy0, dmin, dmax)) {
+      ++cnt;
+      if (chunk == 0) {
+        zmin = min(zmin, clampi((int)round_away(height(R2, dmax) - ag.fz) + g.V / 2, 0, g.V - 1));
+        zmax = max(zmax, clampi((int)round_away(height(R2, dmin) - ag.fz) + g.V / 2, 0, g.V - 1));
+      }
+    }
+  }
+  const int total = block_sum256(cnt, wsum);
+  if (chunk == 0) {
+    for (int o = 32; o > 0; o >>= 1) {
+      zmin = min(zmin, __shfl_xor(zmin, o, 64));
+      zmax = max(zmax, __shfl_xor(zmax, o, 64));
+    }
+    if (lane == 0) { zr[0][tid >> 6] = zmin; zr[1][tid >> 6] = zmax; }
+    __syncthreads();
+    if (tid == 0) {
+      const int lo = min(min(zr[0][0], zr[0][1]), min(zr[0][2], zr[0][3]));
+      const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
+      tinfo[tile] = make_int4(total, lo, hi, 0);
+    }
+  }
+  const int off = chunk == 0 ? 0 : total;
+  // this chunk's entries, in angle order (one wave)
+  if (tid < 64) {
+    const int a = a_own + lane;
     bool act = false;
     int zl = 0, zh = 0;
     AngleGeo ag{};
     if (a < g.A) {
       ag = g.ageo[a];
-      if (!ag.skip) {
-        const int cxl = max(qx0 + ag.fx, -g.nx), cxh = min(qx0 + GT - 1 + ag.fx, g.nx - 1);
-        const int cyl = max(qy0 + ag.fy, -g.nx), cyh = min(qy0 + GT - 1 + ag.fy, g.nx - 1);
-        if (cxl <= cxh && cyl <= cyh) {
-          const int mx = cxl > 0 ? cxl : (cxh < 0 ? -cxh : 0);
-          const int my = cyl > 0 ? cyl : (cyh < 0 ? -cyh : 0);
-          const int Mx = max(abs(cxl), abs(cxh)), My = max(abs(cyl), abs(cyh));
-          const int dmin = mx * mx + my * my;
-          if (dmin <= g.r2max) {
-            const int dmax = min(Mx * Mx + My * My, g.r2max);
-            zl = clampi((int)round_away(g.htab[dmax] - ag.fz) + g.V / 2, 0, g.V - 1);
-            zh = clampi((int)round_away(g.htab[dmin] - ag.fz) + g.V / 2, 0, g.V - 1);
-            act = true;
-          }
-        }
+      int dmin, dmax;
+      if (tile_touch(g, ag, qx0, qy0, dmin, dmax)) {
+        zl = clampi((int)round_away(height(R2, dmax) - ag.fz) + g.V / 2, 0, g.V - 1);
+        zh = clampi((int)round_away(height(R2, dmin) - ag.fz) + g.V / 2, 0, g.V - 1);
+        act = true;
       }
     }
-    // deterministic block scan of the active flags (angle order kept)
     const unsigned long long bal = __ballot(act);
-    const int before = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wave] = __popcll(bal);
-    __syncthreads();
-    int off = 0;
-    for (int w = 0; w < wave; ++w) off += wsum[w];
-    const int n = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     if (act) {
-      const int p = off + before;
-      lst[running + p] = TileEntry{a, zl | (zh << 16), (ag.fx & 0xffff) | (ag.fy << 16), 0};
+      const int p = __popcll(bal & ((1ull << lane) - 1ull));
+      list[(size_t)tile * g.A + off + p] =
+          TileEntry{a, zl | (zh << 16), (ag.fx & 0xffff) | (ag.fy << 16), 0};
       sent[p] = make_int4(a, zl, ag.fx, ag.fy);
       sfz[p] = ag.fz;
-      zmin = min(zmin, zl);
-      zmax = max(zmax, zh);
     }
-    __syncthreads();
-    // exact hit heights of this thread's column for the n new entries
-    for (int i0 = 0; i0 < n; i0 += kPrepBatch) {
-      // branch-free batch: the table loads of kPrepBatch entries are in flight
-      // together (slots past n repeat the last entry and are not stored)
-      double hb[kPrepBatch];
-      bool inp[kPrepBatch];
-#pragma unroll
-      for (int t = 0; t < kPrepBatch; ++t) {
-        const int4 e = sent[min(i0 + t, n - 1)];
-        const int fdx = qx + e.z, fdy = qy + e.w;
-        const int r2 = fdx * fdx + fdy * fdy;
-        inp[t] = fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max;
-        // h(r2) = sqrt(R^2 - r2) computed here: IEEE sqrt of the same exact
-        // double is the host table's value bit for bit, without a 64-way
-        // scattered table load per entry
-        hb[t] = sqrt(R2 - (double)(inp[t] ? r2 : 0));
-      }
-#pragma unroll
-      for (int t = 0; t < kPrepBatch; ++t) {
-        const int it = min(i0 + t, n - 1);
-        const int z = (int)round_away(hb[t] - sfz[it]) + g.V / 2;
-        const uint8_t o = inp[t] ? (uint8_t)(z - sent[it].y) : (uint8_t)255;
-        if (i0 + t < n) zo[(size_t)(running + i0 + t) * 256] = o;  // uniform
-      }
-    }
-    running += n;
-    __syncthreads();  // sent/sfz/wsum are reused by the next scan
+    if (lane == 0) nown = __popcll(bal);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    zmin = min(zmin, __shfl_xor(zmin, o, 64));
-    zmax = max(zmax, __shfl_xor(zmax, o, 64));
-  }
-  if (lane == 0) { zr[0][wave] = zmin; zr[1][wave] = zmax; }
   __syncthreads();
-  if (tid == 0) {
-    const int lo = min(min(zr[0][0], zr[0][1]), min(zr[0][2], zr[0][3]));
-    const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
-    tinfo[tile] = make_int4(running, lo, hi, 0);
+  const int n = nown;
+  // exact hit heights of this thread's column for the chunk's entries
+  uint8_t* zo = zoff + ((size_t)tile * g.A + off) * 256 + tid;
+  for (int i0 = 0; i0 < n; i0 += kPrepBatch) {
+    // branch-free batch (slots past n repeat the last entry and are not stored)
+    double hb[kPrepBatch];
+    bool inp[kPrepBatch];
+#pragma unroll
+    for (int t = 0; t < kPrepBatch; ++t) {
+      const int4 e = sent[min(i0 + t, n - 1)];
+      const int fdx = qx + e.z, fdy = qy + e.w;
+      const int r2 = fdx * fdx + fdy * fdy;
+      inp[t] = fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max;
+      hb[t] = height(R2, inp[t] ? r2 : 0);
+    }
+#pragma unroll
+    for (int t = 0; t < kPrepBatch; ++t) {
+      const int it = min(i0 + t, n - 1);
+      const int z = (int)round_away(hb[t] - sfz[it]) + g.V / 2;
+      const uint8_t o = inp[t] ? (uint8_t)(z - sent[it].y) : (uint8_t)255;
+      if (i0 + t < n) zo[(size_t)(i0 + t) * 256] = o;  // uniform
+    }
   }
 }
 
@@ -394,6 +425,8 @@ hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, i
   return hipGetLastError();
 }
 
+static int prep_chunks(int A) { return std::max(1, (A + kTileChunk - 1) / kTileChunk); }
+
 hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          const int* posy, const AngleGeo* ageo, int A, int C,
                          int nxmax, int V, const double* htab,
@@ -412,7 +445,8 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
   if (prepass) {
     Timed tm("ewald_tiles", s);
-    hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
+    hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
+                       prep_chunks(A), list, tinfo, zoff);
   }
   // banded output: only the windows that can meet the band are launched
   // (the exact band is only known on the device)
@@ -444,7 +478,8 @@ hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, 
   uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
   int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
   Timed tm("ewald_tiles", s);
-  hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(256), 0, s, g, ntx, list, tinfo, zoff);
+  hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
+                     prep_chunks(A), list, tinfo, zoff);
   return hipGetLastError();
 }
 
