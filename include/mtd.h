@@ -97,6 +97,11 @@ const char* mtd_version(void);
  * (pretraitement_hors_axe_CPU/main.cpp:122-169,226-346;
  * Reconstruction/main.cpp:200-229,298-330). on=1 starts recording. */
 int mtd_ctx_set_timing(mtd_ctx* ctx, int on);
+/* Restricts the timing to the kernel of that report name (NULL or "": all
+ * kernels). Each timed launch adds a pair of event markers to the stream,
+ * which delays the next dispatch by a few microseconds; timing one kernel
+ * keeps the step close to its untimed duration. No reference counterpart. */
+int mtd_ctx_set_timing_only(mtd_ctx* ctx, const char* kernel);
 /* Synchronises, then writes "name launches total_ms\n" lines (one per kernel,
  * totals since the last reset) into buf. Returns the number of lines. */
 int mtd_ctx_timing_report(mtd_ctx* ctx, char* buf, size_t cap);

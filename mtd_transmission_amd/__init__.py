@@ -59,6 +59,7 @@ SIGNATURES = {
     "mtd_ctx_sync": (_I, [_P]),
     "mtd_last_error": (ctypes.c_char_p, [_P]),
     "mtd_ctx_set_timing": (_I, [_P, _I]),
+    "mtd_ctx_set_timing_only": (_I, [_P, ctypes.c_char_p]),
     "mtd_ctx_timing_report": (_I, [_P, ctypes.c_char_p, ctypes.c_size_t]),
     "mtd_ctx_timing_reset": (_I, [_P]),
     "mtd_config_extract_val": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _D,
@@ -233,7 +234,10 @@ class Context:
     def sync(self):
         self._check(lib().mtd_ctx_sync(self.h), "sync")
 
-    def set_timing(self, on=True):
+    def set_timing(self, on=True, only=None):
+        """Per-kernel event timing; only: time just the kernel of that report
+        name (the others then run without event markers between them)."""
+        lib().mtd_ctx_set_timing_only(self.h, (only or "").encode())
         lib().mtd_ctx_set_timing(self.h, int(bool(on)))
 
     def timing_reset(self):

@@ -75,6 +75,7 @@ struct mtd_ctx {
   mtd_geom plan_g{};
   // per-kernel event timing (mtd_ctx_set_timing)
   bool timing = false;
+  std::string timing_only;  // non-empty: only the kernel of that name is timed
   std::vector<mtd::EvPair> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, mtd::KTime> ktimes;
@@ -93,7 +94,8 @@ struct Timed {
   hipStream_t s;
   hipEvent_t a = nullptr;
   Timed(const char* n, hipStream_t st) : name(n), s(st) {
-    if (tl_ctx && tl_ctx->timing) {
+    if (tl_ctx && tl_ctx->timing &&
+        (tl_ctx->timing_only.empty() || tl_ctx->timing_only == n)) {
       c = tl_ctx;
       a = ev_take(c);
       if (a) (void)hipEventRecord(a, s);

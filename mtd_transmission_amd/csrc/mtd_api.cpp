@@ -629,6 +629,12 @@ int mtd_ctx_set_timing(mtd_ctx* ctx, int on) {
   return MTD_OK;
 }
 
+int mtd_ctx_set_timing_only(mtd_ctx* ctx, const char* kernel) {
+  if (!ctx) return MTD_ERR_ARG;
+  ctx->timing_only = kernel ? kernel : "";
+  return MTD_OK;
+}
+
 int mtd_ctx_timing_report(mtd_ctx* ctx, char* buf, size_t cap) {
   if (!ctx || !buf || cap == 0) return -1;
   timing_collect(ctx);
