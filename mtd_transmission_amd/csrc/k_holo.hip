@@ -53,7 +53,7 @@ constexpr int holo_rows_waves() {
 template <int N, typename TIn>
 __global__ void __launch_bounds__(256, holo_rows_waves<N>())
 k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
-            const float2* __restrict__ tw, int x0, int C, int W,
+            const float2* __restrict__ tw, int x0, int C, int lw,
             float2* __restrict__ t1) {
   using F = Fft2<N, false>;
   constexpr int NP = holo_pairs<N>();
@@ -86,9 +86,11 @@ k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
   // The values of the first pass stay in registers (crops up to N/2 wide)
   // so that each output is stored once, as a whole complex value.
   // Bin d of row y0 sits at q0[tile(d)], of row y1 at q1[tile(d)].
+  // W = 2^lw: shifts and masks, no integer division in the store loop
+  const int W = 1 << lw;
   float2* q0 = t1 + (size_t)a * N * C + (size_t)y0 * W;
   float2* q1 = q0 + W;
-  auto tile = [W](int d) { return (d / W) * (N * W) + d % W; };
+  auto tile = [lw, W](int d) { return (((d >> lw) * N) << lw) + (d & (W - 1)); };
   constexpr int DK = (N / 2 + F::T - 1) / F::T;
   float r0[DK], r1[DK];
   const bool keep = C <= N / 2;  // uniform
@@ -206,20 +208,22 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
   constexpr int NP = holo_pairs<N>();
   if (C % holo_cols_nc<N>()) return hipErrorInvalidValue;  // whole column tiles
   const int W = holo_tile_w(C, holo_cols_nc<N>());
+  int lw = 0;
+  while ((1 << lw) < W) ++lw;  // W is a power of two
   dim3 grid(N / (2 * NP), A), block(NP * Fft2<N, false>::T);
   Timed tm("holo_rows", s);
   switch (dtype) {
     case MTD_U8:
       hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
-                         (const uint8_t*)holos, win, tw, x0, C, W, t1);
+                         (const uint8_t*)holos, win, tw, x0, C, lw, t1);
       break;
     case MTD_U16:
       hipLaunchKernelGGL((k_holo_rows<N, uint16_t>), grid, block, 0, s,
-                         (const uint16_t*)holos, win, tw, x0, C, W, t1);
+                         (const uint16_t*)holos, win, tw, x0, C, lw, t1);
       break;
     default:
       hipLaunchKernelGGL((k_holo_rows<N, float>), grid, block, 0, s,
-                         (const float*)holos, win, tw, x0, C, W, t1);
+                         (const float*)holos, win, tw, x0, C, lw, t1);
       break;
   }
   return hipGetLastError();
