@@ -183,6 +183,36 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
                           int dtype, int A, const uint8_t* mask,
                           int32_t* posspec, float* index_re, float* index_im);
 
+/* ---- batched form ---------------------------------------------------------
+ * No reference counterpart: the reference reconstructs one acquisition per
+ * run of its two programs. n independent acquisitions, each with its own
+ * geometry (hologram side, crop, volume), angle count, hologram stack
+ * (device memory), host mask (or NULL), posspec (or NULL) and output
+ * volumes, reconstructed by the plain method in order on the context's
+ * stream; each result equals mtd_reconstruct_plain's on its own.
+ * MTD_BATCH_GRAPH: the first call runs eagerly, then captures the whole batch
+ * (every kernel of every acquisition) into one HIP graph; a later call with
+ * the same arguments (geometries, pointers, counts, dtypes, mask contents)
+ * replays it with a single launch, reading the stacks' current contents.
+ * Other arguments recapture, and so does any call after which the context
+ * moved a workspace or fit plan the graph uses (another entry point needed a
+ * larger buffer). MTD_BATCH_AUTOTUNE: at the first sight of a
+ * (hologram side, crop, dtype), the hologram-stage chunk (holograms per
+ * row/column launch pair) is picked by timing candidates on that
+ * acquisition; graphs then use the picked chunks. acq_ms (host, n floats, or
+ * NULL): each acquisition's device time in this call, from wall-clock stamps
+ * a one-thread kernel writes between acquisitions; the call then waits for
+ * the batch to finish. */
+enum mtd_batch_flags { MTD_BATCH_GRAPH = 1, MTD_BATCH_AUTOTUNE = 2 };
+int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms,
+                          const void* const* holos, const int* dtypes,
+                          const int* counts, const uint8_t* const* masks,
+                          int32_t* const* posspec, float* const* index_re,
+                          float* const* index_im, int flags, float* acq_ms);
+/* The hologram chunk the autotuner picked for (hologram side n, crop C,
+ * dtype); 0 if that shape was not tuned. */
+int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype);
+
 /* ---- Born forward model (Simul_holo) and its adjoint -------------------- */
 /* The simulator's hologram-spectrum synthesis (Simul_holo/main.cpp:106-109,
  * 162-171; calcHolo, Simul_holo/src/fonctions.cpp:109-140): TF3D =

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libmtd.so")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 MTD_U8, MTD_U16, MTD_F32 = 0, 1, 2
+MTD_BATCH_GRAPH, MTD_BATCH_AUTOTUNE = 1, 2
 
 
 class MTDError(RuntimeError):
@@ -76,6 +77,8 @@ SIGNATURES = {
     "mtd_ewald_accumulate": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_finalize": (_I, [_P, _GP, _P, _P, _P, _P, _P]),
     "mtd_reconstruct_plain": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P, _P]),
+    "mtd_reconstruct_batch": (_I, [_P, _I, _GP, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "mtd_batch_tuned_chunk": (_I, [_P, _I, _I, _I]),
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
@@ -306,6 +309,36 @@ class Context:
                                                 _dtype_code(holos), holos.shape[0], mp,
                                                 _ptr(posspec), _ptr(index_re),
                                                 _ptr(index_im)), "reconstruct_plain")
+
+    def reconstruct_batch(self, geoms, holos, masks, index_re, index_im, posspecs=None,
+                          graph=True, autotune=False, times=False):
+        """Batched form: acquisition i = (geoms[i], holos[i] (GPU tensor),
+        masks[i] (host C x C or None), index_re[i], index_im[i], posspecs[i]
+        or None), reconstructed in order; graph: captured on the first call
+        and replayed on later calls with the same arguments. times: returns
+        each acquisition's device milliseconds (the call then waits)."""
+        import numpy as np
+        self._stream()
+        n = len(geoms)
+        G = (Geom * n)(*geoms)
+        VP = ctypes.c_void_p * n
+        keep = [_mask_ptr(masks[i] if masks else None, geoms[i].C) for i in range(n)]
+        hp = VP(*[_ptr_holos(h) for h in holos])
+        dt = (ctypes.c_int * n)(*[_dtype_code(h) for h in holos])
+        ct = (ctypes.c_int * n)(*[h.shape[0] for h in holos])
+        mp = VP(*[k[1] for k in keep])
+        pp = VP(*[_ptr(p) for p in posspecs]) if posspecs else None
+        rp = VP(*[_ptr(t) for t in index_re])
+        ip = VP(*[_ptr(t) for t in index_im])
+        ms = np.zeros(n, dtype=np.float32) if times else None
+        flags = (MTD_BATCH_GRAPH if graph else 0) | (MTD_BATCH_AUTOTUNE if autotune else 0)
+        self._check(lib().mtd_reconstruct_batch(
+            self.h, n, G, hp, dt, ct, mp, pp, rp, ip, flags,
+            None if ms is None else ms.ctypes.data_as(ctypes.c_void_p)), "reconstruct_batch")
+        return ms
+
+    def batch_tuned_chunk(self, n, C, dtype=MTD_U16):
+        return lib().mtd_batch_tuned_chunk(self.h, n, C, dtype)
 
     # ---- Born forward model and adjoint ---------------------------------
     def spectra_to_fields(self, g: Geom, spectra, posspec, fields):

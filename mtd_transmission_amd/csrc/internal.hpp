@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <list>
 #include <map>
 #include <string>
 #include <tuple>
@@ -67,7 +68,22 @@ struct mtd_ctx {
   std::map<std::tuple<uint64_t, int, uint32_t, int>, float*> beta2tab;  // + NXMAX
   std::map<std::tuple<uint64_t, int, uint64_t, uint64_t>, float*> qtab;  // (R, r2max, lambda, n0)
   std::map<int, mtd::Buffer> ws;                          // scratch slots
-  mtd::LsqPlan lsq;
+  // least-squares plans per (crop side, mask); lsq points at the current one
+  std::list<mtd::LsqPlan> lsq_plans;
+  mtd::LsqPlan* lsq = nullptr;
+  // batched form (mtd_reconstruct_batch): the captured graph of the last
+  // batch, the arguments it was captured for, its capture stream, the events
+  // between its acquisitions, and the hologram chunk picked per shape
+  hipStream_t graph_stream = nullptr;
+  hipGraph_t batch_graph = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  std::vector<uint8_t> batch_key;
+  // bumped whenever a workspace or fit plan is freed or moved: a graph
+  // captured at another generation holds stale addresses and is recaptured
+  uint64_t ws_gen = 0;
+  uint64_t batch_gen = 0;
+  std::map<std::tuple<int, int, int>, int> holo_chunk_tuned;  // (N, C, dtype) -> holograms
+  int holo_chunk_override = 0;  // > 0: holograms per hologram-stage chunk
   // geometry plan of the forward model / adjoint (mtd_born_plan): speculars,
   // angle geometry, hit band and the gather's tile lists, in WS_PLAN* slots
   bool plan_valid = false;
@@ -114,7 +130,7 @@ struct Timed {
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
   WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART,
-  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER, WS_HSTAGE
+  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER, WS_HSTAGE, WS_BATCHTS
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -210,6 +226,7 @@ hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, flo
                                    float scale, const int2* band, const float2* tw,
                                    hipStream_t s);
 // One fused FISTA step with the smoothed TV gradient (k_tv.hip).
+hipError_t launch_stamp(unsigned long long* out, hipStream_t s);
 hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
                            float sd, float lam, float eps, float step, float beta,
                            int positivity, hipStream_t s);

@@ -70,6 +70,7 @@ void* ws_get(mtd_ctx* ctx, int slot, size_t bytes) {
     (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
+    ++ctx->ws_gen;
   }
   if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
     b.ptr = nullptr;
@@ -166,11 +167,16 @@ const float* tukey_signed_table(mtd_ctx* ctx, int n, double alpha) {
 // ---------------------------------------------------------------------------
 // least-squares plan (Correction_aberration.cpp:86-149,186-217)
 // ---------------------------------------------------------------------------
+static const size_t kLsqPlans = 64;
+
 int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
   std::vector<uint8_t> m(C * C, 255);
   if (mask) std::memcpy(m.data(), mask, C * C);
-  LsqPlan& p = ctx->lsq;
-  if (p.d_samples && p.C == C && p.mask == m) return MTD_OK;
+  for (LsqPlan& q : ctx->lsq_plans)
+    if (q.C == C && q.mask == m) {
+      ctx->lsq = &q;
+      return MTD_OK;
+    }
   std::vector<int2> s;
   for (int y = 0; y < C; y += 5)
     for (int x = 0; x < C; x += 5)
@@ -207,11 +213,20 @@ int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
       for (int j = k + 1; j < 15; ++j) G[i][j] -= G[i][k] * G[k][j];
     }
   }
-  // the plan is about to change: no queued fit may still read the old one
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(p.d_samples);
-  (void)hipFree(p.d_lu);
-  (void)hipFree(p.d_perm);
+  // a new plan; beyond kLsqPlans the oldest is freed, after the queued fits
+  // that may still read it
+  if (ctx->lsq_plans.size() >= kLsqPlans) {
+    (void)hipStreamSynchronize(ctx->stream);
+    LsqPlan& o = ctx->lsq_plans.front();
+    (void)hipFree(o.d_samples);
+    (void)hipFree(o.d_lu);
+    (void)hipFree(o.d_perm);
+    ctx->lsq_plans.pop_front();
+    ++ctx->ws_gen;
+  }
+  ctx->lsq_plans.emplace_back();
+  LsqPlan& p = ctx->lsq_plans.back();
+  ctx->lsq = nullptr;
   p.d_samples = upload(s.data(), s.size());
   p.d_lu = upload(&G[0][0], 225);
   // the sequential swaps as one permutation: (Pb)[i] = b[perm[i]]
@@ -225,6 +240,7 @@ int lsq_prepare(mtd_ctx* ctx, int C, const uint8_t* mask) {
   p.M = M;
   p.C = C;
   p.mask = m;
+  ctx->lsq = &p;
   return MTD_OK;
 }
 
@@ -440,7 +456,7 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
                       "phase unwrapping"));
   }
   if (g->c_aber)
-    MTD_TRY(hip_check(ctx, launch_lsq(C, iu, phu, n, ctx->lsq, coef, ctx->stream), "aberration fit"));
+    MTD_TRY(hip_check(ctx, launch_lsq(C, iu, phu, n, *ctx->lsq, coef, ctx->stream), "aberration fit"));
   if (out_fields)
     MTD_TRY(hip_check(ctx, launch_form_field(C, iu, crop, posx, posy, n, coef, mode, phu, out_fields,
                                              ctx->stream), "field formation"));
@@ -510,7 +526,9 @@ static int copy_engine(mtd_ctx* ctx) {
 static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
                       int n, float2* crop, float2* part, int S) {
   const int N = g->n, C = 2 * g->nxmax;
-  const int hc = chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
+  const int hc = ctx->holo_chunk_override > 0
+                     ? std::min(ctx->holo_chunk_override, std::max(n, 1))
+                     : chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
   auto part_at = [&](int h0) { return part ? part + (size_t)h0 * S : nullptr; };
   if (!is_host_ptr(holos)) {
     for (int h0 = 0; h0 < n; h0 += hc)
@@ -600,9 +618,14 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
     }
     (void)hipStreamDestroy(ctx->copy_stream);
   }
-  (void)hipFree(ctx->lsq.d_samples);
-  (void)hipFree(ctx->lsq.d_lu);
-  (void)hipFree(ctx->lsq.d_perm);
+  for (LsqPlan& p : ctx->lsq_plans) {
+    (void)hipFree(p.d_samples);
+    (void)hipFree(p.d_lu);
+    (void)hipFree(p.d_perm);
+  }
+  if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+  if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+  if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);
   delete ctx;
   return MTD_OK;
 }
@@ -1151,6 +1174,200 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   // rest as the zeros it is and never reads it.
   MTD_TRY(scatter_impl(ctx, g, F, 0, pos, A, (float*)vol, nullptr, 1 | 2, band));
   return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, index_re, index_im, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: independent acquisitions of their own shapes in one graph
+// ---------------------------------------------------------------------------
+static void key_put(std::vector<uint8_t>& k, const void* p, size_t n) {
+  k.insert(k.end(), (const uint8_t*)p, (const uint8_t*)p + n);
+}
+
+static uint64_t fnv1a(const uint8_t* p, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  return h;
+}
+
+// one acquisition with the hologram chunk picked for its shape
+static int batch_one(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
+                     const uint8_t* mask, int32_t* pos, float* re, float* im) {
+  const auto it = ctx->holo_chunk_tuned.find(std::make_tuple(g->n, 2 * g->nxmax, dtype));
+  ctx->holo_chunk_override = it == ctx->holo_chunk_tuned.end() ? 0 : it->second;
+  const int rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
+  ctx->holo_chunk_override = 0;
+  return rc;
+}
+
+// Times the plain reconstruction of one acquisition with each candidate
+// hologram chunk (holograms per row/column launch pair; the launch count
+// against the tail of each launch) and keeps the fastest for its (N, C,
+// dtype). Each candidate runs twice, the second run timed.
+static int tune_holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
+                           const uint8_t* mask, int32_t* pos, float* re, float* im) {
+  const auto key = std::make_tuple(g->n, 2 * g->nxmax, dtype);
+  if (ctx->holo_chunk_tuned.count(key) || A <= 0) return MTD_OK;
+  const int cap = (int)std::max<size_t>(
+      1, (size_t)2 * kChunkBudget / ((size_t)g->n * 2 * g->nxmax * sizeof(float2)));
+  std::vector<int> cand;
+  for (int c : {16, 32, 64, 128, 256, 512})
+    if (c < A && c <= 4 * cap) cand.push_back(c);
+  cand.push_back(std::min(A, 4 * cap));
+  hipEvent_t e0, e1;
+  MTD_TRY(hip_check(ctx, hipEventCreate(&e0), "tuner event"));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_err(ctx, MTD_ERR_HIP, "tuner event");
+  }
+  int best = 0;
+  float best_ms = 0.f;
+  int rc = MTD_OK;
+  for (int c : cand) {
+    ctx->holo_chunk_override = c;
+    rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
+    if (rc == MTD_OK && (rc = hip_check(ctx, hipEventRecord(e0, ctx->stream), "tuner")) == MTD_OK)
+      rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
+    if (rc == MTD_OK) rc = hip_check(ctx, hipEventRecord(e1, ctx->stream), "tuner");
+    if (rc == MTD_OK) rc = hip_check(ctx, hipEventSynchronize(e1), "tuner");
+    float ms = 0.f;
+    if (rc == MTD_OK) rc = hip_check(ctx, hipEventElapsedTime(&ms, e0, e1), "tuner");
+    if (rc != MTD_OK) break;
+    if (!best || ms < best_ms) { best = c; best_ms = ms; }
+  }
+  ctx->holo_chunk_override = 0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == MTD_OK) ctx->holo_chunk_tuned[key] = best;
+  return rc;
+}
+
+// per-acquisition times: wall-clock stamps written by a one-thread kernel
+// before the first and after every acquisition (WS_BATCHTS, n + 1 stamps)
+static unsigned long long* batch_stamps(mtd_ctx* ctx, int n) {
+  return (unsigned long long*)ws_get(ctx, WS_BATCHTS, (size_t)(n + 1) * 8);
+}
+
+static int batch_stamp(mtd_ctx* ctx, int i) {
+  return hip_check(ctx, launch_stamp(batch_stamps(ctx, i) + i, ctx->stream), "batch stamp");
+}
+
+static int batch_times(mtd_ctx* ctx, int n, float* acq_ms) {
+  std::vector<unsigned long long> t(n + 1);
+  MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch wait"));
+  MTD_TRY(hip_check(ctx, hipMemcpy(t.data(), batch_stamps(ctx, n), t.size() * 8,
+                                   hipMemcpyDeviceToHost), "batch stamps"));
+  int khz = 0;
+  MTD_TRY(hip_check(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                                               ctx->device), "wall clock rate"));
+  for (int i = 0; i < n; ++i) acq_ms[i] = (float)((double)(t[i + 1] - t[i]) / khz);
+  return MTD_OK;
+}
+
+int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void* const* holos,
+                          const int* dtypes, const int* counts, const uint8_t* const* masks,
+                          int32_t* const* posspec, float* const* index_re,
+                          float* const* index_im, int flags, float* acq_ms) {
+  CtxScope scope(ctx);
+  if (!ctx) return MTD_ERR_ARG;
+  if (n < 0 || (n > 0 && (!geoms || !holos || !dtypes || !counts || !index_re || !index_im)))
+    return set_err(ctx, MTD_ERR_ARG, "null batch array");
+  if (n == 0) return MTD_OK;
+  const bool graph = flags & MTD_BATCH_GRAPH;
+  // the arguments a captured graph holds: geometry bytes, every pointer,
+  // counts, dtypes, mask contents and the tuned chunks
+  std::vector<uint8_t> key;
+  key_put(key, &n, sizeof n);
+  for (int i = 0; i < n; ++i) {
+    const mtd_geom& g = geoms[i];
+    MTD_TRY(validate(ctx, &g, counts[i], NEED_HOLO | NEED_VOL));
+    if (counts[i] > 0 && !holos[i]) return set_err(ctx, MTD_ERR_ARG, "null hologram stack");
+    if (!index_re[i] || !index_im[i]) return set_err(ctx, MTD_ERR_ARG, "null output volume");
+    if (graph && counts[i] > 0 && is_host_ptr(holos[i]))
+      return set_err(ctx, MTD_ERR_ARG,
+                     "batched graph form: hologram stacks must be in device memory");
+    key_put(key, &g, sizeof g);
+    key_put(key, &holos[i], sizeof(void*));
+    key_put(key, &dtypes[i], sizeof(int));
+    key_put(key, &counts[i], sizeof(int));
+    const uint8_t* m = masks ? masks[i] : nullptr;
+    const uint64_t mh = m ? fnv1a(m, (size_t)4 * g.nxmax * g.nxmax) : 0;
+    key_put(key, &mh, sizeof mh);
+    int32_t* p = posspec ? posspec[i] : nullptr;
+    key_put(key, &p, sizeof p);
+    key_put(key, &index_re[i], sizeof(float*));
+    key_put(key, &index_im[i], sizeof(float*));
+  }
+  // stamps sized before any capture (no allocation inside one)
+  if (!batch_stamps(ctx, n)) return set_err(ctx, MTD_ERR_HIP, "out of device memory (stamps)");
+  auto mask_of = [&](int i) { return masks ? masks[i] : nullptr; };
+  auto pos_of = [&](int i) { return posspec ? posspec[i] : nullptr; };
+  if (graph && ctx->batch_exec && key == ctx->batch_key && ctx->batch_gen == ctx->ws_gen) {
+    MTD_TRY(hip_check(ctx, hipGraphLaunch(ctx->batch_exec, ctx->stream), "batch graph launch"));
+    return acq_ms ? batch_times(ctx, n, acq_ms) : MTD_OK;
+  }
+  if (flags & MTD_BATCH_AUTOTUNE)
+    for (int i = 0; i < n; ++i)
+      MTD_TRY(tune_holo_chunk(ctx, &geoms[i], holos[i], dtypes[i], counts[i], mask_of(i),
+                              pos_of(i), index_re[i], index_im[i]));
+  // eager pass: the results of this call, and every table, plan and
+  // workspace the graph will use is in place before the capture
+  MTD_TRY(batch_stamp(ctx, 0));
+  for (int i = 0; i < n; ++i) {
+    MTD_TRY(batch_one(ctx, &geoms[i], holos[i], dtypes[i], counts[i], mask_of(i), pos_of(i),
+                      index_re[i], index_im[i]));
+    MTD_TRY(batch_stamp(ctx, i + 1));
+  }
+  if (acq_ms) MTD_TRY(batch_times(ctx, n, acq_ms));
+  if (!graph) return MTD_OK;
+  // capture on a private stream (a graph cannot be captured from the legacy
+  // default stream); launches then go to the context's stream
+  if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+  if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+  ctx->batch_exec = nullptr;
+  ctx->batch_graph = nullptr;
+  ctx->batch_key.clear();
+  if (!ctx->graph_stream)
+    MTD_TRY(hip_check(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking),
+                      "graph stream"));
+  MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch sync"));
+  MTD_TRY(hip_check(ctx, hipStreamBeginCapture(ctx->graph_stream, hipStreamCaptureModeThreadLocal),
+                    "batch capture"));
+  const hipStream_t saved = ctx->stream;
+  const bool saved_timing = ctx->timing;
+  ctx->stream = ctx->graph_stream;
+  ctx->timing = false;
+  int rc = batch_stamp(ctx, 0);
+  for (int i = 0; i < n && rc == MTD_OK; ++i) {
+    rc = batch_one(ctx, &geoms[i], holos[i], dtypes[i], counts[i], mask_of(i), pos_of(i),
+                   index_re[i], index_im[i]);
+    if (rc == MTD_OK) rc = batch_stamp(ctx, i + 1);
+  }
+  ctx->stream = saved;
+  ctx->timing = saved_timing;
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->graph_stream, &gr);
+  if (rc != MTD_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  MTD_TRY(hip_check(ctx, ec, "batch capture"));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    return hip_check(ctx, ei, "batch graph instantiate");
+  }
+  ctx->batch_graph = gr;
+  ctx->batch_exec = ex;
+  ctx->batch_key = std::move(key);
+  ctx->batch_gen = ctx->ws_gen;
+  return MTD_OK;
+}
+
+int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype) {
+  if (!ctx) return -1;
+  const auto it = ctx->holo_chunk_tuned.find(std::make_tuple(n, C, dtype));
+  return it == ctx->holo_chunk_tuned.end() ? 0 : it->second;
 }
 
 }  // extern "C"
