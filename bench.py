@@ -242,6 +242,103 @@ def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     return {"bytes_per_copy": 2 * nbytes, "GBps": round(2 * nbytes / dt / 1e9, 1)}
 
 
+def batch_shapes(seed=2027, count=32):
+    """The batched workload: per acquisition an angle count drawn from
+    50-800, a hologram side from {512, 768, 1024} and a crop side from
+    128-384 in steps of 32, V = 2C; crops are drawn among those whose window
+    (carrier +- NXMAX) stays inside the hologram, NXMAX <= 0.2 N."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        A = int(rng.integers(50, 801))
+        n = int(rng.choice([512, 768, 1024]))
+        crops = [c for c in range(128, 385, 32) if c // 2 <= 0.2 * n]
+        C = int(rng.choice(crops))
+        out.append(dict(n=n, nxmax=C // 2, v=2 * C, A=A))
+    return out
+
+
+def run_batched(args):
+    """Batched form: 32 acquisitions of uneven shapes through
+    mtd_reconstruct_batch, eagerly and as one replayed HIP graph (the timed
+    steps); per-acquisition latency p50/p99 from the library's wall-clock
+    stamps over the timed replays."""
+    import numpy as np
+    import torch
+
+    import mtd_transmission_amd as m
+    from mtd_transmission_amd import synth
+
+    shapes = batch_shapes()
+    geoms, holos, masks = [], [], []
+    t0 = time.perf_counter()
+    for i, s in enumerate(shapes):
+        w = dict(s, seed=3000 + i)
+        n, nx, V, A, pix, cx, cy = geometry(w)
+        ph = synth.make_phantom(w["seed"], 2)
+        kx, ky = synth.scan_fermat(A, nx, 0.95)
+        h, mask = synth.simulate_torch(n, nx, cx, cy, pix, LAM, N0, kx, ky, ph,
+                                       seed=w["seed"] + 1, lowpass=False)
+        geoms.append(m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix))
+        holos.append(h)
+        masks.append(mask)
+    torch.cuda.synchronize()
+    total_A = sum(s["A"] for s in shapes)
+    print(f"[bench] synthesized {len(shapes)} acquisitions, {total_A} holograms in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    re = [torch.empty((g.dim_final,) * 3, device="cuda") for g in geoms]
+    im = [torch.empty_like(r) for r in re]
+    pos = [torch.empty((2 * h.shape[0],), dtype=torch.int32, device="cuda") for h in holos]
+    ctx = m.Context(0)
+    # eager batch (no graph) for comparison, after a warm-up
+    ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # capture, then timed replays
+    ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
+    for _ in range(args.warmup):
+        ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    print(f"[bench] {args.steps} graph replays, {ms:.3f} ms per batch (eager {eager_ms:.3f})",
+          file=sys.stderr, flush=True)
+    lat = []
+    for _ in range(args.steps):  # per-acquisition stamps (each call waits)
+        lat.extend(ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True,
+                                         autotune=True, times=True).tolist())
+    lat = np.asarray(lat)
+    per_acq = [dict(s, ms=round(float(t), 4)) for s, t in zip(shapes, lat[-len(shapes):])]
+    chunks = {f"{g.n}x{g.C}": ctx.batch_tuned_chunk(g.n, g.C) for g in geoms}
+    out = {
+        "metric": METRIC, "value": round(total_A / (ms / 1e3), 2), "unit": "holograms/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched form: {len(shapes)} acquisitions, angles 50-800, "
+                               "holograms {512,768,1024}^2 uint16, crops 128-384 (V = 2C), "
+                               "one captured HIP graph per step",
+                   "acquisitions": len(shapes), "holograms": total_A},
+        "batched": {"p50_ms": round(float(np.percentile(lat, 50)), 4),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                    "eager_ms_per_batch": round(eager_ms, 4),
+                    "graph_ms_per_batch": round(ms, 4),
+                    "tuned_hologram_chunks": chunks,
+                    "acquisitions": per_acq},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +366,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        run_batched(args)
+        return
 
     import numpy as np
     import torch
