@@ -250,7 +250,9 @@ int mtd_born_plan(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec, int A
  * gradient of the adjoint-iterative reconstruction): out = A^H spectra, the
  * sum over angles of i q S on each cap voxel (ascending angle order, as
  * retroPropag_Born) followed by the centred, unnormalised backward 3-D FFT
- * times Tp^3. out_re/out_im: float V^3. */
+ * times Tp^3. out_re/out_im: float V^3; out_im may be NULL (the imaginary
+ * part is then not stored, as the gradient of a real unknown needs only the
+ * real part). */
 int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                         const int32_t* posspec, int A, float* out_re,
                         float* out_im);
@@ -273,6 +275,23 @@ int mtd_cap_weights(mtd_ctx* ctx, const mtd_geom* g, const int32_t* posspec,
 int mtd_tv_fista_step(mtd_ctx* ctx, int V, const float* z, const float* grad_data,
                       float* x, float* z_next, float data_scale, float lam,
                       float eps, float step, float beta, int positivity);
+
+/* The whole adjoint-iterative solver loop in the library (no counterpart in
+ * the reference): FISTA on 1/2 inv_kappa ||A x - spectra||_W^2 + lam
+ * TV_eps(x), x >= 0 when positivity, n_iter iterations from x (clamped to
+ * >= 0 first), on the geometry plan of mtd_born_plan (same geometry and A).
+ * Each iteration: mtd_forward_spectra, the weighted residual W (S - y) in
+ * place (weights: float [A][C][C] of mtd_cap_weights, or NULL for W = 1),
+ * the real part of mtd_adjoint_spectra, and mtd_tv_fista_step with step
+ * 1/(1 + 12 lam/eps) and the momentum t_{k+1} = (1 + sqrt(1 + 4 t_k^2))/2,
+ * beta = (t_k - 1)/t_{k+1}, t_1 = 1. x: float V^3, in/out.
+ * MTD_TV_GRAPH: the n_iter iterations are captured into one HIP graph (its
+ * momentum coefficients baked in) and launched once. The iterates equal
+ * the step-by-step calls bit for bit. */
+enum mtd_tv_flags { MTD_TV_GRAPH = 1 };
+int mtd_tv_solve(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                 const float* weights, int A, int n_iter, float lam, float eps,
+                 float inv_kappa, int positivity, float* x, int flags);
 
 #ifdef __cplusplus
 }

@@ -86,6 +86,8 @@ SIGNATURES = {
     "mtd_spectra_to_fields": (_I, [_P, _GP, _P, _P, _I, _P]),
     "mtd_tv_fista_step": (_I, [_P, _I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
                                ctypes.c_float, ctypes.c_float, ctypes.c_float, _I]),
+    "mtd_tv_solve": (_I, [_P, _GP, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, _I, _P, _I]),
 }
 
 
@@ -393,3 +395,14 @@ class Context:
         self._check(lib().mtd_tv_fista_step(self.h, V, _ptr(z), _ptr(grad_data), _ptr(x),
                                             _ptr(z_next), data_scale, lam, eps, step, beta,
                                             int(bool(positivity))), "tv_fista_step")
+
+    def tv_solve(self, g: Geom, spectra, weights, x, n_iter, lam, eps, inv_kappa,
+                 positivity=True, graph=False):
+        """n_iter FISTA/TV iterations in the library on the geometry plan
+        (born_plan first); x (float V^3) is the start and the result.
+        graph: the iterations are captured into one HIP graph, launched once."""
+        self._stream()
+        self._check(lib().mtd_tv_solve(self.h, ctypes.byref(g), _ptr(spectra), _ptr(weights),
+                                       spectra.shape[0], int(n_iter), lam, eps, inv_kappa,
+                                       int(bool(positivity)), _ptr(x), 1 if graph else 0),
+                    "tv_solve")

@@ -28,6 +28,7 @@ import math
 import time
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 
@@ -111,7 +112,8 @@ class TVResult:
 
 
 def reconstruct_tv(mtd, g, posspec, y, n_iter=100, lam=1e-3, eps=1e-3, dn0=None,
-                   positivity=True, kappa=None, weights=None, record=False, monitor=None):
+                   positivity=True, kappa=None, weights=None, record=False, monitor=None,
+                   fused=False, graph=False):
     """FISTA on 1/2 ||A dn - y||_W^2 / kappa + lam TV_eps(dn), dn >= 0.
 
     y: complex64 [A][C][C] measured spectra (field_to_spectrum frame).
@@ -124,7 +126,10 @@ def reconstruct_tv(mtd, g, posspec, y, n_iter=100, lam=1e-3, eps=1e-3, dn0=None,
     directly), then one fused library step (TV gradient, gradient step,
     projection, momentum). record: the objective at
     each iterate z. monitor(k, x): called after iteration k (1-based) with
-    the current estimate. Returns a TVResult."""
+    the current estimate. fused: the same iterations run by the library
+    (mtd_tv_solve: no autograd, identical iterates); graph: and captured as
+    one HIP graph. record and monitor need the step-by-step loop. Returns a
+    TVResult."""
     V = g.dim_final
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -133,8 +138,17 @@ def reconstruct_tv(mtd, g, posspec, y, n_iter=100, lam=1e-3, eps=1e-3, dn0=None,
     # one geometry plan for all the forward / adjoint calls of the loop
     A = posspec.numel() // 2
     mtd.born_plan(g, posspec)
+    # lam and eps as the library receives them (float32), so that the step
+    # is the one mtd_tv_solve derives from them
+    lam, eps = float(np.float32(lam)), float(np.float32(eps))
     step = 1.0 / (1.0 + 12.0 * lam / eps)
     x = torch.zeros((V, V, V), device="cuda") if dn0 is None else dn0.detach().clone()
+    if fused:
+        if record or monitor is not None:
+            raise ValueError("record and monitor need the step-by-step loop (fused=False)")
+        mtd.tv_solve(g, y, weights, x, n_iter, lam, eps, 1.0 / kappa, positivity, graph)
+        torch.cuda.synchronize()
+        return TVResult(x, [], kappa, n_iter, time.perf_counter() - t0)
     if positivity:
         x.clamp_(min=0.0)
     z = x.clone()

@@ -77,6 +77,9 @@ struct mtd_ctx {
   hipStream_t graph_stream = nullptr;
   hipGraph_t batch_graph = nullptr;
   hipGraphExec_t batch_exec = nullptr;
+  // the last graph-captured solver loop (mtd_tv_solve), freed at the next one
+  hipGraph_t tv_graph = nullptr;
+  hipGraphExec_t tv_exec = nullptr;
   std::vector<uint8_t> batch_key;
   // bumped whenever a workspace or fit plan is freed or moved: a graph
   // captured at another generation holds stale addresses and is recaptured
@@ -130,7 +133,8 @@ struct Timed {
 enum WsSlot {
   WS_T1 = 0, WS_CROP, WS_WORK, WS_IU, WS_POS, WS_COEF, WS_AGEO, WS_F,
   WS_VOL, WS_CNT, WS_VWORK, WS_HOLOTMP, WS_GATHER, WS_ARGMAX, WS_BAND, WS_PHU, WS_UNWRAP, WS_HOLOPART,
-  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER, WS_HSTAGE, WS_BATCHTS
+  WS_PLANPOS, WS_PLANAGEO, WS_PLANBAND, WS_PLANGATHER, WS_HSTAGE, WS_BATCHTS,
+  WS_TVZ, WS_TVZN, WS_TVGD, WS_TVS
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
@@ -227,6 +231,8 @@ hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, flo
                                    hipStream_t s);
 // One fused FISTA step with the smoothed TV gradient (k_tv.hip).
 hipError_t launch_stamp(unsigned long long* out, hipStream_t s);
+hipError_t launch_residual(float2* S, const float2* y, const float* w, size_t n, hipStream_t s);
+hipError_t launch_tv_start(float* x, float* z, size_t n, int positivity, hipStream_t s);
 hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
                            float sd, float lam, float eps, float step, float beta,
                            int positivity, hipStream_t s);

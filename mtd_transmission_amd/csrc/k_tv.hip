@@ -82,6 +82,47 @@ k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __r
   zn[i] = xn + beta * (xn - xo);
 }
 
+// Data-term gradient of the weighted least squares on the spectra, in place:
+// S = (S - y) * w (w NULL: 1), the product taken per component as a complex
+// times a real weight.
+__global__ void __launch_bounds__(256)
+k_residual(float2* __restrict__ S, const float2* __restrict__ y, const float* __restrict__ w,
+           size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float2 s = S[i], m = y[i];
+  float2 r = make_float2(s.x - m.x, s.y - m.y);
+  if (w) {
+    const float k = w[i];
+    r = make_float2(r.x * k, r.y * k);
+  }
+  S[i] = r;
+}
+
+// Solver start: x = max(x, 0) when positivity holds, z = x.
+__global__ void __launch_bounds__(256)
+k_tv_start(float* __restrict__ x, float* __restrict__ z, size_t n, int positivity) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float v = x[i];
+  if (positivity) v = fmaxf(v, 0.f);
+  x[i] = v;
+  z[i] = v;
+}
+
+hipError_t launch_residual(float2* S, const float2* y, const float* w, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  Timed tm("residual", s);
+  hipLaunchKernelGGL(k_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, y, w, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tv_start(float* x, float* z, size_t n, int positivity, hipStream_t s) {
+  hipLaunchKernelGGL(k_tv_start, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, z, n,
+                     positivity);
+  return hipGetLastError();
+}
+
 hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
                            float sd, float lam, float eps, float step, float beta,
                            int positivity, hipStream_t s) {

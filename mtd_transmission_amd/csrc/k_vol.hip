@@ -569,7 +569,7 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
     const float sg = ((x + y + z) & 1) ? -gsign : gsign;
     const size_t i = ((size_t)z * V + y) * V + x;
     re[i] = (v.x * sg * df3) * cte;
-    im[i] = (v.y * sg * df3) * cte;
+    if (im) im[i] = (v.y * sg * df3) * cte;  // uniform: NULL skips the imaginary part
   }
 };
 

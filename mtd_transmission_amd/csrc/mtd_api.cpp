@@ -625,6 +625,8 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   }
   if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
   if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+  if (ctx->tv_exec) (void)hipGraphExecDestroy(ctx->tv_exec);
+  if (ctx->tv_graph) (void)hipGraphDestroy(ctx->tv_graph);
   if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);
   delete ctx;
   return MTD_OK;
@@ -1046,7 +1048,7 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                         const int32_t* posspec, int A, float* out_re, float* out_im) {
   CtxScope scope(ctx);
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
-  if (!out_re || !out_im || (A > 0 && !spectra)) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (!out_re || (A > 0 && !spectra)) return set_err(ctx, MTD_ERR_ARG, "null array");
   if (A > 0 && !posspec && !ctx->plan_valid)
     return set_err(ctx, MTD_ERR_ARG, "null array (posspec, and no geometry plan)");
   const int V = g->dim_final;
@@ -1139,6 +1141,94 @@ int mtd_tv_fista_step(mtd_ctx* ctx, int V, const float* z, const float* grad_dat
   return hip_check(ctx, launch_tv_fista(V, z, grad_data, x, z_next, data_scale, lam, eps, step,
                                         beta, positivity, ctx->stream),
                    "TV step");
+}
+
+// The solver loop of the adjoint-iterative form: n_iter iterations of
+// forward, weighted residual, adjoint (real part) and the fused TV/FISTA
+// step, on the geometry plan. The momentum coefficients are the host's
+// t-sequence, so a captured loop bakes them in as kernel arguments.
+static int tv_loop(mtd_ctx* ctx, const mtd_geom* g, const float* y, const float* w, int A,
+                   int n_iter, float lam, float eps, float inv_kappa, int positivity, float* x,
+                   float* z, float* zn, float* gd, float* S) {
+  const int V = g->dim_final, C = 2 * g->nxmax;
+  const size_t nv = (size_t)V * V * V, ns = (size_t)A * C * C;
+  const float step = (float)(1.0 / (1.0 + 12.0 * (double)lam / (double)eps));
+  MTD_TRY(hip_check(ctx, launch_tv_start(x, z, nv, positivity, ctx->stream), "solver start"));
+  double t = 1.0;
+  for (int k = 1; k <= n_iter; ++k) {
+    MTD_TRY(mtd_forward_spectra(ctx, g, z, nullptr, nullptr, A, S));
+    MTD_TRY(hip_check(ctx, launch_residual((float2*)S, (const float2*)y, w, ns, ctx->stream),
+                      "residual"));
+    MTD_TRY(mtd_adjoint_spectra(ctx, g, S, nullptr, A, gd, nullptr));
+    const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+    MTD_TRY(mtd_tv_fista_step(ctx, V, z, gd, x, zn, inv_kappa, lam, eps, step,
+                              (float)((t - 1.0) / tn), positivity));
+    std::swap(z, zn);
+    t = tn;
+  }
+  return MTD_OK;
+}
+
+int mtd_tv_solve(mtd_ctx* ctx, const mtd_geom* g, const float* spectra, const float* weights,
+                 int A, int n_iter, float lam, float eps, float inv_kappa, int positivity,
+                 float* x, int flags) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, NEED_VOL));
+  if (!x || (A > 0 && !spectra)) return set_err(ctx, MTD_ERR_ARG, "null array");
+  if (n_iter < 0) return set_err(ctx, MTD_ERR_ARG, "negative iteration count");
+  if (!(eps > 0.f)) return set_err(ctx, MTD_ERR_ARG, "eps must be > 0");
+  if (A > 0 && !(ctx->plan_valid && ctx->plan_A == A &&
+                 std::memcmp(&ctx->plan_g, g, sizeof *g) == 0))
+    return set_err(ctx, MTD_ERR_ARG, "no geometry plan for this geometry and angle count "
+                                     "(mtd_born_plan first)");
+  const int V = g->dim_final, C = 2 * g->nxmax;
+  const size_t nv = (size_t)V * V * V;
+  float* z = (float*)ws_get(ctx, WS_TVZ, nv * sizeof(float));
+  float* zn = (float*)ws_get(ctx, WS_TVZN, nv * sizeof(float));
+  float* gd = (float*)ws_get(ctx, WS_TVGD, nv * sizeof(float));
+  float* S = (float*)ws_get(ctx, WS_TVS, (size_t)std::max(A, 1) * C * C * sizeof(float2));
+  if (!z || !zn || !gd || !S) return set_err(ctx, MTD_ERR_HIP, "out of device memory (solver)");
+  if (!(flags & MTD_TV_GRAPH) || n_iter == 0)
+    return tv_loop(ctx, g, spectra, weights, A, n_iter, lam, eps, inv_kappa, positivity, x, z,
+                   zn, gd, S);
+  // one forward and one adjoint on scratch first: every table and workspace
+  // the loop uses exists before the capture (nothing allocates inside it)
+  MTD_TRY(mtd_forward_spectra(ctx, g, x, nullptr, nullptr, A, S));
+  MTD_TRY(mtd_adjoint_spectra(ctx, g, S, nullptr, A, gd, nullptr));
+  MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "solver sync"));
+  if (ctx->tv_exec) (void)hipGraphExecDestroy(ctx->tv_exec);
+  if (ctx->tv_graph) (void)hipGraphDestroy(ctx->tv_graph);
+  ctx->tv_exec = nullptr;
+  ctx->tv_graph = nullptr;
+  if (!ctx->graph_stream)
+    MTD_TRY(hip_check(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking),
+                      "graph stream"));
+  MTD_TRY(hip_check(ctx, hipStreamBeginCapture(ctx->graph_stream, hipStreamCaptureModeThreadLocal),
+                    "solver capture"));
+  const hipStream_t saved = ctx->stream;
+  const bool saved_timing = ctx->timing;
+  ctx->stream = ctx->graph_stream;
+  ctx->timing = false;
+  const int rc = tv_loop(ctx, g, spectra, weights, A, n_iter, lam, eps, inv_kappa, positivity, x,
+                         z, zn, gd, S);
+  ctx->stream = saved;
+  ctx->timing = saved_timing;
+  hipGraph_t gr = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->graph_stream, &gr);
+  if (rc != MTD_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  MTD_TRY(hip_check(ctx, ec, "solver capture"));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    return hip_check(ctx, ei, "solver graph instantiate");
+  }
+  ctx->tv_graph = gr;
+  ctx->tv_exec = ex;
+  return hip_check(ctx, hipGraphLaunch(ex, ctx->stream), "solver graph launch");
 }
 
 int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
