@@ -98,17 +98,21 @@ def kernel_bytes(name, n, C, V, A, P, M, B, b_in=2):
     return table.get(name)
 
 
-def adjoint_quality(ctx, torch, np, m, synth):
-    """The differentiation form against the plain method on its workload: 400
-    off-axis holograms 768^2 uint16 of a seeded ellipsoid phantom (Fermat
-    spiral at 0.95 NA, Poisson noise), crop 128^2, 256^3 index volume, the
-    shipped recon.txt field settings (BORN 0 Rytov, DEROUL 1 Volkov
-    unwrapping, C_ABER 1). Plain = mtd_reconstruct_plain; adjoint-TV = the
-    same preprocessing, per-angle spectra y, then FISTA on
+def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
+                    curve_iters=TV_CURVE):
+    """The differentiation form against the plain method on its workload
+    (default: 400 off-axis holograms 768^2 uint16 of a seeded ellipsoid
+    phantom, Fermat spiral at 0.95 NA, Poisson noise, crop 128^2, 256^3 index
+    volume; "large": 1000 x 2048^2 -> crop 512^2 -> 1024^3), the shipped
+    recon.txt field settings (BORN 0 Rytov, DEROUL 1 Volkov unwrapping,
+    C_ABER 1). Plain = mtd_reconstruct_plain; adjoint-TV = the same
+    preprocessing, per-angle spectra y, then n_iter FISTA iterations on
     1/2 ||A dn - y||_W^2 / kappa + lam TV(dn) with dn >= 0 from the plain
-    start. Index RMSE against the voxelized phantom; GPU times of both."""
+    start, run by the library and captured as one HIP graph
+    (mtd_tv_solve). Index RMSE against the voxelized phantom; GPU times of
+    both."""
     from mtd_transmission_amd import adjoint as adj
-    w = WORKLOADS["diff"]
+    w = WORKLOADS[wname]
     n, nx, V, A, pix, cx, cy = geometry(w)
     C = 2 * nx
     rng = np.random.default_rng(w["seed"])
@@ -139,8 +143,8 @@ def adjoint_quality(ctx, torch, np, m, synth):
     def tv():
         tv_setup()
         kappa = (1.0 / (V * g.delta_f)) ** 6 * V ** 3  # ||A^H W A||, closed form
-        return adj.reconstruct_tv(ctx, g, pos, y, n_iter=TV_ITERS, lam=TV_LAM, eps=TV_EPS,
-                                  dn0=re, kappa=kappa, weights=W)
+        return adj.reconstruct_tv(ctx, g, pos, y, n_iter=n_iter, lam=TV_LAM, eps=TV_EPS,
+                                  dn0=re, kappa=kappa, weights=W, fused=True, graph=True)
 
     def timed(fn, reps):
         fn()
@@ -161,12 +165,12 @@ def adjoint_quality(ctx, torch, np, m, synth):
     # iterations that fit in 10x the plain method's time
     curve = {}
     kappa = (1.0 / (V * g.delta_f)) ** 6 * V ** 3
-    adj.reconstruct_tv(ctx, g, pos, y, n_iter=TV_CURVE, lam=TV_LAM, eps=TV_EPS, dn0=re,
+    adj.reconstruct_tv(ctx, g, pos, y, n_iter=curve_iters, lam=TV_LAM, eps=TV_EPS, dn0=re,
                        kappa=kappa, weights=W,
                        monitor=lambda k, x: curve.__setitem__(k, rmse(x)))
     setup_ms, _ = timed(lambda: tv_setup(), 3)
-    per_iter = max((ms_tv - setup_ms) / TV_ITERS, 1e-6)
-    k10 = int(max(0, min(TV_CURVE, (10 * ms_plain - setup_ms) // per_iter)))
+    per_iter = max((ms_tv - setup_ms) / n_iter, 1e-6)
+    k10 = int(max(0, min(curve_iters, (10 * ms_plain - setup_ms) // per_iter)))
     r10 = curve.get(k10, r_plain)
     print(f"[bench] adjoint-TV: index RMSE plain {r_plain:.5f} ({ms_plain:.2f} ms), "
           f"adjoint-TV {r_tv:.5f} ({ms_tv:.1f} ms); {k10} iterations in 10x plain time: "
@@ -177,7 +181,8 @@ def adjoint_quality(ctx, torch, np, m, synth):
             "rmse_reduction": round(1 - r_tv / r_plain, 4),
             "ms_plain": round(ms_plain, 3), "ms_adjoint_tv": round(ms_tv, 2),
             "time_ratio": round(ms_tv / ms_plain, 1),
-            "iterations": TV_ITERS, "ms_per_iteration": round(per_iter, 3),
+            "iterations": n_iter, "ms_per_iteration": round(per_iter, 3),
+            "solver": "mtd_tv_solve, all iterations captured as one HIP graph",
             "rmse_by_iteration": {str(k): round(curve[k], 6) for k in (5, 10, 15, 20, 30, 50)
                                   if k in curve},
             "iterations_within_10x_plain": k10,
@@ -490,7 +495,12 @@ def main():
                "ms_per_volume": round(cpu_ms_volume, 1)}
 
     hbm = copy_bandwidth(torch)
-    quality = None if args.no_quality else adjoint_quality(ctx, torch, np, m, synth)
+    if args.no_quality:
+        quality = None
+    elif args.workload == "large":  # 50 iterations at 1024^3 on the large acquisition
+        quality = adjoint_quality(ctx, torch, np, m, synth, "large", 50, 50)
+    else:
+        quality = adjoint_quality(ctx, torch, np, m, synth)
 
     out = {
         "metric": METRIC, "value": round(A / (ms / 1e3), 2), "unit": "holograms/s",
