@@ -431,6 +431,17 @@ def main():
                             "ms_per_step": round(tot / args.steps, 4)}
         if b is not None:
             per_kernel[name]["GBps"] = round(b / lps / (avg * 1e-3) / 1e9, 1)
+    # stage times from the breakdown: S1 hologram stage, S2 fields and
+    # per-angle spectra, S3 + accumulation (Ewald gather and its geometry),
+    # S4 finalisation (3-D inverse FFT to the index volume)
+    stage_of = {"holo_rows": "S1", "holo_cols": "S1", "argmax_fin": "S2", "demod_rows": "S2",
+                "demod_cols": "S2", "lsq": "S2", "fwd_rows_field": "S2", "fwd_cols": "S2",
+                "angle_geo": "S3_acc", "ewald_band": "S3_acc", "ewald_tiles": "S3_acc",
+                "ewald_gather": "S3_acc", "vol_x": "S4", "vol_y": "S4", "vol_z": "S4"}
+    stages = {}
+    for name, (launches, tot) in kt_all.items():
+        st = stage_of.get(name, "other")
+        stages[st] = round(stages.get(st, 0.0) + tot / args.steps, 4)
     # the kernel with the largest total time among those with an algorithmic
     # byte count (under a counter-collecting profiler the order can change)
     dom = max((k for k in kt_all if kernel_bytes(k, n, C, V, A, P, M, B) is not None),
@@ -520,6 +531,8 @@ def main():
         "hbm_copy": hbm,
         "adjoint_tv": quality,
         "kernels": per_kernel,
+        "stages_ms": stages,
+        "holograms_per_s_s1_to_s3": round(A / ((ms - stages.get("S4", 0.0)) / 1e3), 2),
         "kernels_note": "HIP-event durations per kernel from K untimed steps with every "
                         "kernel timed; the timed region times only the roofline kernel",
     }

@@ -143,6 +143,28 @@ def test_herraez_is_unsupported(gpu_ctx):
         gpu_ctx.crop_to_field(g, crops, case.acq.mask, pos, fields)
 
 
+def test_specular_argmax_ties(gpu_ctx):
+    """All-zero holograms among ordinary ones: every crop key of those is 0,
+    the exact tie the column pass resolves with the reference's keys (first
+    index, slot 0); the speculars of the whole stack equal the oracle's."""
+    import torch
+    case = make_case("small")
+    _say("specular argmax with all-zero holograms")
+    g = case.lib_geom()
+    holos = case.acq.holos.copy()
+    holos[1] = 0
+    holos[3] = 0
+    _, pos32 = ref.preprocess(holos, case.g, case.acq.mask, "f32")
+    A, C = holos.shape[0], g.C
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.preprocess(g, _t(holos), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    got = pos.cpu().numpy()
+    assert got[1] == 0 and got[1 + A] == 0 and got[3] == 0 and got[3 + A] == 0
+    assert np.array_equal(got, pos32)
+
+
 @pytest.mark.parametrize("name", ["nomask", "small", "cfg1"])
 def test_preprocess(gpu_ctx, name):
     import torch
