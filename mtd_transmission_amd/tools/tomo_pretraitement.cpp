@@ -100,6 +100,7 @@ int main(int argc, char** argv) {
     ++A;
   }
   std::cout << "NbAngle=" << nb_angle << " trouves=" << A << "\n";
+  std::cout << "Temps chargement hologrammes (disque)= " << t_load.s() << "\n";
 
   std::vector<uint8_t> mask;
   const uint8_t* maskp = nullptr;
@@ -147,6 +148,7 @@ int main(int argc, char** argv) {
   std::cout << "Temps pour FFT holo+découpe Spectre+part2 (GPU)= " << t_gpu.s() << "\n";
 
   // outputs (main.cpp:171-173,325-326,351-358)
+  Clock t_write;
   const std::string dimImg = std::to_string(C) + "x" + std::to_string(C) + "x" + std::to_string(A);
   const size_t np = (size_t)A * C * C;
   std::vector<double> re(np), im(np);
@@ -172,6 +174,7 @@ int main(int argc, char** argv) {
   const bool ok5 = mtd::write_tiff_f32(chemin_result + "/centres.tif", centre.data(), C, C, 1,
                                        g.na / g.nxmax, 1);
   mtd_ctx_destroy(ctx);
+  std::cout << "Temps ecriture (disque)= " << t_write.s() << "\n";
   if (!(ok1 && ok2 && ok3 && ok4 && ok5)) {
     std::cerr << "Erreur d'écriture dans " << chemin_result << "\n";
     return 1;
