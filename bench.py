@@ -324,7 +324,8 @@ def run_batched(args):
                                          autotune=True, times=True).tolist())
     lat = np.asarray(lat)
     per_acq = [dict(s, ms=round(float(t), 4)) for s, t in zip(shapes, lat[-len(shapes):])]
-    chunks = {f"{g.n}x{g.C}": ctx.batch_tuned_chunk(g.n, g.C) for g in geoms}
+    chunks = {f"{g.n}x{g.C}": [ctx.batch_tuned_chunk(g.n, g.C), ctx.batch_tuned_window(g.n, g.C)]
+              for g in geoms}
     out = {
         "metric": METRIC, "value": round(total_A / (ms / 1e3), 2), "unit": "holograms/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -338,7 +339,9 @@ def run_batched(args):
                     "p99_ms": round(float(np.percentile(lat, 99)), 4),
                     "eager_ms_per_batch": round(eager_ms, 4),
                     "graph_ms_per_batch": round(ms, 4),
-                    "tuned_hologram_chunks": chunks,
+                    "tuned_schedules": {"key": "hologram side x crop",
+                                        "value": "[holograms per hologram launch pair, "
+                                                 "gather window height]", **chunks},
                     "acquisitions": per_acq},
         "roofline": None, "cpu_baseline": None,
     }

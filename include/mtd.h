@@ -97,6 +97,12 @@ const char* mtd_version(void);
  * (pretraitement_hors_axe_CPU/main.cpp:122-169,226-346;
  * Reconstruction/main.cpp:200-229,298-330). on=1 starts recording. */
 int mtd_ctx_set_timing(mtd_ctx* ctx, int on);
+/* Launch schedule of the plain path (no reference counterpart; results are
+ * the same for every schedule): holo_chunk = holograms per hologram-stage
+ * row/column launch pair (0: as many as fit 128 MiB of row intermediate),
+ * gather_window = height of the Ewald gather's windows, 8, 16 or 24 voxels
+ * (0: 16). The batched form's autotuner picks both per shape. */
+int mtd_ctx_set_schedule(mtd_ctx* ctx, int holo_chunk, int gather_window);
 /* Restricts the timing to the kernel of that report name (NULL or "": all
  * kernels). Each timed launch adds a pair of event markers to the stream,
  * which delays the next dispatch by a few microseconds; timing one kernel
@@ -198,8 +204,9 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
  * moved a workspace or fit plan the graph uses (another entry point needed a
  * larger buffer). MTD_BATCH_AUTOTUNE: at the first sight of a
  * (hologram side, crop, dtype), the hologram-stage chunk (holograms per
- * row/column launch pair) is picked by timing candidates on that
- * acquisition; graphs then use the picked chunks. acq_ms (host, n floats, or
+ * row/column launch pair) and then the Ewald gather's height window (8, 16
+ * or 24 voxels) are picked by timing candidates on that acquisition; graphs
+ * then use the picked schedules (results are the same for every schedule). acq_ms (host, n floats, or
  * NULL): each acquisition's device time in this call, from wall-clock stamps
  * a one-thread kernel writes between acquisitions; the call then waits for
  * the batch to finish. */
@@ -209,9 +216,11 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms,
                           const int* counts, const uint8_t* const* masks,
                           int32_t* const* posspec, float* const* index_re,
                           float* const* index_im, int flags, float* acq_ms);
-/* The hologram chunk the autotuner picked for (hologram side n, crop C,
- * dtype); 0 if that shape was not tuned. */
+/* The hologram chunk and the gather height window (8, 16 or 24 voxels) the
+ * autotuner picked for (hologram side n, crop C, dtype); 0 if that shape was
+ * not tuned. */
 int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype);
+int mtd_batch_tuned_window(mtd_ctx* ctx, int n, int C, int dtype);
 
 /* ---- Born forward model (Simul_holo) and its adjoint -------------------- */
 /* The simulator's hologram-spectrum synthesis (Simul_holo/main.cpp:106-109,

@@ -61,6 +61,7 @@ SIGNATURES = {
     "mtd_last_error": (ctypes.c_char_p, [_P]),
     "mtd_ctx_set_timing": (_I, [_P, _I]),
     "mtd_ctx_set_timing_only": (_I, [_P, ctypes.c_char_p]),
+    "mtd_ctx_set_schedule": (_I, [_P, _I, _I]),
     "mtd_ctx_timing_report": (_I, [_P, ctypes.c_char_p, ctypes.c_size_t]),
     "mtd_ctx_timing_reset": (_I, [_P]),
     "mtd_config_extract_val": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _D,
@@ -79,6 +80,7 @@ SIGNATURES = {
     "mtd_reconstruct_plain": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P, _P]),
     "mtd_reconstruct_batch": (_I, [_P, _I, _GP, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "mtd_batch_tuned_chunk": (_I, [_P, _I, _I, _I]),
+    "mtd_batch_tuned_window": (_I, [_P, _I, _I, _I]),
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
@@ -245,6 +247,12 @@ class Context:
         lib().mtd_ctx_set_timing_only(self.h, (only or "").encode())
         lib().mtd_ctx_set_timing(self.h, int(bool(on)))
 
+    def set_schedule(self, holo_chunk=0, gather_window=0):
+        """Launch schedule: holograms per hologram launch pair and the gather's
+        window height (0: defaults); results do not depend on it."""
+        self._check(lib().mtd_ctx_set_schedule(self.h, int(holo_chunk), int(gather_window)),
+                    "set_schedule")
+
     def timing_reset(self):
         lib().mtd_ctx_timing_reset(self.h)
 
@@ -341,6 +349,9 @@ class Context:
 
     def batch_tuned_chunk(self, n, C, dtype=MTD_U16):
         return lib().mtd_batch_tuned_chunk(self.h, n, C, dtype)
+
+    def batch_tuned_window(self, n, C, dtype=MTD_U16):
+        return lib().mtd_batch_tuned_window(self.h, n, C, dtype)
 
     # ---- Born forward model and adjoint ---------------------------------
     def spectra_to_fields(self, g: Geom, spectra, posspec, fields):

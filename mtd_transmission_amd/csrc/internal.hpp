@@ -85,8 +85,10 @@ struct mtd_ctx {
   // captured at another generation holds stale addresses and is recaptured
   uint64_t ws_gen = 0;
   uint64_t batch_gen = 0;
-  std::map<std::tuple<int, int, int>, int> holo_chunk_tuned;  // (N, C, dtype) -> holograms
+  // (N, C, dtype) -> (holograms per hologram-stage chunk, gather window)
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> batch_tuned;
   int holo_chunk_override = 0;  // > 0: holograms per hologram-stage chunk
+  int gather_zw_override = 0;   // > 0: gather height window (8, 16, 24)
   // geometry plan of the forward model / adjoint (mtd_born_plan): speculars,
   // angle geometry, hit band and the gather's tile lists, in WS_PLAN* slots
   bool plan_valid = false;
@@ -218,7 +220,9 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
                          int nxmax, int V, const double* htab,
                          const float* beta, int r2max, float2* vol,
                          int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, bool prepass, hipStream_t s);
+                         int zlo, int zhi, bool prepass, int zw, hipStream_t s);
+// gather height windows built (8, 16, 24; others fall back to 16)
+bool gather_window_supported(int zw);
 // the gather's prepass alone (per-tile angle lists into scratch)
 hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
                                  const double* htab, int r2max, void* scratch, hipStream_t s);

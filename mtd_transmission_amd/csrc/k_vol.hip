@@ -427,27 +427,13 @@ hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, i
 
 static int prep_chunks(int A) { return std::max(1, (A + kTileChunk - 1) / kTileChunk); }
 
-hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
-                         const int* posy, const AngleGeo* ageo, int A, int C,
-                         int nxmax, int V, const double* htab,
-                         const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, bool prepass, hipStream_t s) {
-  (void)posx;
-  (void)posy;
-  GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
-  constexpr int ZW = 16;
-  const int ntx = V / GT, tiles = ntx * ntx;
-  // scratch: [tiles][A] entries | [tiles][A][256] height offsets | [tiles] info
-  const size_t nA = (size_t)(A > 0 ? A : 1);
-  TileEntry* list = (TileEntry*)scratch;
-  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
-  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
-  if (prepass) {
-    Timed tm("ewald_tiles", s);
-    hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
-                       prep_chunks(A), list, tinfo, zoff);
-  }
+// the windowed gather with height windows of ZW voxels (LDS: 10 B per
+// voxel of the 256 x ZW block)
+template <int ZW>
+static hipError_t gather_windows(const GatherArgs& g, int ntx, int tiles, const TileEntry* list,
+                                 const int4* tinfo, const uint8_t* zoff, int flags,
+                                 const int2* band, int zlo, int zhi, hipStream_t s) {
+  const int V = g.V;
   // banded output: only the windows that can meet the band are launched
   // (the exact band is only known on the device)
   int w0 = 0, w1 = (V + ZW - 1) / ZW - 1;
@@ -459,14 +445,44 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   Timed tm("ewald_gather", s);
   if (flags & 2) {
     hipLaunchKernelGGL((k_gather_win<ZW, true>), dim3(items), dim3(256), 0, s, g, ntx, w0, nwin,
-                       items, (const TileEntry*)list, (const int4*)tinfo, (const uint8_t*)zoff,
-                       flags, band);
+                       items, list, tinfo, zoff, flags, band);
   } else {
     hipLaunchKernelGGL((k_gather_win<ZW, false>), dim3(items), dim3(256), 0, s, g, ntx, w0,
-                       nwin, items, (const TileEntry*)list, (const int4*)tinfo,
-                       (const uint8_t*)zoff, flags, band);
+                       nwin, items, list, tinfo, zoff, flags, band);
   }
   return hipGetLastError();
+}
+
+bool gather_window_supported(int zw) { return zw == 8 || zw == 16 || zw == 24; }
+
+hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
+                         const int* posy, const AngleGeo* ageo, int A, int C,
+                         int nxmax, int V, const double* htab,
+                         const float* beta, int r2max, float2* vol,
+                         int32_t* cnt, void* scratch, int flags, const int2* band,
+                         int zlo, int zhi, bool prepass, int zw, hipStream_t s) {
+  (void)posx;
+  (void)posy;
+  GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
+  const int ntx = V / GT, tiles = ntx * ntx;
+  // scratch: [tiles][A] entries | [tiles][A][256] height offsets | [tiles] info
+  const size_t nA = (size_t)(A > 0 ? A : 1);
+  TileEntry* list = (TileEntry*)scratch;
+  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
+  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
+  if (prepass) {
+    Timed tm("ewald_tiles", s);
+    hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
+                       prep_chunks(A), list, tinfo, zoff);
+  }
+  switch (zw) {
+    case 8:
+      return gather_windows<8>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
+    case 24:
+      return gather_windows<24>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
+    default:
+      return gather_windows<16>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
+  }
 }
 
 hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,

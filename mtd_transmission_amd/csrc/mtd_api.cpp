@@ -654,6 +654,17 @@ int mtd_ctx_set_timing(mtd_ctx* ctx, int on) {
   return MTD_OK;
 }
 
+int mtd_ctx_set_schedule(mtd_ctx* ctx, int holo_chunk, int gather_window) {
+  if (!ctx) return MTD_ERR_ARG;
+  if (holo_chunk < 0) return set_err(ctx, MTD_ERR_ARG, "negative hologram chunk");
+  if (gather_window != 0 && !gather_window_supported(gather_window))
+    return set_err(ctx, MTD_ERR_UNSUPPORTED, "gather window " + std::to_string(gather_window) +
+                                                 " not built (8, 16, 24)");
+  ctx->holo_chunk_override = holo_chunk;
+  ctx->gather_zw_override = gather_window;
+  return MTD_OK;
+}
+
 int mtd_ctx_set_timing_only(mtd_ctx* ctx, const char* kernel) {
   if (!ctx) return MTD_ERR_ARG;
   ctx->timing_only = kernel ? kernel : "";
@@ -889,7 +900,9 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
                                        2 * g->nxmax, g->nxmax, V, t.htab,
                                        weights ? weights : t.beta, t.r2max, (float2*)vol, counts,
                                        scratch, flags, band,
-                                       zlo, zhi, plan == nullptr, ctx->stream),
+                                       zlo, zhi, plan == nullptr,
+                                       ctx->gather_zw_override > 0 ? ctx->gather_zw_override : 16,
+                                       ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
 }
@@ -1279,24 +1292,32 @@ static uint64_t fnv1a(const uint8_t* p, size_t n) {
   return h;
 }
 
-// one acquisition with the hologram chunk picked for its shape
+// one acquisition with the schedule picked for its shape
 static int batch_one(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                      const uint8_t* mask, int32_t* pos, float* re, float* im) {
-  const auto it = ctx->holo_chunk_tuned.find(std::make_tuple(g->n, 2 * g->nxmax, dtype));
-  ctx->holo_chunk_override = it == ctx->holo_chunk_tuned.end() ? 0 : it->second;
+  const auto it = ctx->batch_tuned.find(std::make_tuple(g->n, 2 * g->nxmax, dtype));
+  const int chunk = ctx->holo_chunk_override, zw = ctx->gather_zw_override;
+  if (it != ctx->batch_tuned.end()) {
+    ctx->holo_chunk_override = it->second.first;
+    ctx->gather_zw_override = it->second.second;
+  }
   const int rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
-  ctx->holo_chunk_override = 0;
+  ctx->holo_chunk_override = chunk;
+  ctx->gather_zw_override = zw;
   return rc;
 }
 
-// Times the plain reconstruction of one acquisition with each candidate
-// hologram chunk (holograms per row/column launch pair; the launch count
-// against the tail of each launch) and keeps the fastest for its (N, C,
-// dtype). Each candidate runs twice, the second run timed.
+// Times the plain reconstruction of one acquisition under each candidate
+// schedule and keeps the fastest for its (N, C, dtype): first the hologram
+// chunk (holograms per row/column launch pair: the launch count against
+// the tail of each launch) with 16-high gather windows, then the gather's
+// window height (8, 16, 24: LDS per block, and so blocks per CU, against
+// the entries each window rescans) with that chunk. Each candidate runs
+// twice, the second run timed.
 static int tune_holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                            const uint8_t* mask, int32_t* pos, float* re, float* im) {
   const auto key = std::make_tuple(g->n, 2 * g->nxmax, dtype);
-  if (ctx->holo_chunk_tuned.count(key) || A <= 0) return MTD_OK;
+  if (ctx->batch_tuned.count(key) || A <= 0) return MTD_OK;
   const int cap = (int)std::max<size_t>(
       1, (size_t)2 * kChunkBudget / ((size_t)g->n * 2 * g->nxmax * sizeof(float2)));
   std::vector<int> cand;
@@ -1309,25 +1330,38 @@ static int tune_holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos, i
     (void)hipEventDestroy(e0);
     return set_err(ctx, MTD_ERR_HIP, "tuner event");
   }
-  int best = 0;
+  auto time_one = [&](float* ms) -> int {
+    MTD_TRY(mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im));
+    MTD_TRY(hip_check(ctx, hipEventRecord(e0, ctx->stream), "tuner"));
+    MTD_TRY(mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im));
+    MTD_TRY(hip_check(ctx, hipEventRecord(e1, ctx->stream), "tuner"));
+    MTD_TRY(hip_check(ctx, hipEventSynchronize(e1), "tuner"));
+    return hip_check(ctx, hipEventElapsedTime(ms, e0, e1), "tuner");
+  };
+  const int user_chunk = ctx->holo_chunk_override, user_zw = ctx->gather_zw_override;
+  ctx->gather_zw_override = 16;
+  int best = 0, best_zw = 16;
   float best_ms = 0.f;
   int rc = MTD_OK;
   for (int c : cand) {
     ctx->holo_chunk_override = c;
-    rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
-    if (rc == MTD_OK && (rc = hip_check(ctx, hipEventRecord(e0, ctx->stream), "tuner")) == MTD_OK)
-      rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
-    if (rc == MTD_OK) rc = hip_check(ctx, hipEventRecord(e1, ctx->stream), "tuner");
-    if (rc == MTD_OK) rc = hip_check(ctx, hipEventSynchronize(e1), "tuner");
     float ms = 0.f;
-    if (rc == MTD_OK) rc = hip_check(ctx, hipEventElapsedTime(&ms, e0, e1), "tuner");
-    if (rc != MTD_OK) break;
+    if ((rc = time_one(&ms)) != MTD_OK) break;
     if (!best || ms < best_ms) { best = c; best_ms = ms; }
   }
-  ctx->holo_chunk_override = 0;
+  ctx->holo_chunk_override = best;
+  for (int zw : {8, 24}) {
+    if (rc != MTD_OK) break;
+    ctx->gather_zw_override = zw;
+    float ms = 0.f;
+    if ((rc = time_one(&ms)) != MTD_OK) break;
+    if (ms < best_ms) { best_zw = zw; best_ms = ms; }
+  }
+  ctx->holo_chunk_override = user_chunk;
+  ctx->gather_zw_override = user_zw;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc == MTD_OK) ctx->holo_chunk_tuned[key] = best;
+  if (rc == MTD_OK) ctx->batch_tuned[key] = std::make_pair(best, best_zw);
   return rc;
 }
 
@@ -1456,8 +1490,14 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
 
 int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype) {
   if (!ctx) return -1;
-  const auto it = ctx->holo_chunk_tuned.find(std::make_tuple(n, C, dtype));
-  return it == ctx->holo_chunk_tuned.end() ? 0 : it->second;
+  const auto it = ctx->batch_tuned.find(std::make_tuple(n, C, dtype));
+  return it == ctx->batch_tuned.end() ? 0 : it->second.first;
+}
+
+int mtd_batch_tuned_window(mtd_ctx* ctx, int n, int C, int dtype) {
+  if (!ctx) return -1;
+  const auto it = ctx->batch_tuned.find(std::make_tuple(n, C, dtype));
+  return it == ctx->batch_tuned.end() ? 0 : it->second.second;
 }
 
 }  // extern "C"

@@ -119,8 +119,10 @@ def test_batch_autotune_same_results(gpu_ctx):
     torch.cuda.synchronize()
     _same(list(zip(re, im, pos)), want)
     chunks = [gpu_ctx.batch_tuned_chunk(g.n, g.C) for g in geoms]
-    _say(f"  tuned chunks {chunks}")
+    windows = [gpu_ctx.batch_tuned_window(g.n, g.C) for g in geoms]
+    _say(f"  tuned chunks {chunks}, gather windows {windows}")
     assert all(c > 0 for c in chunks)
+    assert all(w in (8, 16, 24) for w in windows)
     gpu_ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
     torch.cuda.synchronize()
     _same(list(zip(re, im, pos)), want)

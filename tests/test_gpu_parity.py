@@ -143,6 +143,36 @@ def test_herraez_is_unsupported(gpu_ctx):
         gpu_ctx.crop_to_field(g, crops, case.acq.mask, pos, fields)
 
 
+def test_schedules_bit_identical(gpu_ctx):
+    """The launch schedule (hologram chunk, gather window 8/16/24) changes
+    no bit of the speculars or the index volume (cfg1: 100 x 512^2 ->
+    256^3)."""
+    import torch
+    import mtd_transmission_amd as m
+    case = make_case("cfg1")
+    _say("schedules: hologram chunks and gather windows")
+    g = case.lib_geom()
+    V, A = g.dim_final, case.acq.holos.shape[0]
+    holos = _t(case.acq.holos)
+    outs = []
+    try:
+        for chunk, zw in [(0, 0), (7, 8), (0, 24), (1000, 16), (33, 24)]:
+            gpu_ctx.set_schedule(chunk, zw)
+            re = _empty((V, V, V), torch.float32)
+            im = _empty((V, V, V), torch.float32)
+            pos = _empty((2 * A,), torch.int32)
+            gpu_ctx.reconstruct_plain(g, holos, case.acq.mask, re, im, pos)
+            torch.cuda.synchronize()
+            outs.append((re, im, pos))
+        with pytest.raises(m.MTDError):
+            gpu_ctx.set_schedule(0, 12)
+    finally:
+        gpu_ctx.set_schedule(0, 0)
+    for re, im, pos in outs[1:]:
+        assert torch.equal(pos, outs[0][2])
+        assert torch.equal(re, outs[0][0]) and torch.equal(im, outs[0][1])
+
+
 def test_specular_argmax_ties(gpu_ctx):
     """All-zero holograms among ordinary ones: every crop key of those is 0,
     the exact tie the column pass resolves with the reference's keys (first
