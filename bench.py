@@ -169,6 +169,19 @@ def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
     adj.reconstruct_tv(ctx, g, pos, y, n_iter=curve_iters, lam=TV_LAM, eps=TV_EPS, dn0=re,
                        kappa=kappa, weights=W,
                        monitor=lambda k, x: curve.__setitem__(k, rmse(x)))
+    # the configuration's literal settings: lam_TV in {1e-4, 1e-3}, 100
+    # iterations (lam in index units of the kappa-normalised objective)
+    sweep = {}
+    if wname == "diff":
+        for lam in (1e-4, 1e-3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = adj.reconstruct_tv(ctx, g, pos, y, n_iter=100, lam=lam, eps=TV_EPS, dn0=re,
+                                   kappa=kappa, weights=W, fused=True)
+            torch.cuda.synchronize()
+            sweep[f"{lam:g}"] = {"iterations": 100, "index_rmse": round(rmse(r.dn), 6),
+                                 "rmse_reduction": round(1 - rmse(r.dn) / r_plain, 4),
+                                 "ms_solver": round((time.perf_counter() - t0) * 1e3, 2)}
     setup_ms, _ = timed(lambda: tv_setup(), 3)
     per_iter = max((ms_tv - setup_ms) / n_iter, 1e-6)
     k10 = int(max(0, min(curve_iters, (10 * ms_plain - setup_ms) // per_iter)))
@@ -190,6 +203,7 @@ def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
             "iterations_within_10x_plain": k10,
             "rmse_reduction_within_10x_plain": round(1 - r10 / r_plain, 4),
             "lam_tv": TV_LAM, "eps_tv": TV_EPS,
+            "lam_sweep_100_iterations": sweep,
             "weights": "1/(q^2 hits)", "start": "plain"}
 
 
