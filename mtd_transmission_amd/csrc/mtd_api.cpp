@@ -1398,7 +1398,8 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
   if (n == 0) return MTD_OK;
   const bool graph = flags & MTD_BATCH_GRAPH;
   // the arguments a captured graph holds: geometry bytes, every pointer,
-  // counts, dtypes, mask contents and the tuned chunks
+  // counts, dtypes and mask contents (schedules are tuned before a capture,
+  // and every schedule gives the same results)
   std::vector<uint8_t> key;
   key_put(key, &n, sizeof n);
   for (int i = 0; i < n; ++i) {
