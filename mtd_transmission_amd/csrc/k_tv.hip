@@ -13,73 +13,96 @@
 
 namespace mtd {
 
-namespace {
-
-struct Vol {
-  const float* z;
-  int V;
-  __device__ __forceinline__ float at(int x, int y, int k) const {
-    return z[((size_t)k * V + y) * V + x];
-  }
-  // forward differences of voxel (x, y, k) and the reciprocal of their
-  // smoothed norm (one hardware reciprocal square root, about 1 ulp)
-  __device__ __forceinline__ void diffs(int x, int y, int k, float c, float eps2, float& dx,
-                                        float& dy, float& dz, float& rs) const {
-    dx = x + 1 < V ? at(x + 1, y, k) - c : 0.f;
-    dy = y + 1 < V ? at(x, y + 1, k) - c : 0.f;
-    dz = k + 1 < V ? at(x, y, k + 1) - c : 0.f;
-    rs = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz + eps2);
-  }
-};
-
-}  // namespace
+// 2.5-D blocking: a block owns a TVX x TVY (64 x 4) column tile and marches
+// a segment of TVSEG (32) heights. Planes k-1, k, k+1 of the tile with a one-voxel halo in x
+// and y sit in LDS (four rotating slots: plane k+2 is loaded into the slot
+// plane k-2 used while step k reads the other three), so z is read from HBM
+// about once (the halo and the segment ends re-read a few percent, mostly
+// from L2) instead of through thirteen cached loads per voxel.
+constexpr int TVX = 64, TVY = 4, TVSEG = 32;
+constexpr int THX = TVX + 2, THY = TVY + 2, THP = THX * THY;
 
 __global__ void __launch_bounds__(256)
 k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __restrict__ x,
            float* __restrict__ zn, int V, float sd, float lam, float eps, float step,
            float beta, int positivity) {
-  // 1-D grid of V*V rows x cx = ceil(V/256) segments. Blocks are dealt
-  // round-robin to the 8 XCDs; renumbered so that XCD k sweeps the k-th
-  // eighth of the rows in order, the rows y +- 1 and planes k +- 1 the
-  // stencil reads were just read by the same XCD (its L2).
-  const int cx = (V + 255) / 256;
-  const unsigned nb = gridDim.x;
-  unsigned r = blockIdx.x;
-  if (nb % 8 == 0) r = (r % 8) * (nb / 8) + r / 8;
-  const int seg = (int)(r % cx);
-  const unsigned row = r / cx;
-  const int xi = seg * 256 + threadIdx.x, yi = (int)(row % V), ki = (int)(row / V);
-  if (xi >= V) return;
-  const size_t i = ((size_t)ki * V + yi) * V + xi;
-  const Vol vz{z, V};
+  __shared__ float pl[4][THP];
+  const int nbx = (V + TVX - 1) / TVX, nby = (V + TVY - 1) / TVY;
+  int b = blockIdx.x;
+  const int bx = b % nbx;
+  b /= nbx;
+  const int by = b % nby, seg = b / nby;
+  const int X0 = bx * TVX, Y0 = by * TVY;
+  const int k0 = seg * TVSEG, k1 = min(k0 + TVSEG, V);
+  const int tid = threadIdx.x, tx = tid % TVX, ty = tid / TVX;
+  const int X = X0 + tx, Y = Y0 + ty;
+  const size_t plane = (size_t)V * V;
+  // plane k of the tile + halo into dst; voxels outside the volume hold 0
+  // (no term reads them: the far-face conditions below exclude them)
+  auto load_plane = [&](int k, float* dst) {
+    for (int e = tid; e < THP; e += 256) {
+      const int hy = e / THX, hx = e - hy * THX;
+      const int gx = X0 + hx - 1, gy = Y0 + hy - 1;
+      float v = 0.f;
+      if (k >= 0 && k < V && gx >= 0 && gx < V && gy >= 0 && gy < V)
+        v = z[(size_t)k * plane + (size_t)gy * V + gx];
+      dst[e] = v;
+    }
+  };
+  // plane k lives in slot (k - k0 + 1) & 3
+  load_plane(k0 - 1, pl[0]);
+  load_plane(k0, pl[1]);
+  load_plane(k0 + 1, pl[2]);
+  __syncthreads();
   const float eps2 = eps * eps;
-  const float c = z[i];
-  float dx, dy, dz, rs;
-  vz.diffs(xi, yi, ki, c, eps2, dx, dy, dz, rs);
-  float gtv = -(dx + dy + dz) * rs;
-  if (xi > 0) {
-    const float cl = vz.at(xi - 1, yi, ki);
-    float ax, ay, az, ar;
-    vz.diffs(xi - 1, yi, ki, cl, eps2, ax, ay, az, ar);
-    gtv += ax * ar;
+  const bool live = X < V && Y < V;
+  for (int k = k0; k < k1; ++k) {
+    if (k + 1 < k1) load_plane(k + 2, pl[(k - k0 + 3) & 3]);
+    const float* Pm = pl[(k - k0) & 3];
+    const float* Pc = pl[(k - k0 + 1) & 3];
+    const float* Pp = pl[(k - k0 + 2) & 3];
+    auto at = [&](const float* P, int dx, int dy) {
+      return P[(ty + 1 + dy) * THX + (tx + 1 + dx)];
+    };
+    if (live) {
+      // forward differences (0 across the far faces) and the reciprocal of
+      // their smoothed norm, of this voxel and of its three lower neighbours
+      const float c = at(Pc, 0, 0);
+      const float dx = X + 1 < V ? at(Pc, 1, 0) - c : 0.f;
+      const float dy = Y + 1 < V ? at(Pc, 0, 1) - c : 0.f;
+      const float dz = k + 1 < V ? at(Pp, 0, 0) - c : 0.f;
+      const float rs = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz + eps2);
+      float gtv = -(dx + dy + dz) * rs;
+      if (X > 0) {
+        const float cl = at(Pc, -1, 0);
+        const float ax = c - cl;
+        const float ay = Y + 1 < V ? at(Pc, -1, 1) - cl : 0.f;
+        const float az = k + 1 < V ? at(Pp, -1, 0) - cl : 0.f;
+        gtv += ax * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az + eps2);
+      }
+      if (Y > 0) {
+        const float cl = at(Pc, 0, -1);
+        const float ax = X + 1 < V ? at(Pc, 1, -1) - cl : 0.f;
+        const float ay = c - cl;
+        const float az = k + 1 < V ? at(Pp, 0, -1) - cl : 0.f;
+        gtv += ay * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az + eps2);
+      }
+      if (k > 0) {
+        const float cl = at(Pm, 0, 0);
+        const float ax = X + 1 < V ? at(Pm, 1, 0) - cl : 0.f;
+        const float ay = Y + 1 < V ? at(Pm, 0, 1) - cl : 0.f;
+        const float az = c - cl;
+        gtv += az * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az + eps2);
+      }
+      const size_t i = (size_t)k * plane + (size_t)Y * V + X;
+      float xn = c - step * (sd * gd[i] + lam * gtv);
+      if (positivity) xn = fmaxf(xn, 0.f);
+      const float xo = x[i];
+      x[i] = xn;
+      zn[i] = xn + beta * (xn - xo);
+    }
+    __syncthreads();  // plane k+2 is complete; slot of plane k-1 is free
   }
-  if (yi > 0) {
-    const float cl = vz.at(xi, yi - 1, ki);
-    float ax, ay, az, ar;
-    vz.diffs(xi, yi - 1, ki, cl, eps2, ax, ay, az, ar);
-    gtv += ay * ar;
-  }
-  if (ki > 0) {
-    const float cl = vz.at(xi, yi, ki - 1);
-    float ax, ay, az, ar;
-    vz.diffs(xi, yi, ki - 1, cl, eps2, ax, ay, az, ar);
-    gtv += az * ar;
-  }
-  float xn = c - step * (sd * gd[i] + lam * gtv);
-  if (positivity) xn = fmaxf(xn, 0.f);
-  const float xo = x[i];
-  x[i] = xn;
-  zn[i] = xn + beta * (xn - xo);
 }
 
 // Data-term gradient of the weighted least squares on the spectra, in place:
@@ -127,7 +150,8 @@ hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, flo
                            float sd, float lam, float eps, float step, float beta,
                            int positivity, hipStream_t s) {
   Timed tm("tv_fista", s);
-  const unsigned nb = (unsigned)((V + 255) / 256) * (unsigned)V * (unsigned)V;
+  const unsigned nb = (unsigned)((V + TVX - 1) / TVX) * (unsigned)((V + TVY - 1) / TVY) *
+                      (unsigned)((V + TVSEG - 1) / TVSEG);
   hipLaunchKernelGGL(k_tv_fista, dim3(nb), dim3(256), 0, s, z, gd, x, zn, V, sd, lam, eps, step,
                      beta, positivity);
   return hipGetLastError();
