@@ -37,27 +37,49 @@ k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __r
   const int tid = threadIdx.x, tx = tid % TVX, ty = tid / TVX;
   const int X = X0 + tx, Y = Y0 + ty;
   const size_t plane = (size_t)V * V;
-  // plane k of the tile + halo into dst; voxels outside the volume hold 0
-  // (no term reads them: the far-face conditions below exclude them)
-  auto load_plane = [&](int k, float* dst) {
-    for (int e = tid; e < THP; e += 256) {
-      const int hy = e / THX, hx = e - hy * THX;
-      const int gx = X0 + hx - 1, gy = Y0 + hy - 1;
-      float v = 0.f;
-      if (k >= 0 && k < V && gx >= 0 && gx < V && gy >= 0 && gy < V)
-        v = z[(size_t)k * plane + (size_t)gy * V + gx];
-      dst[e] = v;
-    }
+  // The tile + halo of a plane is THP < 512 values: elements tid and
+  // tid + 256 of each thread. Voxels outside the volume hold 0 (no term
+  // reads them: the far-face conditions below exclude them).
+  auto fetch = [&](int k, int e) {
+    const int hy = e / THX, hx = e - hy * THX;
+    const int gx = X0 + hx - 1, gy = Y0 + hy - 1;
+    float v = 0.f;
+    if (e < THP && k >= 0 && k < V && gx >= 0 && gx < V && gy >= 0 && gy < V)
+      v = z[(size_t)k * plane + (size_t)gy * V + gx];
+    return v;
+  };
+  auto put = [&](float* dst, float v0, float v1) {
+    dst[tid] = v0;
+    if (tid + 256 < THP) dst[tid + 256] = v1;
   };
   // plane k lives in slot (k - k0 + 1) & 3
-  load_plane(k0 - 1, pl[0]);
-  load_plane(k0, pl[1]);
-  load_plane(k0 + 1, pl[2]);
+  put(pl[0], fetch(k0 - 1, tid), fetch(k0 - 1, tid + 256));
+  put(pl[1], fetch(k0, tid), fetch(k0, tid + 256));
+  put(pl[2], fetch(k0 + 1, tid), fetch(k0 + 1, tid + 256));
   __syncthreads();
   const float eps2 = eps * eps;
   const bool live = X < V && Y < V;
+  const size_t col = (size_t)Y * V + X;
+  // this column's data gradient and x at the current height, loaded one
+  // step ahead (each thread owns its column: no other thread writes them)
+  float gdc = 0.f, xc = 0.f;
+  if (live) {
+    gdc = gd[(size_t)k0 * plane + col];
+    xc = x[(size_t)k0 * plane + col];
+  }
   for (int k = k0; k < k1; ++k) {
-    if (k + 1 < k1) load_plane(k + 2, pl[(k - k0 + 3) & 3]);
+    // issue the next plane's and the next height's loads first; their
+    // latency hides behind this height's arithmetic
+    const bool more = k + 1 < k1;
+    float n0 = 0.f, n1 = 0.f, gdn = 0.f, xn1 = 0.f;
+    if (more) {
+      n0 = fetch(k + 2, tid);
+      n1 = fetch(k + 2, tid + 256);
+      if (live) {
+        gdn = gd[(size_t)(k + 1) * plane + col];
+        xn1 = x[(size_t)(k + 1) * plane + col];
+      }
+    }
     const float* Pm = pl[(k - k0) & 3];
     const float* Pc = pl[(k - k0 + 1) & 3];
     const float* Pp = pl[(k - k0 + 2) & 3];
@@ -94,13 +116,16 @@ k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __r
         const float az = c - cl;
         gtv += az * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az + eps2);
       }
-      const size_t i = (size_t)k * plane + (size_t)Y * V + X;
-      float xn = c - step * (sd * gd[i] + lam * gtv);
+      const size_t i = (size_t)k * plane + col;
+      float xn = c - step * (sd * gdc + lam * gtv);
       if (positivity) xn = fmaxf(xn, 0.f);
-      const float xo = x[i];
       x[i] = xn;
-      zn[i] = xn + beta * (xn - xo);
+      zn[i] = xn + beta * (xn - xc);
     }
+    // plane k+2 into the slot plane k-2 used (no thread reads it this step)
+    if (more) put(pl[(k - k0 + 3) & 3], n0, n1);
+    gdc = gdn;
+    xc = xn1;
     __syncthreads();  // plane k+2 is complete; slot of plane k-1 is free
   }
 }
