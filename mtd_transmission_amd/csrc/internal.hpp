@@ -67,6 +67,7 @@ struct mtd_ctx {
   std::map<std::tuple<uint64_t, int, uint32_t>, float*> betatab;  // (R, r2max, k0) bits
   std::map<std::tuple<uint64_t, int, uint32_t, int>, float*> beta2tab;  // + NXMAX
   std::map<std::tuple<uint64_t, int, uint64_t, uint64_t>, float*> qtab;  // (R, r2max, lambda, n0)
+  std::map<std::tuple<uint64_t, int, uint64_t, uint64_t, int>, float*> q2tab;  // + NXMAX
   std::map<int, mtd::Buffer> ws;                          // scratch slots
   // least-squares plans per (crop side, mask); lsq points at the current one
   std::list<mtd::LsqPlan> lsq_plans;
@@ -235,7 +236,8 @@ hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, flo
                                    hipStream_t s);
 // One fused FISTA step with the smoothed TV gradient (k_tv.hip).
 hipError_t launch_stamp(unsigned long long* out, hipStream_t s);
-hipError_t launch_residual(float2* S, const float2* y, const float* w, size_t n, hipStream_t s);
+hipError_t launch_residual(float2* S, const float2* y, const float* w, const float* q2, int A,
+                           int C2, hipStream_t s);
 hipError_t launch_tv_start(float* x, float* z, size_t n, int positivity, hipStream_t s);
 hipError_t launch_tv_fista(int V, const float* z, const float* gd, float* x, float* zn,
                            float sd, float lam, float eps, float step, float beta,

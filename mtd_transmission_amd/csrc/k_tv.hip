@@ -132,16 +132,23 @@ k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __r
 
 // Data-term gradient of the weighted least squares on the spectra, in place:
 // S = (S - y) * w (w NULL: 1), the product taken per component as a complex
-// times a real weight.
+// times a real weight; q2 (C x C pupil frame, or NULL): then also times the
+// forward factor q of the bin, the factor the adjoint's gather would apply
+// per hit (the same FP32 product, so the adjoint can skip it).
 __global__ void __launch_bounds__(256)
 k_residual(float2* __restrict__ S, const float2* __restrict__ y, const float* __restrict__ w,
-           size_t n) {
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+           const float* __restrict__ q2, int C2) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= C2) return;
+  const size_t i = (size_t)blockIdx.y * C2 + j;
   const float2 s = S[i], m = y[i];
   float2 r = make_float2(s.x - m.x, s.y - m.y);
   if (w) {
     const float k = w[i];
+    r = make_float2(r.x * k, r.y * k);
+  }
+  if (q2) {
+    const float k = q2[j];
     r = make_float2(r.x * k, r.y * k);
   }
   S[i] = r;
@@ -158,10 +165,11 @@ k_tv_start(float* __restrict__ x, float* __restrict__ z, size_t n, int positivit
   z[i] = v;
 }
 
-hipError_t launch_residual(float2* S, const float2* y, const float* w, size_t n, hipStream_t s) {
-  if (n == 0) return hipSuccess;
+hipError_t launch_residual(float2* S, const float2* y, const float* w, const float* q2, int A,
+                           int C2, hipStream_t s) {
+  if (A <= 0 || C2 <= 0) return hipSuccess;
   Timed tm("residual", s);
-  hipLaunchKernelGGL(k_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, y, w, n);
+  hipLaunchKernelGGL(k_residual, dim3((C2 + 255) / 256, A), dim3(256), 0, s, S, y, w, q2, C2);
   return hipGetLastError();
 }
 

@@ -252,6 +252,7 @@ struct EwaldTables {
   const float* beta;   // per r2 <= r2max
   const float* beta2;  // C x C, pupil frame: beta at fd = (c - NXMAX, r - NXMAX), 0 outside
   const float* q;      // Born forward weights per r2 <= r2max (born_weights)
+  const float* q2;     // the same in the C x C pupil frame, 0 outside the pupil
   int r2tab;
   int r2max;
   double R2;
@@ -609,6 +610,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (auto& kv : ctx->betatab) (void)hipFree(kv.second);
   for (auto& kv : ctx->beta2tab) (void)hipFree(kv.second);
   for (auto& kv : ctx->qtab) (void)hipFree(kv.second);
+  for (auto& kv : ctx->q2tab) (void)hipFree(kv.second);
   for (auto& kv : ctx->ws) (void)hipFree(kv.second.ptr);
   if (ctx->copy_stream) {
     (void)hipStreamSynchronize(ctx->copy_stream);
@@ -820,9 +822,13 @@ static int born_weights(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
   const double c = (1.0 / (2 * PI)) * (PI / k0) * (2 * kv * kv * g->n0);
   const double R = g->rayon;
   const auto key = std::make_tuple(bits_of(R), t->r2max, bits_of(g->lambda), bits_of(g->n0));
+  const int nx = g->nxmax, C = 2 * nx;
+  const auto key2 = std::make_tuple(bits_of(R), t->r2max, bits_of(g->lambda), bits_of(g->n0), nx);
   auto it = ctx->qtab.find(key);
-  if (it != ctx->qtab.end()) {
+  auto it2 = ctx->q2tab.find(key2);
+  if (it != ctx->qtab.end() && it2 != ctx->q2tab.end()) {
     t->q = it->second;
+    t->q2 = it2->second;
     return MTD_OK;
   }
   std::vector<float> q(t->r2max + 1);
@@ -830,10 +836,25 @@ static int born_weights(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
     const double sdz = std::sqrt(t->R2 - (double)r) / R;
     q[r] = (float)(c / sdz);
   }
-  float* d_q = upload(q.data(), q.size());
-  if (!d_q) return set_err(ctx, MTD_ERR_HIP, "out of device memory (forward weights)");
-  ctx->qtab[key] = d_q;
-  t->q = d_q;
+  if (it == ctx->qtab.end()) {
+    float* d_q = upload(q.data(), q.size());
+    if (!d_q) return set_err(ctx, MTD_ERR_HIP, "out of device memory (forward weights)");
+    it = ctx->qtab.emplace(key, d_q).first;
+  }
+  if (it2 == ctx->q2tab.end()) {
+    // the pupil-frame layout of the spectra: bin (c, r) is fd = (c - NXMAX, r - NXMAX)
+    std::vector<float> q2((size_t)C * C, 0.f);
+    for (int r = 0; r < C; ++r)
+      for (int cc = 0; cc < C; ++cc) {
+        const int fdx = cc - nx, fdy = r - nx, r2 = fdx * fdx + fdy * fdy;
+        if (r2 <= t->r2max) q2[(size_t)r * C + cc] = q[r2];
+      }
+    float* d_q2 = upload(q2.data(), q2.size());
+    if (!d_q2) return set_err(ctx, MTD_ERR_HIP, "out of device memory (forward weights)");
+    it2 = ctx->q2tab.emplace(key2, d_q2).first;
+  }
+  t->q = it->second;
+  t->q2 = it2->second;
   return MTD_OK;
 }
 
@@ -1057,9 +1078,19 @@ int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re, con
                    "cap sampling");
 }
 
+// pre: the spectra already carry the forward factor q of their bin (the
+// solver's residual applies it), so the gather adds no per-hit factor
+static int adjoint_impl(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                        const int32_t* posspec, int A, float* out_re, float* out_im, bool pre);
+
 int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
                         const int32_t* posspec, int A, float* out_re, float* out_im) {
   CtxScope scope(ctx);
+  return adjoint_impl(ctx, g, spectra, posspec, A, out_re, out_im, false);
+}
+
+static int adjoint_impl(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
+                        const int32_t* posspec, int A, float* out_re, float* out_im, bool pre) {
   MTD_TRY(validate(ctx, g, A, NEED_VOL));
   if (!out_re || (A > 0 && !spectra)) return set_err(ctx, MTD_ERR_ARG, "null array");
   if (A > 0 && !posspec && !ctx->plan_valid)
@@ -1077,13 +1108,13 @@ int mtd_adjoint_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
     GeoBufs geo;
     MTD_TRY(plan_bufs(ctx, g, A, &geo));
     band = geo.band;
-    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, nullptr, A, (float*)vol, nullptr, 4,
-                         band, t.q, &geo));
+    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, nullptr, A, (float*)vol, nullptr,
+                         pre ? 4 | 2 : 4, band, t.q, &geo));
   } else {
     band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
     if (!band) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
-    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, (float*)vol, nullptr, 4,
-                         band, t.q));
+    MTD_TRY(scatter_impl(ctx, g, (const float2*)spectra, 1, posspec, A, (float*)vol, nullptr,
+                         pre ? 4 | 2 : 4, band, t.q));
   }
   const double tp = tomo_pixel(g);
   return finalize_impl(ctx, g, (const float*)vol, nullptr, 1, band, out_re, out_im, nullptr,
@@ -1164,15 +1195,20 @@ static int tv_loop(mtd_ctx* ctx, const mtd_geom* g, const float* y, const float*
                    int n_iter, float lam, float eps, float inv_kappa, int positivity, float* x,
                    float* z, float* zn, float* gd, float* S) {
   const int V = g->dim_final, C = 2 * g->nxmax;
-  const size_t nv = (size_t)V * V * V, ns = (size_t)A * C * C;
+  const size_t nv = (size_t)V * V * V;
   const float step = (float)(1.0 / (1.0 + 12.0 * (double)lam / (double)eps));
   MTD_TRY(hip_check(ctx, launch_tv_start(x, z, nv, positivity, ctx->stream), "solver start"));
   double t = 1.0;
+  EwaldTables et;
+  MTD_TRY(ewald_tables(ctx, g, &et));
+  MTD_TRY(born_weights(ctx, g, &et));
   for (int k = 1; k <= n_iter; ++k) {
     MTD_TRY(mtd_forward_spectra(ctx, g, z, nullptr, nullptr, A, S));
-    MTD_TRY(hip_check(ctx, launch_residual((float2*)S, (const float2*)y, w, ns, ctx->stream),
+    // W (S - y) times q: the adjoint's gather then takes the spectra as is
+    MTD_TRY(hip_check(ctx, launch_residual((float2*)S, (const float2*)y, w, et.q2, A, C * C,
+                                           ctx->stream),
                       "residual"));
-    MTD_TRY(mtd_adjoint_spectra(ctx, g, S, nullptr, A, gd, nullptr));
+    MTD_TRY(adjoint_impl(ctx, g, S, nullptr, A, gd, nullptr, true));
     const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
     MTD_TRY(mtd_tv_fista_step(ctx, V, z, gd, x, zn, inv_kappa, lam, eps, step,
                               (float)((t - 1.0) / tn), positivity));
@@ -1207,7 +1243,7 @@ int mtd_tv_solve(mtd_ctx* ctx, const mtd_geom* g, const float* spectra, const fl
   // one forward and one adjoint on scratch first: every table and workspace
   // the loop uses exists before the capture (nothing allocates inside it)
   MTD_TRY(mtd_forward_spectra(ctx, g, x, nullptr, nullptr, A, S));
-  MTD_TRY(mtd_adjoint_spectra(ctx, g, S, nullptr, A, gd, nullptr));
+  MTD_TRY(adjoint_impl(ctx, g, S, nullptr, A, gd, nullptr, true));
   MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "solver sync"));
   if (ctx->tv_exec) (void)hipGraphExecDestroy(ctx->tv_exec);
   if (ctx->tv_graph) (void)hipGraphDestroy(ctx->tv_graph);
