@@ -248,7 +248,7 @@ hipError_t launch_cap_weights(const int32_t* cnt, const AngleGeo* ageo, int A,
                               int V, float* w, hipStream_t s);
 // Per-angle cap samples -i q(|fd|^2) vol[voxel] into spec[A][C][C] (+posSpec frame).
 hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,
-                             const float* q, int r2max, int nx, int C, int V, float2* spec,
+                             const float* q2, int r2max, int nx, int C, int V, float2* spec,
                              hipStream_t s);
 // prenorm=1: vol already divided by counts and centring-signed (gather flag 1).
 // band != nullptr: heights outside the band are zero and are not read.

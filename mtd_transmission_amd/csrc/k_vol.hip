@@ -693,12 +693,15 @@ static hipError_t contrast_spectrum_v(const float* re, const float* im, float2* 
 // are 0.
 __global__ void __launch_bounds__(256)
 k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
-             const double* __restrict__ htab, const float* __restrict__ q, int r2max, int nx,
+             const double* __restrict__ htab, const float* __restrict__ q2, int r2max, int nx,
              int C, int V, float2* __restrict__ spec) {
-  const int a = blockIdx.y;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= C * C) return;
-  const int fdx = i % C - nx, fdy = i / C - nx;
+  // grid (column blocks, rows, angles): no division by the runtime C; the
+  // forward factor is read from the C x C pupil-frame table (coalesced)
+  const int a = blockIdx.z, row = blockIdx.y;
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= C) return;
+  const int i = row * C + col;
+  const int fdx = col - nx, fdy = row - nx;
   const int r2 = fdx * fdx + fdy * fdy;
   const AngleGeo ag = ageo[a];
   float2 s = make_float2(0.f, 0.f);
@@ -707,7 +710,7 @@ k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
     if (z >= 0 && z < V) {
       const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
       const float2 t = vol[((size_t)z * V + Y) * V + X];
-      const float w = q[r2];
+      const float w = q2[i];
       s = make_float2(w * t.y, -(w * t.x));
     }
   }
@@ -773,11 +776,12 @@ hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, flo
 }
 
 hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,
-                             const float* q, int r2max, int nx, int C, int V, float2* spec,
+                             const float* q2, int r2max, int nx, int C, int V, float2* spec,
                              hipStream_t s) {
   Timed tm("cap_sample", s);
-  hipLaunchKernelGGL(k_cap_sample, dim3((C * C + 255) / 256, A), dim3(256), 0, s, vol, ageo,
-                     htab, q, r2max, nx, C, V, spec);
+  if (A <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cap_sample, dim3((C + 255) / 256, C, A), dim3(256), 0, s, vol, ageo,
+                     htab, q2, r2max, nx, C, V, spec);
   return hipGetLastError();
 }
 

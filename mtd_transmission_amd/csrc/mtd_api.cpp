@@ -1073,7 +1073,7 @@ int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re, con
                                                   (float)(tp * tp * tp) * gsign, geo.band, twV,
                                                   ctx->stream),
                     "contrast spectrum"));
-  return hip_check(ctx, launch_cap_sample(work, geo.ag, A, t.htab, t.q, t.r2max, g->nxmax, C, V,
+  return hip_check(ctx, launch_cap_sample(work, geo.ag, A, t.htab, t.q2, t.r2max, g->nxmax, C, V,
                                           (float2*)spectra, ctx->stream),
                    "cap sampling");
 }
