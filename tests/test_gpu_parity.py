@@ -432,3 +432,32 @@ def test_host_hologram_stack(gpu_ctx, pinned):
     gpu_ctx.reconstruct_plain(g, hh, case.acq.mask, r1, i1)
     torch.cuda.synchronize()
     assert torch.equal(r0, r1) and torch.equal(i0, i1)
+
+
+def test_timing_only_one_kernel(gpu_ctx):
+    """mtd_ctx_set_timing_only: the report then holds that kernel alone, with
+    one entry per launch; all kernels again after the filter is cleared."""
+    import torch
+    case = make_case("small")
+    g = case.lib_geom()
+    V, A = g.dim_final, case.acq.holos.shape[0]
+    holos = _t(case.acq.holos)
+    re = _empty((V, V, V), torch.float32)
+    im = _empty((V, V, V), torch.float32)
+    try:
+        gpu_ctx.timing_reset()
+        gpu_ctx.set_timing(True, only="holo_rows")
+        gpu_ctx.reconstruct_plain(g, holos, case.acq.mask, re, im, None)
+        gpu_ctx.reconstruct_plain(g, holos, case.acq.mask, re, im, None)
+        gpu_ctx.set_timing(False)
+        kt = gpu_ctx.timing()
+        assert set(kt) == {"holo_rows"} and kt["holo_rows"][0] >= 2
+        gpu_ctx.timing_reset()
+        gpu_ctx.set_timing(True)
+        gpu_ctx.reconstruct_plain(g, holos, case.acq.mask, re, im, None)
+        gpu_ctx.set_timing(False)
+        kt = gpu_ctx.timing()
+        assert {"holo_rows", "holo_cols", "ewald_gather", "vol_z"} <= set(kt)
+    finally:
+        gpu_ctx.set_timing(False)
+        gpu_ctx.timing_reset()
