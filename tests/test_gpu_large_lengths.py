@@ -1,0 +1,87 @@
+"""The large configuration's transform lengths: 2048^2 holograms with crop 512
+(its preprocessing against the FP32 oracle), and the 1024^3 finalisation
+(3-D inverse FFT with the centring signs and constants) against its closed
+form for a spectrum of a few normalised deltas, each a plane wave of known
+frequency, compared at 200 000 sampled voxels."""
+import math
+
+import numpy as np
+import pytest
+
+from cases import rel_l2
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+LAM, N0, NA = 6.33e-7, 1.333, 1.2
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_preprocess_2048_crop_512(gpu_ctx):
+    import torch
+    import mtd_transmission_amd as m
+    from mtd_transmission_amd import synth
+    n, nx, V, A = 2048, 256, 1024, 2
+    _say("preprocess 2048^2 holograms, crop 512")
+    pix = nx * LAM / (NA * n)
+    cx, cy = n // 2 - round(0.3 * n), n // 2 + round(0.3 * n)
+    ph = synth.make_phantom(77, 4)
+    kx, ky = synth.scan_fermat(A, nx, 0.95)
+    holos, mask = synth.simulate_torch(n, nx, cx, cy, pix, LAM, N0, kx, ky, ph, seed=78)
+    gl = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix)
+    go = ref.derive_geom(n, nx, V, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    C = 2 * nx
+    pos = torch.empty((2 * A,), dtype=torch.int32, device="cuda")
+    fields = torch.empty((A, C, C), dtype=torch.complex64, device="cuda")
+    gpu_ctx.preprocess(gl, holos, mask, pos, fields)
+    torch.cuda.synchronize()
+    f32, p32 = ref.preprocess(holos.cpu().numpy(), go, mask, "f32")
+    assert np.array_equal(pos.cpu().numpy(), p32)
+    e = rel_l2(fields.cpu().numpy(), f32)
+    _say(f"  fields rel-L2 vs f32 {e:.3e}")
+    assert e < 1e-5
+
+
+def test_finalize_1024_deltas_closed_form(gpu_ctx):
+    import torch
+    import mtd_transmission_amd as m
+    n, nx, V = 2048, 256, 1024
+    _say("finalize 1024^3: three normalised deltas")
+    pix = nx * LAM / (NA * n)
+    cx, cy = n // 2 - round(0.3 * n), n // 2 + round(0.3 * n)
+    gl = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix)
+    go = ref.derive_geom(n, nx, V, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    # (x, y, z) voxels of the centred spectrum, values, counts
+    deltas = [((512 + 37, 512 - 101, 512 + 3), 6.0 - 3.0j, 3),
+              ((512 - 250, 512 + 7, 512 - 64), -1.5 + 2.5j, 1),
+              ((512, 512, 512), 0.5 + 0.25j, 2)]
+    vol = torch.zeros((V, V, V), dtype=torch.complex64, device="cuda")
+    cnt = torch.zeros((V, V, V), dtype=torch.int32, device="cuda")
+    for (x, y, z), v, c in deltas:
+        vol[z, y, x] = v
+        cnt[z, y, x] = c
+    re = torch.empty((V, V, V), device="cuda")
+    im = torch.empty_like(re)
+    gpu_ctx.finalize(gl, vol, cnt, re, im, None)
+    torch.cuda.synchronize()
+    del vol, cnt
+    rng = np.random.default_rng(9)
+    idx = rng.integers(0, V ** 3, size=200_000)
+    ti = torch.from_numpy(idx).cuda()
+    got = (re.view(-1)[ti].double() + 1j * im.view(-1)[ti].double()).cpu().numpy()
+    z, rem = np.divmod(idx, V * V)
+    y, x = np.divmod(rem, V)
+    kv = 2 * math.pi / go.lam
+    cte = -1 / (2 * go.n0 * kv * kv)
+    want = np.zeros(idx.size, complex)
+    for (ux, uy, uz), v, c in deltas:
+        k = (ux - V // 2, uy - V // 2, uz - V // 2)
+        ph = 2 * np.pi * (k[0] * (x - V // 2) + k[1] * (y - V // 2) + k[2] * (z - V // 2)) / V
+        want += (v / c) * np.exp(1j * ph)
+    want *= cte * go.delta_f ** 3
+    e = rel_l2(got, want)
+    _say(f"  index rel-L2 vs closed form at 200000 voxels {e:.3e}")
+    assert e < 1e-5
