@@ -605,14 +605,24 @@ def ewald_scatter(S, pos, g: Geom, prec="f64"):
     (fdy-fym0+V/2)*V + (koz+V/2)*V^2; the angle is skipped when
     round(R^2-|fm0|^2) < 0. Values (i*beta)*S = (-beta*Si, beta*Sr) summed in
     angle order."""
+    V = g.dim_final
+    vol = np.zeros(V ** 3, dtype=_cplx(prec))
+    cnt = np.zeros(V ** 3, dtype=np.int32)
+    for vox, val in _scatter_hits(S, pos, g, prec):
+        vol[vox] += val
+        cnt[vox] += 1
+    return vol, cnt
+
+
+def _scatter_hits(S, pos, g: Geom, prec):
+    """Per non-skipped angle in order: the voxel indices of its pupil pixels
+    and their values (i*beta)*S (retroPropag_Born's index and value)."""
     ct = _cplx(prec)
     rt = _real(prec)
     A = S.shape[0]
     nx, V, R = g.nxmax, g.dim_final, g.rayon
     half = V // 2
     fdx, fdy, pupil, h, beta = pupil_tables(g)
-    vol = np.zeros(V ** 3, dtype=ct)
-    cnt = np.zeros(V ** 3, dtype=np.int32)
     fdx_p, fdy_p, h_p = fdx[pupil], fdy[pupil], h[pupil]
     beta_p = beta[pupil].astype(rt)
     for a in range(A):
@@ -625,10 +635,28 @@ def ewald_scatter(S, pos, g: Geom, prec="f64"):
         koz = round_half_away(h_p - fzm0).astype(np.int64)
         vox = (fdx_p - fxm0 + half) + (fdy_p - fym0 + half) * V + (koz + half) * V * V
         s = S[a][pupil]
-        val = _mk(-(beta_p * s.imag.astype(rt)), beta_p * s.real.astype(rt), ct)
-        vol[vox] += val
-        cnt[vox] += 1
-    return vol, cnt
+        yield vox, _mk(-(beta_p * s.imag.astype(rt)), beta_p * s.real.astype(rt), ct)
+
+
+def ewald_scatter_sparse(S, pos, g: Geom, prec="f64"):
+    """ewald_scatter restricted to the voxels that receive hits, for volumes
+    too large to hold densely: (sorted voxel indices, sums, counts), each sum
+    accumulated in angle order in the working precision, as the dense form
+    (within one angle no voxel is hit twice)."""
+    vs, xs = [], []
+    for vox, val in _scatter_hits(S, pos, g, prec):
+        vs.append(vox)
+        xs.append(val)
+    if not vs:
+        return np.zeros(0, np.int64), np.zeros(0, _cplx(prec)), np.zeros(0, np.int32)
+    vox = np.concatenate(vs)
+    val = np.concatenate(xs)
+    uniq, inv = np.unique(vox, return_inverse=True)
+    sums = np.zeros(uniq.size, dtype=_cplx(prec))
+    cnt = np.zeros(uniq.size, dtype=np.int32)
+    np.add.at(sums, inv, val)  # unbuffered, in input (angle) order
+    np.add.at(cnt, inv, 1)
+    return uniq, sums, cnt
 
 
 def ewald_bounds_ok(pos, g: Geom):

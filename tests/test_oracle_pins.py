@@ -338,6 +338,23 @@ def test_ewald_counts_and_cap_closed_form():
     assert cnt.sum() == P * (3 - int(skipped))
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ewald_sparse_equals_dense(prec):
+    """The sparse form (for volumes too large to hold densely) holds exactly
+    the dense form's nonzero voxels, bit for bit, with overlapping angles."""
+    g = _ewald_geom(nx=6, V=24)
+    nx, C = g.nxmax, g.C
+    rng = np.random.default_rng(8)
+    A = 9
+    pos = np.concatenate([nx + rng.integers(-3, 4, A), nx + rng.integers(-3, 4, A)])
+    S = rng.normal(size=(A, C, C)) + 1j * rng.normal(size=(A, C, C))
+    vol, cnt = ref.ewald_scatter(S, pos, g, prec)
+    vox, sums, c = ref.ewald_scatter_sparse(S, pos, g, prec)
+    assert np.array_equal(vox, np.flatnonzero(cnt))
+    assert np.array_equal(c, cnt[vox]) and int(c.max()) > 1
+    assert np.array_equal(sums.view(np.uint8), vol[vox].view(np.uint8))
+
+
 # --------------------------------------------------------------------------
 # A18-A22 finalize
 # --------------------------------------------------------------------------
