@@ -126,3 +126,52 @@ def test_scatter_1024_bit_exact_at_hits(gpu_ctx):
          f"nonzero counts {nz}, nonzero values {nzv}")
     assert total == int(c.sum()) and nz == vox.size
     assert nzv == int(np.count_nonzero(sums))
+
+
+def test_forward_1024_point_object_closed_form(gpu_ctx):
+    """Born forward model at 1024^3 with crop 512: a point contrast at
+    centred offset d gives on every cap sample w(fd) Tp^3 c exp(-2 pi i u.d/V),
+    u = (fd - fm0, koz), and 0 off the pupil (the closed form the oracle's
+    forward model is pinned to); 4 angles, one of them off the sphere."""
+    import torch
+    import mtd_transmission_amd as m
+    n, nx, V = 2048, 256, 1024
+    _say("forward model at 1024^3: point contrast")
+    pix = nx * LAM / (NA * n)
+    cx, cy = n // 2 - round(0.3 * n), n // 2 + round(0.3 * n)
+    gl = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix)
+    go = ref.derive_geom(n, nx, V, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    C, R = 2 * nx, go.rayon
+    d = np.array([37, -55, 12])
+    cval = 0.05
+    dn = torch.zeros((V, V, V), device="cuda")
+    dn[V // 2 + d[2], V // 2 + d[1], V // 2 + d[0]] = cval
+    spec = [(nx, nx), (nx + 140, nx - 90), (nx - 200, nx + 60), (0, 0)]
+    A = len(spec)
+    pos = np.array([p[0] for p in spec] + [p[1] for p in spec], dtype=np.int64)
+    S = torch.empty((A, C, C), dtype=torch.complex64, device="cuda")
+    gpu_ctx.forward_spectra(gl, dn, None, torch.from_numpy(pos.astype(np.int32)).cuda(), S)
+    torch.cuda.synchronize()
+    del dn
+    got = S.cpu().numpy().astype(np.complex128)
+    kv = 2 * 3.14159 / go.lam
+    tp3 = ref.tomo_pixel(go) ** 3
+    fy, fx = np.mgrid[-nx:nx, -nx:nx]
+    pupil = fx * fx + fy * fy < nx * nx
+    h = np.sqrt(np.maximum(R * R - fx * fx - fy * fy, 0.0))
+    want = np.zeros_like(got)
+    for a in range(A):
+        fxm0, fym0 = int(pos[a]) - nx, int(pos[a + A]) - nx
+        if ref.round_half_away(R * R - fxm0 ** 2 - fym0 ** 2) < 0:
+            continue  # off the sphere: a skipped angle, all zero
+        fzm0 = math.sqrt(R * R - fxm0 ** 2 - fym0 ** 2)
+        koz = ref.round_half_away(h - fzm0)
+        ud = (fx - fxm0) * d[0] + (fy - fym0) * d[1] + koz * d[2]
+        w = np.where(pupil, (-1j * kv / np.where(pupil, h / R, 1.0)) * tp3 * cval
+                     * np.exp(-2j * np.pi * ud / V), 0)
+        want[a] = w
+    assert not got[3].any()
+    assert np.all(got[:, ~pupil] == 0)
+    e = rel_l2(got, want)
+    _say(f"  spectra rel-L2 vs closed form {e:.3e}")
+    assert e < 1e-5
