@@ -28,7 +28,13 @@ k_tv_fista(const float* __restrict__ z, const float* __restrict__ gd, float* __r
            float beta, int positivity) {
   __shared__ float pl[4][THP];
   const int nbx = (V + TVX - 1) / TVX, nby = (V + TVY - 1) / TVY;
+  // Blocks are dealt round-robin to the 8 XCDs; renumbered so that XCD k
+  // runs the k-th eighth of the tiles in order: the tiles above and below a
+  // tile, whose rows form its y halo, then sit in the same XCD's L2
+  // (a bijection when the block count is a multiple of 8)
+  const unsigned nb = gridDim.x;
   int b = blockIdx.x;
+  if (nb % 8 == 0) b = (int)((blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8);
   const int bx = b % nbx;
   b /= nbx;
   const int by = b % nby, seg = b / nby;
