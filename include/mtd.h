@@ -133,6 +133,16 @@ int mtd_geom_derive(double ccd_roix, double nxmax, double dim_final,
                     mtd_geom* out);
 /* tukey2D 1-D factor (FFT_fonctions.cpp:61-87), host arrays of n doubles. */
 int mtd_tukey1d(int n, double alpha, double* out);
+/* Float32 multipage TIFF writer of indice.tif / absorption.tif / OTF3D.tif
+ * (SAV3D_Tiff_Optimized, Reconstruction/src/fonctions.cpp:2286-2357: one
+ * page per z, one strip per row, resolution 0.01/pixel_m per cm; orientation
+ * 4 = ORIENTATION_BOTLEFT there, 1 for 2-D images). data: host float
+ * [pages][h][w]. format 0: classic TIFF, or BigTIFF (64-bit offsets) when the
+ * file would pass 4 GiB (a 1024^3 volume), where libtiff's classic mode
+ * fails; 1: classic only (returns -2 and writes nothing past 4 GiB);
+ * 2: BigTIFF. Returns 0 (classic written), 1 (BigTIFF written), < 0 error. */
+int mtd_write_tiff_f32(const char* path, const float* data, int w, int h,
+                       int pages, double pixel_m, int orientation, int format);
 
 /* ---- preprocessing (tomo_pretraitement) -------------------------------- */
 /* holo2TF_UBorn over a stack (fonctions.cpp:279-304, loop main.cpp:130-165):

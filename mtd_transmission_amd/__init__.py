@@ -17,6 +17,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 MTD_U8, MTD_U16, MTD_F32 = 0, 1, 2
 MTD_BATCH_GRAPH, MTD_BATCH_AUTOTUNE = 1, 2
+MTD_DEROUL_OFF, MTD_DEROUL_VOLKOV, MTD_DEROUL_HERRAEZ = 0, 1, 2
 
 
 class MTDError(RuntimeError):
@@ -70,6 +71,7 @@ SIGNATURES = {
                                        ctypes.c_char_p, ctypes.c_size_t]),
     "mtd_geom_derive": (_I, [_D] * 13 + [_I, _I, _I, _GP]),
     "mtd_tukey1d": (_I, [_I, _D, ctypes.POINTER(_D)]),
+    "mtd_write_tiff_f32": (_I, [ctypes.c_char_p, _P, _I, _I, _I, _D, _I, _I]),
     "mtd_holo_to_crop": (_I, [_P, _GP, _P, _I, _I, _P]),
     "mtd_crop_to_field": (_I, [_P, _GP, _P, _I, _P, _P, _P]),
     "mtd_preprocess": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P]),
@@ -157,6 +159,26 @@ def tukey1d(n, alpha):
     return [arr[i] for i in range(n)]
 
 
+def write_tiff_f32(path, pages, pixel_m=1.0, orientation=4, fmt=0):
+    """pages: float32 (pages, h, w) or (h, w) host array -> TIFF file (fmt 0:
+    classic, BigTIFF past 4 GiB; 1: classic only; 2: BigTIFF). Returns
+    "tiff" or "bigtiff"."""
+    import numpy as np
+    a = np.ascontiguousarray(pages, dtype=np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    if a.ndim != 3:
+        raise MTDError("pages must be (pages, h, w)")
+    rc = lib().mtd_write_tiff_f32(str(path).encode(), a.ctypes.data_as(ctypes.c_void_p),
+                                  a.shape[2], a.shape[1], a.shape[0], float(pixel_m),
+                                  int(orientation), int(fmt))
+    if rc == -2:
+        raise MTDError("classic TIFF asked for, but the file would pass 4 GiB")
+    if rc < 0:
+        raise MTDError(f"cannot write {path}")
+    return "bigtiff" if rc == 1 else "tiff"
+
+
 # --------------------------------------------------------------------------
 # device context
 # --------------------------------------------------------------------------
@@ -168,6 +190,42 @@ def _ptr(t):
     if not t.is_contiguous():
         raise MTDError("device arrays must be contiguous")
     return ctypes.c_void_p(t.data_ptr())
+
+
+# element kinds of the C ABI's arrays: accepted torch dtypes and the scalars
+# of that dtype per element
+_KINDS = {
+    "c32": (("complex64", 1), ("float32", 2)),  # interleaved (re, im) pairs
+    "f32": (("float32", 1),),
+    "i32": (("int32", 1),),
+    "u8": (("uint8", 1), ("bool", 1)),
+}
+
+
+def _dev(t, kind, count, name, optional=False):
+    """Pointer of a device array after checking its dtype and that it holds
+    at least `count` elements of `kind` (the library reads and writes that
+    many without further checks)."""
+    if t is None:
+        if optional:
+            return None
+        raise MTDError(f"{name}: array required")
+    dt = str(t.dtype).replace("torch.", "")
+    per = dict(_KINDS[kind]).get(dt)
+    if per is None:
+        raise MTDError(f"{name}: dtype {dt} not accepted (expects {kind}: "
+                       f"{', '.join(d for d, _ in _KINDS[kind])})")
+    if t.numel() < per * count:
+        raise MTDError(f"{name}: {t.numel()} {dt} values, needs {per * count}")
+    return _ptr(t)
+
+
+def _holo_count(holos, g):
+    """Angles of a hologram stack [A][N][N] (numel checked against N)."""
+    A = holos.shape[0] if holos.dim() > 0 else 0
+    if holos.numel() < A * g.n * g.n:
+        raise MTDError(f"holograms: {holos.numel()} values for {A} x {g.n}^2")
+    return A
 
 
 def _ptr_holos(t):
@@ -267,58 +325,84 @@ class Context:
         return out
 
     # ---- stages ---------------------------------------------------------
+    # Every wrapper checks dtypes and sizes against the geometry before the
+    # call: the library trusts the pointers it is given.
     def holo_to_crop(self, g: Geom, holos, crops):
         self._stream()
+        A = _holo_count(holos, g)
         self._check(lib().mtd_holo_to_crop(self.h, ctypes.byref(g), _ptr_holos(holos),
-                                           _dtype_code(holos), holos.shape[0],
-                                           _ptr(crops)), "holo_to_crop")
+                                           _dtype_code(holos), A,
+                                           _dev(crops, "c32", A * g.C * g.C, "crops")),
+                    "holo_to_crop")
 
     def crop_to_field(self, g: Geom, crops, mask, posspec, fields):
         self._stream()
         keep, mp = _mask_ptr(mask, g.C)
-        self._check(lib().mtd_crop_to_field(self.h, ctypes.byref(g), _ptr(crops),
-                                            crops.shape[0], mp, _ptr(posspec),
-                                            _ptr(fields)), "crop_to_field")
+        A = crops.shape[0]
+        n = A * g.C * g.C
+        self._check(lib().mtd_crop_to_field(self.h, ctypes.byref(g), _dev(crops, "c32", n, "crops"),
+                                            A, mp, _dev(posspec, "i32", 2 * A, "posspec"),
+                                            _dev(fields, "c32", n, "fields")), "crop_to_field")
 
     def preprocess(self, g: Geom, holos, mask, posspec, fields):
         self._stream()
         keep, mp = _mask_ptr(mask, g.C)
+        A = _holo_count(holos, g)
         self._check(lib().mtd_preprocess(self.h, ctypes.byref(g), _ptr_holos(holos),
-                                         _dtype_code(holos), holos.shape[0], mp,
-                                         _ptr(posspec), _ptr(fields)), "preprocess")
+                                         _dtype_code(holos), A, mp,
+                                         _dev(posspec, "i32", 2 * A, "posspec"),
+                                         _dev(fields, "c32", A * g.C * g.C, "fields")),
+                    "preprocess")
 
     def field_to_spectrum(self, g: Geom, fields, posspec, spectra):
         self._stream()
-        self._check(lib().mtd_field_to_spectrum(self.h, ctypes.byref(g), _ptr(fields),
-                                                _ptr(posspec), fields.shape[0],
-                                                _ptr(spectra)), "field_to_spectrum")
+        A = fields.shape[0]
+        n = A * g.C * g.C
+        self._check(lib().mtd_field_to_spectrum(self.h, ctypes.byref(g),
+                                                _dev(fields, "c32", n, "fields"),
+                                                _dev(posspec, "i32", 2 * A, "posspec"), A,
+                                                _dev(spectra, "c32", n, "spectra")),
+                    "field_to_spectrum")
 
     def ewald_scatter(self, g: Geom, spectra, posspec, vol, counts):
         self._stream()
         A = spectra.shape[0] if spectra is not None else 0
-        self._check(lib().mtd_ewald_scatter(self.h, ctypes.byref(g), _ptr(spectra),
-                                            _ptr(posspec), A, _ptr(vol), _ptr(counts)),
-                    "ewald_scatter")
+        nv = g.dim_final ** 3
+        self._check(lib().mtd_ewald_scatter(
+            self.h, ctypes.byref(g), _dev(spectra, "c32", A * g.C * g.C, "spectra", A == 0),
+            _dev(posspec, "i32", 2 * A, "posspec", A == 0), A, _dev(vol, "c32", nv, "vol"),
+            _dev(counts, "i32", nv, "counts")), "ewald_scatter")
 
     def ewald_accumulate(self, g: Geom, fields, posspec, vol, counts):
         self._stream()
-        self._check(lib().mtd_ewald_accumulate(self.h, ctypes.byref(g), _ptr(fields),
-                                               _ptr(posspec), fields.shape[0], _ptr(vol),
-                                               _ptr(counts)), "ewald_accumulate")
+        A = fields.shape[0]
+        nv = g.dim_final ** 3
+        self._check(lib().mtd_ewald_accumulate(
+            self.h, ctypes.byref(g), _dev(fields, "c32", A * g.C * g.C, "fields"),
+            _dev(posspec, "i32", 2 * A, "posspec"), A, _dev(vol, "c32", nv, "vol"),
+            _dev(counts, "i32", nv, "counts")), "ewald_accumulate")
 
     def finalize(self, g: Geom, vol, counts, index_re, index_im, otf=None):
         self._stream()
-        self._check(lib().mtd_finalize(self.h, ctypes.byref(g), _ptr(vol), _ptr(counts),
-                                       _ptr(index_re), _ptr(index_im), _ptr(otf)),
+        nv = g.dim_final ** 3
+        self._check(lib().mtd_finalize(self.h, ctypes.byref(g), _dev(vol, "c32", nv, "vol"),
+                                       _dev(counts, "i32", nv, "counts"),
+                                       _dev(index_re, "f32", nv, "index_re"),
+                                       _dev(index_im, "f32", nv, "index_im"),
+                                       _dev(otf, "u8", nv, "otf", True)),
                     "finalize")
 
     def reconstruct_plain(self, g: Geom, holos, mask, index_re, index_im, posspec=None):
         self._stream()
         keep, mp = _mask_ptr(mask, g.C)
+        A = _holo_count(holos, g)
+        nv = g.dim_final ** 3
         self._check(lib().mtd_reconstruct_plain(self.h, ctypes.byref(g), _ptr_holos(holos),
-                                                _dtype_code(holos), holos.shape[0], mp,
-                                                _ptr(posspec), _ptr(index_re),
-                                                _ptr(index_im)), "reconstruct_plain")
+                                                _dtype_code(holos), A, mp,
+                                                _dev(posspec, "i32", 2 * A, "posspec", True),
+                                                _dev(index_re, "f32", nv, "index_re"),
+                                                _dev(index_im, "f32", nv, "index_im")),
+                    "reconstruct_plain")
 
     def reconstruct_batch(self, geoms, holos, masks, index_re, index_im, posspecs=None,
                           graph=True, autotune=False, times=False):
@@ -333,13 +417,15 @@ class Context:
         G = (Geom * n)(*geoms)
         VP = ctypes.c_void_p * n
         keep = [_mask_ptr(masks[i] if masks else None, geoms[i].C) for i in range(n)]
+        cnts = [_holo_count(h, g) for h, g in zip(holos, geoms)]
         hp = VP(*[_ptr_holos(h) for h in holos])
         dt = (ctypes.c_int * n)(*[_dtype_code(h) for h in holos])
-        ct = (ctypes.c_int * n)(*[h.shape[0] for h in holos])
+        ct = (ctypes.c_int * n)(*cnts)
         mp = VP(*[k[1] for k in keep])
-        pp = VP(*[_ptr(p) for p in posspecs]) if posspecs else None
-        rp = VP(*[_ptr(t) for t in index_re])
-        ip = VP(*[_ptr(t) for t in index_im])
+        pp = (VP(*[_dev(p, "i32", 2 * a, "posspec", True) for p, a in zip(posspecs, cnts)])
+              if posspecs else None)
+        rp = VP(*[_dev(t, "f32", g.dim_final ** 3, "index_re") for t, g in zip(index_re, geoms)])
+        ip = VP(*[_dev(t, "f32", g.dim_final ** 3, "index_im") for t, g in zip(index_im, geoms)])
         ms = np.zeros(n, dtype=np.float32) if times else None
         flags = (MTD_BATCH_GRAPH if graph else 0) | (MTD_BATCH_AUTOTUNE if autotune else 0)
         self._check(lib().mtd_reconstruct_batch(
@@ -357,16 +443,23 @@ class Context:
     def spectra_to_fields(self, g: Geom, spectra, posspec, fields):
         """The simulator's field synthesis: fields[A][C][C] (UBornfinal layout)."""
         self._stream()
-        self._check(lib().mtd_spectra_to_fields(self.h, ctypes.byref(g), _ptr(spectra),
-                                                _ptr(posspec), spectra.shape[0], _ptr(fields)),
+        A = spectra.shape[0]
+        n = A * g.C * g.C
+        self._check(lib().mtd_spectra_to_fields(self.h, ctypes.byref(g),
+                                                _dev(spectra, "c32", n, "spectra"),
+                                                _dev(posspec, "i32", 2 * A, "posspec"), A,
+                                                _dev(fields, "c32", n, "fields")),
                     "spectra_to_fields")
 
     def born_plan(self, g: Geom, posspec):
         """Geometry plan: later forward/adjoint/weights calls with
         posspec=None reuse its angle geometry, band and tile lists."""
         self._stream()
-        self._check(lib().mtd_born_plan(self.h, ctypes.byref(g), _ptr(posspec),
-                                        posspec.numel() // 2), "born_plan")
+        if posspec.numel() % 2:
+            raise MTDError("posspec: 2*A values (all x, then all y)")
+        A = posspec.numel() // 2
+        self._check(lib().mtd_born_plan(self.h, ctypes.byref(g),
+                                        _dev(posspec, "i32", 2 * A, "posspec"), A), "born_plan")
 
     @staticmethod
     def _angles(posspec, arr):
@@ -377,25 +470,35 @@ class Context:
         posspec=None: the geometry plan."""
         self._stream()
         A = self._angles(posspec, spectra)
-        self._check(lib().mtd_forward_spectra(self.h, ctypes.byref(g), _ptr(dn_re),
-                                              _ptr(dn_im), _ptr(posspec), A,
-                                              _ptr(spectra)), "forward_spectra")
+        nv = g.dim_final ** 3
+        self._check(lib().mtd_forward_spectra(self.h, ctypes.byref(g),
+                                              _dev(dn_re, "f32", nv, "dn_re"),
+                                              _dev(dn_im, "f32", nv, "dn_im", True),
+                                              _dev(posspec, "i32", 2 * A, "posspec", True), A,
+                                              _dev(spectra, "c32", A * g.C * g.C, "spectra")),
+                    "forward_spectra")
 
     def adjoint_spectra(self, g: Geom, spectra, posspec, out_re, out_im):
         """(out_re, out_im) = A^H spectra; posspec=None: the geometry plan."""
         self._stream()
         A = self._angles(posspec, spectra)
-        self._check(lib().mtd_adjoint_spectra(self.h, ctypes.byref(g), _ptr(spectra),
-                                              _ptr(posspec), A, _ptr(out_re),
-                                              _ptr(out_im)), "adjoint_spectra")
+        nv = g.dim_final ** 3
+        self._check(lib().mtd_adjoint_spectra(self.h, ctypes.byref(g),
+                                              _dev(spectra, "c32", A * g.C * g.C, "spectra"),
+                                              _dev(posspec, "i32", 2 * A, "posspec", True), A,
+                                              _dev(out_re, "f32", nv, "out_re"),
+                                              _dev(out_im, "f32", nv, "out_im", True)),
+                    "adjoint_spectra")
 
     def cap_weights(self, g: Geom, posspec, weights):
         """weights[A][C][C] float = 1 / (q^2 * hits of the sample's voxel);
         posspec=None: the geometry plan."""
         self._stream()
         A = self._angles(posspec, weights)
-        self._check(lib().mtd_cap_weights(self.h, ctypes.byref(g), _ptr(posspec), A,
-                                          _ptr(weights)), "cap_weights")
+        self._check(lib().mtd_cap_weights(self.h, ctypes.byref(g),
+                                          _dev(posspec, "i32", 2 * A, "posspec", True), A,
+                                          _dev(weights, "f32", A * g.C * g.C, "weights")),
+                    "cap_weights")
 
     def tv_fista_step(self, z, grad_data, x, z_next, data_scale, lam, eps, step, beta,
                       positivity=True):
@@ -403,9 +506,15 @@ class Context:
         z_next = x' + beta (x' - x), x <- x' (float V^3 volumes)."""
         self._stream()
         V = z.shape[-1]
-        self._check(lib().mtd_tv_fista_step(self.h, V, _ptr(z), _ptr(grad_data), _ptr(x),
-                                            _ptr(z_next), data_scale, lam, eps, step, beta,
-                                            int(bool(positivity))), "tv_fista_step")
+        nv = V ** 3
+        if z.numel() != nv:
+            raise MTDError(f"z: {z.numel()} values, not a {V}^3 volume")
+        self._check(lib().mtd_tv_fista_step(self.h, V, _dev(z, "f32", nv, "z"),
+                                            _dev(grad_data, "f32", nv, "grad_data"),
+                                            _dev(x, "f32", nv, "x"),
+                                            _dev(z_next, "f32", nv, "z_next"), data_scale, lam,
+                                            eps, step, beta, int(bool(positivity))),
+                    "tv_fista_step")
 
     def tv_solve(self, g: Geom, spectra, weights, x, n_iter, lam, eps, inv_kappa,
                  positivity=True, graph=False):
@@ -413,7 +522,11 @@ class Context:
         (born_plan first); x (float V^3) is the start and the result.
         graph: the iterations are captured into one HIP graph, launched once."""
         self._stream()
-        self._check(lib().mtd_tv_solve(self.h, ctypes.byref(g), _ptr(spectra), _ptr(weights),
-                                       spectra.shape[0], int(n_iter), lam, eps, inv_kappa,
-                                       int(bool(positivity)), _ptr(x), 1 if graph else 0),
+        A = spectra.shape[0]
+        n = A * g.C * g.C
+        self._check(lib().mtd_tv_solve(self.h, ctypes.byref(g), _dev(spectra, "c32", n, "spectra"),
+                                       _dev(weights, "f32", n, "weights", True), A, int(n_iter),
+                                       lam, eps, inv_kappa, int(bool(positivity)),
+                                       _dev(x, "f32", g.dim_final ** 3, "x"),
+                                       1 if graph else 0),
                     "tv_solve")

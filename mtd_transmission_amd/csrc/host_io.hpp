@@ -28,8 +28,16 @@ bool write_raw(const std::string& path, const void* src, size_t bytes, bool appe
 
 // Multi-page float32 TIFF, one strip per row, uncompressed; resolution
 // 0.01/pixel_m pixels per centimetre; orientation 1 (top-left) or 4
-// (bottom-left, the reference's 3-D writer).
+// (bottom-left, the reference's 3-D writer). A file whose offsets would not
+// fit 32 bits (over 4 GiB, e.g. a 1024^3 volume) is written as BigTIFF.
 bool write_tiff_f32(const std::string& path, const float* data, int w, int h, int pages,
                     double pixel_m, int orientation);
+// format 0: classic TIFF, BigTIFF when it would exceed 4 GiB; 1: classic
+// only (-2, nothing written, when it would exceed 4 GiB); 2: BigTIFF.
+// Returns 0 (classic written), 1 (BigTIFF written) or < 0 on failure.
+int write_tiff_f32_fmt(const std::string& path, const float* data, int w, int h, int pages,
+                       double pixel_m, int orientation, int format);
+// file size of the layout write_tiff_f32_fmt produces
+uint64_t tiff_file_bytes(bool big, int w, int h, int pages);
 
 }  // namespace mtd

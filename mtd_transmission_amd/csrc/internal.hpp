@@ -59,6 +59,10 @@ struct mtd_ctx {
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_ready[2] = {nullptr, nullptr};
   hipEvent_t ev_free[2] = {nullptr, nullptr};
+  // slot k of the staging buffer has been written by an upload since the
+  // context was created: the next upload into it (in this call or a later
+  // one) must wait for ev_free[k], the last pass that read it
+  bool slot_used[2] = {false, false};
   std::string err;
   std::map<int, float2*> twiddles;                       // per FFT length
   std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)

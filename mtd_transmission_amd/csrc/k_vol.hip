@@ -690,7 +690,8 @@ static hipError_t contrast_spectrum_v(const float* re, const float* im, float2* 
 // coordinates is the same integer map), and the value is
 // -i * q(|fd|^2) * TF3D[voxel] with q = kv / (sqrt(R^2 - |fd|^2) / R). Bins
 // outside the pupil, of skipped angles, or whose height leaves the volume
-// are 0.
+// are 0, and so are bins whose voxel would leave the volume laterally (a
+// caller's specular outside the crop; the accumulation has no hit there).
 __global__ void __launch_bounds__(256)
 k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
              const double* __restrict__ htab, const float* __restrict__ q2, int r2max, int nx,
@@ -707,8 +708,8 @@ k_cap_sample(const float2* __restrict__ vol, const AngleGeo* __restrict__ ageo,
   float2 s = make_float2(0.f, 0.f);
   if (!ag.skip && r2 <= r2max) {
     const int z = (int)round_away(htab[r2] - ag.fz) + V / 2;
-    if (z >= 0 && z < V) {
-      const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+    const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+    if (z >= 0 && z < V && X >= 0 && X < V && Y >= 0 && Y < V) {
       const float2 t = vol[((size_t)z * V + Y) * V + X];
       const float w = q2[i];
       s = make_float2(w * t.y, -(w * t.x));
@@ -735,8 +736,8 @@ k_cap_weights(const int32_t* __restrict__ cnt, const AngleGeo* __restrict__ ageo
   float out = 0.f;
   if (!ag.skip && r2 <= r2max) {
     const int z = (int)round_away(htab[r2] - ag.fz) + V / 2;
-    if (z >= 0 && z < V) {
-      const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+    const int X = fdx - ag.fx + V / 2, Y = fdy - ag.fy + V / 2;
+    if (z >= 0 && z < V && X >= 0 && X < V && Y >= 0 && Y < V) {
       const int c = cnt[((size_t)z * V + Y) * V + X];  // >= 1: this sample hits it
       const float qq = q[r2];
       out = 1.f / (qq * qq * (float)c);

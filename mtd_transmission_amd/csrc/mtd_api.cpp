@@ -542,6 +542,14 @@ static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dt
   char* slots = (char*)ws_get(ctx, WS_HSTAGE, 2 * (size_t)hc * hb);
   if (!slots) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram staging)");
   const int nchunks = (n + hc - 1) / hc;
+  // A previous call may have used another chunk size, so its slots can
+  // overlap either slot of this call: the first upload waits for the last
+  // readers of both. Later uploads into slot k wait for the pass that last
+  // read slot k.
+  for (int k = 0; k < 2; ++k)
+    if (ctx->slot_used[k])
+      MTD_TRY(hip_check(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->ev_free[k], 0),
+                        "copy wait"));
   bool used[2] = {false, false};
   auto prefetch = [&](int i) -> int {
     const int k = i % 2, h0 = i * hc, m = std::min(hc, n - h0);
@@ -565,6 +573,7 @@ static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dt
     MTD_TRY(holo_chunk(ctx, g, slots + (size_t)k * hc * hb, dtype, 0, m,
                        crop + (size_t)h0 * C * C, part_at(h0)));
     MTD_TRY(hip_check(ctx, hipEventRecord(ctx->ev_free[k], ctx->stream), "slot event"));
+    ctx->slot_used[k] = true;
   }
   return MTD_OK;
 }
@@ -1481,7 +1490,9 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
   if (acq_ms) MTD_TRY(batch_times(ctx, n, acq_ms));
   if (!graph) return MTD_OK;
   // capture on a private stream (a graph cannot be captured from the legacy
-  // default stream); launches then go to the context's stream
+  // default stream); launches then go to the context's stream. A replay of
+  // the previous graph may still run on that stream: wait before freeing it.
+  MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch sync"));
   if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
   if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
   ctx->batch_exec = nullptr;
@@ -1490,7 +1501,6 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
   if (!ctx->graph_stream)
     MTD_TRY(hip_check(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking),
                       "graph stream"));
-  MTD_TRY(hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch sync"));
   MTD_TRY(hip_check(ctx, hipStreamBeginCapture(ctx->graph_stream, hipStreamCaptureModeThreadLocal),
                     "batch capture"));
   const hipStream_t saved = ctx->stream;

@@ -755,7 +755,9 @@ def _cap_voxels(pos, a, g: Geom):
     Reconstruction/src/fonctions.cpp:918-926, which is calcHolo's
     fobj = round(fd - fi) in centred coordinates, Simul_holo/src/fonctions.cpp:
     124-136 with Point3D::coordI, src/Point3D.cpp:55-58). None for a skipped
-    angle; -1 where the height leaves the volume."""
+    angle; -1 where the voxel leaves the volume (the height, or laterally for
+    a specular outside the crop, where the reference would index out of
+    bounds)."""
     A = len(pos) // 2
     nx, V, R = g.nxmax, g.dim_final, g.rayon
     half = V // 2
@@ -765,8 +767,11 @@ def _cap_voxels(pos, a, g: Geom):
     if round_half_away(fz2) < 0:
         return None
     koz = round_half_away(h[pupil] - math.sqrt(fz2)).astype(np.int64) + half
-    vox = (fdx[pupil] - fxm0 + half) + (fdy[pupil] - fym0 + half) * V + koz * V * V
-    return np.where((koz >= 0) & (koz < V), vox, -1)
+    X = fdx[pupil] - fxm0 + half
+    Y = fdy[pupil] - fym0 + half
+    vox = X + Y * V + koz * V * V
+    ok = (koz >= 0) & (koz < V) & (X >= 0) & (X < V) & (Y >= 0) & (Y < V)
+    return np.where(ok, vox, -1)
 
 
 def born_forward(dn, pos, g: Geom, prec="f64"):

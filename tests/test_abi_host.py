@@ -116,3 +116,31 @@ def test_synthetic_cases_are_seeded_and_well_posed():
     A = len(a.acq.kx)
     assert np.array_equal(pos[:A] - a.g.nxmax, a.acq.kx)
     assert np.array_equal(pos[A:] - a.g.nxmax, a.acq.ky)
+
+
+def test_tiff_writer_classic_and_bigtiff(tmp_path):
+    """The float32 TIFF writer: small files are classic TIFF (magic 42) and
+    read back exactly, for 1-, 2- and 3-row pages; BigTIFF (magic 43, 64-bit
+    offsets, values of up to 8 bytes stored in the entries) reads back
+    exactly too; a classic file that would pass 4 GiB (a 1024^3 float
+    volume) is refused before anything is written, where the default format
+    switches to BigTIFF."""
+    from tiffread import read_tiff_f32
+    rng = np.random.default_rng(3)
+    for h in (1, 2, 3):
+        a = rng.normal(size=(4, h, 5)).astype(np.float32)
+        for fmt, kind in ((0, "tiff"), (1, "tiff"), (2, "bigtiff")):
+            p = tmp_path / f"v{h}_{fmt}.tif"
+            assert m.write_tiff_f32(p, a, 1e-7, 4, fmt) == kind
+            back, k = read_tiff_f32(p, with_format=True)
+            assert k == kind and np.array_equal(back, a)
+    big = np.zeros((1, 1, 1), np.float32)  # never read: refused on size
+    p = tmp_path / "huge.tif"
+    rc = m.lib().mtd_write_tiff_f32(str(p).encode(), big.ctypes.data_as(m.ctypes.c_void_p),
+                                    1024, 1024, 1024, 1e-7, 4, 1)
+    assert rc == -2 and not p.exists()
+    # 1023 pages: the pixels alone fit 32-bit offsets, with each page's strip
+    # tables (8 KiB) the file does not
+    rc = m.lib().mtd_write_tiff_f32(str(p).encode(), big.ctypes.data_as(m.ctypes.c_void_p),
+                                    1024, 1024, 1023, 1e-7, 4, 1)
+    assert rc == -2 and not p.exists()

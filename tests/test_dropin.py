@@ -3,7 +3,6 @@ exactly like the reference: a HOME with .config/gui_tomo.conf pointing at
 recon.txt / config_manip.txt, holograms i%03d.pgm and Image_mask.pgm, outputs
 in CHEMIN_RESULT in the reference's file formats."""
 import os
-import struct
 import subprocess
 
 import numpy as np
@@ -12,32 +11,10 @@ import pytest
 import mtd_transmission_amd as m
 from cases import LAM, N0, NA, make_case, rel_l2
 from oracle import ref
+from tiffread import read_tiff_f32
 
 PRE = os.path.join(m.BIN_DIR, "tomo_pretraitement")
 REC = os.path.join(m.BIN_DIR, "tomo_reconstruction")
-
-
-def read_tiff_f32(path):
-    """Pages of an uncompressed little-endian float32 TIFF -> (pages, h, w)."""
-    b = open(path, "rb").read()
-    assert b[:4] == b"II*\x00"
-    off = struct.unpack_from("<I", b, 4)[0]
-    pages = []
-    while off:
-        n = struct.unpack_from("<H", b, off)[0]
-        tags = {}
-        for i in range(n):
-            tag, typ, cnt, val = struct.unpack_from("<HHII", b, off + 2 + 12 * i)
-            if typ == 3 and cnt == 1:
-                val &= 0xFFFF
-            tags[tag] = (typ, cnt, val)
-        w, h = tags[256][2], tags[257][2]
-        so = tags[273]
-        offs = [so[2]] if so[1] == 1 else list(struct.unpack_from(f"<{so[1]}I", b, so[2]))
-        rows = [np.frombuffer(b, "<f4", w * (len(offs) and h // len(offs)), o) for o in offs]
-        pages.append(np.concatenate(rows).reshape(h, w))
-        off = struct.unpack_from("<I", b, off + 2 + 12 * n)[0]
-    return np.stack(pages)
 
 
 def write_pgm16(path, img):

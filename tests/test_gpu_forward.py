@@ -93,6 +93,40 @@ def test_forward_spectra(gpu_ctx, name, cplx):
     assert e32 < TOL
 
 
+def test_specular_outside_crop(gpu_ctx):
+    """Speculars a caller places outside the crop (x or y = -1 or C + 1, not
+    skipped since |fm0| < R): bins whose voxel would leave the volume
+    laterally are 0 (the reference indexes out of bounds there), the others
+    equal the oracle's, for the forward model and its adjoint."""
+    import torch
+    nx, V = 16, 64
+    C = 2 * nx
+    _say("forward/adjoint with speculars outside the crop")
+    g, gl = _geoms(nx, V)
+    pos = np.array([-1, C + 1, nx, nx, nx, nx, -1, nx])  # x of 4 angles, then y
+    A = 4
+    dn, _ = _phantom(V, 9)
+    S32 = ref.born_forward(dn.astype(np.complex64), pos, g, "f32")
+    S64 = ref.born_forward(dn.astype(np.complex128), pos, g, "f64")
+    assert (S64[0] == 0).sum() > (S64[3] == 0).sum()  # lateral exits happen
+    out = _empty((A, C, C), torch.complex64)
+    tpos = _t(pos.astype(np.int32))
+    gpu_ctx.forward_spectra(gl, _t(dn), None, tpos, out)
+    rng = np.random.default_rng(4)
+    y = (rng.normal(size=(A, C, C)) + 1j * rng.normal(size=(A, C, C))).astype(np.complex64)
+    re = _empty((V, V, V), torch.float32)
+    im = _empty((V, V, V), torch.float32)
+    gpu_ctx.adjoint_spectra(gl, _t(y), tpos, re, im)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.all(got[S64 == 0] == 0)
+    e32 = rel_l2(got, S32)
+    adj = re.cpu().numpy() + 1j * im.cpu().numpy()
+    e64 = rel_l2(adj, ref.born_adjoint(y.astype(np.complex128), pos, g))
+    _say(f"  forward rel-L2 vs f32 {e32:.3e}; adjoint vs f64 {e64:.3e}")
+    assert e32 < TOL and e64 < TOL
+
+
 @pytest.mark.parametrize("name", ["small", "mid"])
 def test_adjoint_spectra(gpu_ctx, name):
     import torch

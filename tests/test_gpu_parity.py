@@ -3,11 +3,13 @@
 Bars: posspec and hit counts bit-exact; the Ewald accumulation bit-exact
 against the FP32 oracle on identical spectra; complex crops, fields, spectra
 and the index volume within rel-L2 1e-5 of the FP32 oracle (the reference FP32
-path). Each test also prints its error against the FP64 oracle.
+path), and within twice the FP32 oracle's own distance from the FP64 oracle
+(at least 1e-5) of the FP64 oracle, which each test also prints.
 """
 import numpy as np
 import pytest
 
+from lengths import fp64_bound
 from cases import make_case, rel_l2
 from oracle import ref
 
@@ -58,7 +60,7 @@ def test_holo_to_crop(gpu_ctx, name):
     exp64 = np.stack([ref.holo_to_crop(h, case.g, "f64") for h in case.acq.holos])
     e32, e64 = rel_l2(got, exp32), rel_l2(got, exp64)
     _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}  (f32 oracle vs f64 {rel_l2(exp32, exp64):.3e})")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(exp32, exp64)
 
 
 @pytest.mark.parametrize("name", ["nomask", "small", "u16", "cfg1"])
@@ -80,7 +82,7 @@ def test_crop_to_field(gpu_ctx, name):
     got = fields.cpu().numpy()
     e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
     _say(f"  fields rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(fields32, fields64)
 
 
 @pytest.mark.parametrize("name", ["small", "u16"])
@@ -125,7 +127,7 @@ def test_crop_to_field_volkov(gpu_ctx, name, born):
     e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
     _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}  (f32 oracle vs f64 "
          f"{rel_l2(fields32, fields64):.3e})")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(fields32, fields64)
 
 
 def test_herraez_is_unsupported(gpu_ctx):
@@ -212,7 +214,7 @@ def test_preprocess(gpu_ctx, name):
     got = fields.cpu().numpy()
     e32, e64 = rel_l2(got, fields32), rel_l2(got, fields64)
     _say(f"  fields rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(fields32, fields64)
 
 
 @pytest.mark.parametrize("name", ["tiny", "small", "cfg1"])
@@ -231,7 +233,7 @@ def test_field_to_spectrum(gpu_ctx, name):
     got = out.cpu().numpy()
     e32, e64 = rel_l2(got, S32), rel_l2(got, S64)
     _say(f"  spectra rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(S32, S64)
 
 
 @pytest.mark.parametrize("name", ["tiny", "small", "u16", "cfg1"])
@@ -277,7 +279,7 @@ def test_finalize(gpu_ctx, name):
     e32, e64 = rel_l2(got, idx32), rel_l2(got, idx64)
     _say(f"  index rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
     assert np.array_equal(otf.cpu().numpy().astype(bool), otf32)
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(idx32, idx64)
 
 
 @pytest.mark.parametrize("name,c_aber", [("nomask", 1), ("small", 1), ("cfg1", 1),
@@ -304,7 +306,7 @@ def test_reconstruct_plain(gpu_ctx, name, c_aber):
     got = re.cpu().numpy() + 1j * im.cpu().numpy()
     e32, e64 = rel_l2(got, idx32), rel_l2(got, idx64)
     _say(f"  index rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e} (oracle f32 vs f64 {rel_l2(idx32, idx64):.3e})")
-    assert e32 < TOL
+    assert e32 < TOL and e64 <= fp64_bound(idx32, idx64)
 
 
 def test_accumulate_counts_match_scatter(gpu_ctx):
@@ -432,6 +434,47 @@ def test_host_hologram_stack(gpu_ctx, pinned):
     gpu_ctx.reconstruct_plain(g, hh, case.acq.mask, r1, i1)
     torch.cuda.synchronize()
     assert torch.equal(r0, r1) and torch.equal(i0, i1)
+
+
+def test_host_stack_back_to_back(gpu_ctx):
+    """Two host (pinned) stacks preprocessed back to back with no wait between
+    the calls, each spanning two field chunks (Volkov unwrapping at crop 256:
+    about 150 angles per chunk, 170 angles) and with different hologram chunk
+    sizes (50, then 70): the second call's uploads must wait for the first
+    call's passes that still read the staging slots. Fields and speculars are
+    bit-identical to the same stacks preprocessed from device memory."""
+    import torch
+    import mtd_transmission_amd as m
+    from cases import LAM, N0, NA
+    _say("host stacks back to back, two field chunks each")
+    n, nx, A = 1024, 128, 170
+    pix = nx * LAM / (NA * n)
+    g = m.geom_derive(n, nx, 512, n // 2 - round(0.3 * n), n // 2 + round(0.3 * n), NA, N0,
+                      LAM, pix, 1.0, 1.0, 1.0, 1.0, 1, 1, m.MTD_DEROUL_VOLKOV)
+    C = g.C
+    gen = torch.Generator().manual_seed(11)
+    hs = [torch.randint(0, 60000, (A, n, n), generator=gen, dtype=torch.int32)
+          .to(torch.uint16).pin_memory() for _ in range(2)]
+    ref_f, ref_p = [], []
+    for h in hs:
+        f = _empty((A, C, C), torch.complex64)
+        p = _empty((2 * A,), torch.int32)
+        gpu_ctx.preprocess(g, h.cuda(), None, p, f)
+        ref_f.append(f)
+        ref_p.append(p)
+    torch.cuda.synchronize()
+    got_f = [_empty((A, C, C), torch.complex64) for _ in range(2)]
+    got_p = [_empty((2 * A,), torch.int32) for _ in range(2)]
+    try:
+        for k, chunk in enumerate((50, 70)):
+            gpu_ctx.set_schedule(holo_chunk=chunk)
+            gpu_ctx.preprocess(g, hs[k], None, got_p[k], got_f[k])
+        torch.cuda.synchronize()
+    finally:
+        gpu_ctx.set_schedule(0, 0)
+    for k in range(2):
+        assert torch.equal(got_p[k], ref_p[k])
+        assert torch.equal(torch.view_as_real(got_f[k]), torch.view_as_real(ref_f[k]))
 
 
 def test_timing_only_one_kernel(gpu_ctx):
