@@ -1,5 +1,6 @@
 // Complex helpers and small in-register DFT kernels (radix 2, 3, 4, 5, 7, 8,
-// 11) used by the register FFTs of fft_reg.hpp.
+// 11, 13) used by the register FFTs of fft_reg.hpp and the runtime-length
+// FFTs of gen_fft.hpp.
 //
 // Sign convention (FFTW): forward  X[k] = sum x[n] exp(-2 pi i n k / L),
 //                         backward X[k] = sum x[n] exp(+2 pi i n k / L),
@@ -122,6 +123,26 @@ struct OddConst<11> {
   }
 };
 
+template <>
+struct OddConst<13> {
+  __device__ static constexpr float c(int k) {
+    return k == 1 ? 0.88545602565320990f
+         : k == 2 ? 0.56806474673115581f
+         : k == 3 ? 0.12053668025532300f
+         : k == 4 ? -0.35460488704253545f
+         : k == 5 ? -0.74851074817110120f
+         : k == 6 ? -0.97094181742605200f : 0.f;
+  }
+  __device__ static constexpr float s(int k) {
+    return k == 1 ? 0.46472317204376850f
+         : k == 2 ? 0.82298386589365640f
+         : k == 3 ? 0.99270887409805400f
+         : k == 4 ? 0.93501624268541480f
+         : k == 5 ? 0.66312265824079520f
+         : k == 6 ? 0.23931566428755768f : 0.f;
+  }
+};
+
 template <int R, bool INV>
 struct DftOdd {
   __device__ __forceinline__ static void run(float2* a) {
@@ -164,6 +185,7 @@ template <bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
 template <bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
 template <bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
 template <bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
+template <bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
 
 // Largest d <= cap dividing n (at least 1).
 __host__ __device__ constexpr int divisor_le(int n, int cap) {

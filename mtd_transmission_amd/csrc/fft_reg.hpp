@@ -91,7 +91,8 @@ __host__ __device__ constexpr int reg_radix(int P) {
        : (P % 3 == 0) ? 3
        : (P % 5 == 0) ? 5
        : (P % 7 == 0) ? 7
-       : (P % 11 == 0) ? 11 : 0;
+       : (P % 11 == 0) ? 11
+       : (P % 13 == 0) ? 13 : 0;
 }
 
 // In-register DFT of size P (in place, natural order in and out).

@@ -35,6 +35,16 @@ struct LsqPlan {
   int* d_perm = nullptr;      // row permutation of the pivoting: (Pb)[i] = b[perm[i]]
 };
 
+struct GenPlan;  // gen_fft.hpp
+
+// Tables of one transform length: the twiddles exp(-2 pi i k / L) of the
+// compiled register FFTs, and for a length a stage has no compiled FFT for,
+// the runtime-length plan (its device tables inside; held by the context).
+struct FftTab {
+  const float2* tw = nullptr;
+  const GenPlan* gen = nullptr;
+};
+
 struct Buffer {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -65,6 +75,10 @@ struct mtd_ctx {
   bool slot_used[2] = {false, false};
   std::string err;
   std::map<int, float2*> twiddles;                       // per FFT length
+  // runtime-length FFT plans per length (gen_fft.hpp) and their Bluestein
+  // tables (chirp, chirp spectrum)
+  std::map<int, mtd::GenPlan*> gen_plans;
+  std::map<int, std::pair<float2*, float2*>> bluestein;
   std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)
   std::map<std::pair<int, uint32_t>, float*> tukey_signed;
   std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
@@ -145,9 +159,37 @@ enum WsSlot {
 };
 
 int set_err(mtd_ctx* ctx, int code, const std::string& msg);
+// geometry and argument checks of every entry point; need: NEED_HOLO
+// (hologram side and crop window), NEED_VOL (volume side and optics)
+enum { NEED_HOLO = 1, NEED_VOL = 2 };
+int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need);
+size_t dtype_bytes(int dtype);
+// host copies of the device tables: twiddles exp(-2 pi i k / L), and the
+// hologram window (float)tukey1d(n, alpha)[i] * (-1)^i
+float2 twiddle_at(int L, int k);
+void twiddle_host(int L, float2* out);
+float tukey_signed_at(int n, double alpha, int i);
+void tukey_signed_host(int n, double alpha, float* out);
+// Hologram stage of holograms [a0, a0 + n) with the given tables and a row
+// intermediate t1 of n * holo_t1_bytes(g) bytes -> crops (and, when part_out
+// is set, the argmax partials).
+size_t holo_t1_bytes(const mtd_geom* g);
+int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int a0, int n,
+                    float2* crop_out, float2* part_out, const float* win, const FftTab& ftN,
+                    float2* t1);
 int hip_check(mtd_ctx* ctx, hipError_t e, const char* what);
 void* ws_get(mtd_ctx* ctx, int slot, size_t bytes);
 const float2* twiddle(mtd_ctx* ctx, int L);
+// FFT tables of length L; compiled: the stage has a register FFT of that
+// length (twiddles only), else the runtime-length plan is built too.
+// Returns false when out of memory or L is beyond the runtime-length FFT.
+bool fft_tab(mtd_ctx* ctx, int L, bool compiled, FftTab* out);
+bool holo_len_supported(int N);
+bool crop_len_supported(int C);
+bool unwrap_len_supported(int C);
+bool vol_len_supported(int V);
+// largest length the runtime-length FFT takes (direct: 4096, Bluestein: 2048)
+bool gen_len_ok(int L);
 const double* tukey_table(mtd_ctx* ctx, int n, double alpha);
 // (float)tukey1d[i] * (-1)^i
 const float* tukey_signed_table(mtd_ctx* ctx, int n, double alpha);
@@ -159,14 +201,18 @@ bool fft_len_supported(int L);
 // ---- kernel launchers (return hipError_t of the launch) -------------------
 // ws: float tukey1d(N, alpha_holo) with the centring sign (-1)^i folded in
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
-                            const float* ws, const float2* tw, int x0,
+                            const float* ws, const FftTab& ft, int x0,
                             int C, float2* t1, hipStream_t s);
 // part != nullptr: also writes per-block argmax partials of the crops
 // (holo_argmax_parts(N, C) per angle) for launch_argmax_fin
-hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
+hipError_t launch_holo_cols(int N, const float2* t1, int A, const FftTab& ft,
                             int C, int x0, int y0, float2* crop, float2* part,
                             hipStream_t s);
+// argmax partials per angle the column pass writes (0: none, the field
+// stage then finds the specular from the crops)
 int holo_argmax_parts(int N, int C);
+// row intermediate columns per hologram row (C rounded up to whole tiles)
+int holo_t1_cols(int N, int C);
 
 hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
                          int* posy, float2* part, hipStream_t s);
@@ -175,12 +221,12 @@ hipError_t launch_argmax_fin(int C, const float2* part, int S, int A, int* posx,
                              hipStream_t s);
 size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
-                        const int* posy, int A, const float2* tw,
+                        const int* posy, int A, const FftTab& ft,
                         float2* work, float2* iu, hipStream_t s);
 // The simulator's field synthesis: shift by -posSpec, unnormalised inverse
 // FFT, shift by C/2, times scale (delta_f^2): fields[A][C][C].
 hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, const int* posy,
-                                    int A, const float2* tw, float2* work, float scale,
+                                    int A, const FftTab& ft, float2* work, float scale,
                                     float2* fields, hipStream_t s);
 // phu: unwrapped phase [A][C][C] in the U frame, or nullptr for the wrapped
 // atan2 of IU (DEROUL 0)
@@ -194,8 +240,8 @@ hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
 // Volkov unwrapping of the wrapped phase of IU into phu; scratch buffers:
 // gx, gy [A][2C][2C] float, P [A][2C][2C] float2, Q [A][2C][C] float2;
 // tw2C: twiddles of length 2C.
-hipError_t launch_unwrap(int C, const float2* iu, int A, const float2* tw2C, float* gx, float* gy,
-                         float2* P, float2* Q, float* phu, hipStream_t s);
+hipError_t launch_unwrap(int C, const float2* iu, int A, const FftTab& ft2C, float* gx,
+                         float* gy, float2* P, float2* Q, float* phu, hipStream_t s);
 size_t unwrap_scratch_bytes(int C, int A);
 // Per-angle forward spectrum. from_iu=0: src are fields (U frame);
 // from_iu=1: src is IU and the field formation runs inside the load.
@@ -206,7 +252,7 @@ size_t unwrap_scratch_bytes(int C, int A);
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
                                int mode, const float* phu, const int* posx, const int* posy,
-                               int A, const double* win, const float2* tw,
+                               int A, const double* win, const FftTab& ft,
                                float dx2, int shift_pos, const float* beta2,
                                float2* work, float2* out, hipStream_t s);
 
@@ -236,7 +282,7 @@ size_t gather_scratch_bytes(int V, int A);
 // im may be nullptr) times scale (Tp^3 * gsign), stored only on the heights
 // of the device band.
 hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
-                                   float scale, const int2* band, const float2* tw,
+                                   float scale, const int2* band, const FftTab& ft,
                                    hipStream_t s);
 // One fused FISTA step with the smoothed TV gradient (k_tv.hip).
 hipError_t launch_stamp(unsigned long long* out, hipStream_t s);
@@ -259,6 +305,6 @@ hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, con
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
                            int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
-                           uint8_t* otf, const float2* tw, hipStream_t s);
+                           uint8_t* otf, const FftTab& ft, hipStream_t s);
 
 }  // namespace mtd

@@ -13,6 +13,7 @@
 //              forward 2-D FFT * pixUBorn^2, optional shift by +posSpec.
 #include "argmax.hpp"
 #include "dft.hpp"
+#include "gen_fft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
@@ -23,11 +24,11 @@ namespace mtd {
 // ---------------------------------------------------------------------------
 constexpr int ARGMAX_SEG = 16;  // blocks per crop in the first stage
 
-template <int C>
+template <int CT>
 __global__ void __launch_bounds__(256)
-k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
+k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part, Dim<CT> C) {
   const int a = blockIdx.y, seg = blockIdx.x;
-  constexpr int LEN = (C * C + ARGMAX_SEG - 1) / ARGMAX_SEG;
+  const int LEN = (C * C + ARGMAX_SEG - 1) / ARGMAX_SEG;
   const float2* p = crop + (size_t)a * C * C;
   float best = -1.f;
   int bi = 0x7fffffff;
@@ -42,9 +43,9 @@ k_argmax_part(const float2* __restrict__ crop, float2* __restrict__ part) {
 }
 
 // second stage: one thread per angle over its S partials
-template <int C>
+template <int CT>
 __global__ void k_argmax_fin(const float2* __restrict__ part, int S, int A, int* __restrict__ posx,
-                             int* __restrict__ posy) {
+                             int* __restrict__ posy, Dim<CT> C) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= A) return;
   float best = -1.f;
@@ -61,7 +62,7 @@ __global__ void k_argmax_fin(const float2* __restrict__ part, int S, int A, int*
 // ---------------------------------------------------------------------------
 // demodulation: rows then columns of the backward FFT
 // ---------------------------------------------------------------------------
-template <int C>
+template <int CT>
 struct DemodRowOp {
   const float2* crop;
   const int* posx;
@@ -76,9 +77,10 @@ struct DemodRowOp {
   __device__ void store(int a, int v, int u, float2 x) const {
     out[((size_t)a * C + v) * C + u] = x;
   }
+  Dim<CT> C{};
 };
 
-template <int C>
+template <int CT>
 struct ColInPlaceOp {  // column pass reading and writing one stack
   const float2* in;
   float2* out;
@@ -90,11 +92,12 @@ struct ColInPlaceOp {  // column pass reading and writing one stack
   __device__ void store(int a, int col, int row, float2 x) const {
     out[((size_t)a * C + row) * C + col] = x;
   }
+  Dim<CT> C{};
 };
 
 // Column pass of the simulator's field synthesis: the IU frame moved to the
 // centred field frame (fftshift2D, a shift by C/2) and scaled by delta_f^2.
-template <int C>
+template <int CT>
 struct ColCenterScaleOp {
   const float2* in;
   float2* out;
@@ -108,6 +111,7 @@ struct ColCenterScaleOp {
     out[((size_t)a * C + (row + C / 2) % C) * C + (col + C / 2) % C] =
         make_float2(x.x * scale, x.y * scale);
   }
+  Dim<CT> C{};
 };
 
 // ---------------------------------------------------------------------------
@@ -147,11 +151,11 @@ __device__ __forceinline__ void lu_solve15(const double* __restrict__ lu, double
   }
 }
 
-template <int C>
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_lsq(const float2* __restrict__ iu, const float* __restrict__ phu,
       const int2* __restrict__ samples, int M, const double* __restrict__ lu,
-      const int* __restrict__ perm, double* __restrict__ coef) {
+      const int* __restrict__ perm, double* __restrict__ coef, Dim<CT> C) {
   const int a = blockIdx.x;
   const float2* u = iu + (size_t)a * C * C;
   double rp[NCOEF], ra[NCOEF];
@@ -283,12 +287,12 @@ __device__ __forceinline__ float2 spec_norm_of(const float2* crop_a, int C,
   return make_float2((float)(re / mod), (float)(-im / mod));
 }
 
-template <int C>
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
              const int* __restrict__ posx, const int* __restrict__ posy,
              const double* __restrict__ coef, int mode, const float* __restrict__ phu,
-             float2* __restrict__ fields) {
+             float2* __restrict__ fields, Dim<CT> C) {
   const int a = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= C * C) return;
@@ -317,7 +321,7 @@ k_form_field(const float2* __restrict__ iu, const float2* __restrict__ crop,
 // per-angle forward spectrum
 // G[y][x] = field_U[(y+NX)%C][(x+NX)%C] * w[(y+NX)%C] * w[(x+NX)%C]
 // ---------------------------------------------------------------------------
-template <int C>
+template <int CT>
 struct FwdRowOp {
   const float2* src;      // fields (U frame) or IU
   const float2* crop;
@@ -374,9 +378,10 @@ struct FwdRowOp {
   __device__ void store(int a, int y, int x, float2 v) const {
     out[((size_t)a * C + y) * C + x] = v;
   }
+  Dim<CT> C{};
 };
 
-template <int C>
+template <int CT>
 struct FwdColOp {
   const float2* in;
   float2* out;
@@ -410,161 +415,169 @@ struct FwdColOp {
     }
     out[((size_t)a * C + (row + py) % C) * C + (col + px) % C] = s;
   }
+  Dim<CT> C{};
 };
 
 // ---------------------------------------------------------------------------
-// launchers
+// launchers: compiled crop sides (CT > 0, register FFTs) or any other side
+// (CT == 0, the runtime-length FFTs of gen_fft.hpp)
 // ---------------------------------------------------------------------------
-template <int C>
-static hipError_t argmax_c(const float2* crop, int A, int* posx, int* posy,
+template <int CT>
+static hipError_t argmax_c(Dim<CT> C, const float2* crop, int A, int* posx, int* posy,
                            float2* part, hipStream_t s) {
   {
     Timed tm("argmax", s);
-    hipLaunchKernelGGL((k_argmax_part<C>), dim3(ARGMAX_SEG, A), dim3(256), 0, s, crop, part);
+    hipLaunchKernelGGL((k_argmax_part<CT>), dim3(ARGMAX_SEG, A), dim3(256), 0, s, crop, part, C);
   }
   Timed tm("argmax_fin", s);
-  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, ARGMAX_SEG, A,
-                     posx, posy);
+  hipLaunchKernelGGL((k_argmax_fin<CT>), dim3((A + 63) / 64), dim3(64), 0, s, part, ARGMAX_SEG, A,
+                     posx, posy, C);
   return hipGetLastError();
 }
 
-template <int C>
-static hipError_t argmax_fin_c(const float2* part, int S, int A, int* posx, int* posy,
+template <int CT>
+static hipError_t argmax_fin_c(Dim<CT> C, const float2* part, int S, int A, int* posx, int* posy,
                                hipStream_t s) {
   Timed tm("argmax_fin", s);
-  hipLaunchKernelGGL((k_argmax_fin<C>), dim3((A + 63) / 64), dim3(64), 0, s, part, S, A, posx,
-                     posy);
+  hipLaunchKernelGGL((k_argmax_fin<CT>), dim3((A + 63) / 64), dim3(64), 0, s, part, S, A, posx,
+                     posy, C);
   return hipGetLastError();
 }
 
 size_t argmax_scratch_bytes(int A) { return (size_t)(A > 0 ? A : 1) * ARGMAX_SEG * sizeof(float2); }
 
-template <int C>
-static hipError_t demod_c(const float2* crop, const int* posx, const int* posy,
-                          int A, const float2* tw, float2* work, float2* iu,
-                          hipStream_t s) {
-  DemodRowOp<C> r{crop, posx, posy, work, tw, 0, 0};
-  {
-    Timed tm("demod_rows", s);
-    hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A),
-                       rows_block<C>(), 0, s, r);
+// one row pass and one column pass of op types R and K over A batches
+template <int CT, bool INV, class R, class K>
+static hipError_t rows_cols(Dim<CT> C, const R& r, const K& c, int A, const FftTab& ft,
+                            const char* rname, const char* cname, hipStream_t s) {
+  if constexpr (CT > 0) {
+    {
+      Timed tm(rname, s);
+      hipLaunchKernelGGL((k_rows<CT, INV, R>), rows_grid<CT>(A), rows_block<CT>(), 0, s, r);
+    }
+    Timed tm(cname, s);
+    hipLaunchKernelGGL((k_cols<CT, INV, K>), (cols_grid<CT, K>(A)), (cols_block<CT, K>()), 0, s,
+                       c);
+    return hipGetLastError();
+  } else {
+    if (!ft.gen) return hipErrorInvalidValue;
+    hipError_t e;
+    {
+      Timed tm(rname, s);
+      e = launch_gen<INV, true>(r, *ft.gen, C, A, s);
+    }
+    if (e != hipSuccess) return e;
+    Timed tm(cname, s);
+    return launch_gen<INV, false>(c, *ft.gen, C, A, s);
   }
-  ColInPlaceOp<C> c{work, iu, tw};
-  Timed tm("demod_cols", s);
-  hipLaunchKernelGGL((k_cols<C, true, ColInPlaceOp<C>>), cols_grid<C>(A), cols_block<C>(), 0, s,
-                     c);
-  return hipGetLastError();
 }
 
-template <int C>
-static hipError_t lsq_c(const float2* iu, const float* phu, int A, const LsqPlan& plan,
-                        double* coef, hipStream_t s) {
+template <int CT>
+static hipError_t demod_c(Dim<CT> C, const float2* crop, const int* posx, const int* posy,
+                          int A, const FftTab& ft, float2* work, float2* iu,
+                          hipStream_t s) {
+  DemodRowOp<CT> r{crop, posx, posy, work, ft.tw, 0, 0, C};
+  ColInPlaceOp<CT> c{work, iu, ft.tw, C};
+  return rows_cols<CT, true>(C, r, c, A, ft, "demod_rows", "demod_cols", s);
+}
+
+template <int CT>
+static hipError_t lsq_c(Dim<CT> C, const float2* iu, const float* phu, int A,
+                        const LsqPlan& plan, double* coef, hipStream_t s) {
   Timed tm("lsq", s);
-  hipLaunchKernelGGL((k_lsq<C>), dim3(A), dim3(256), 0, s, iu, phu, plan.d_samples,
-                     plan.M, plan.d_lu, plan.d_perm, coef);
+  hipLaunchKernelGGL((k_lsq<CT>), dim3(A), dim3(256), 0, s, iu, phu, plan.d_samples,
+                     plan.M, plan.d_lu, plan.d_perm, coef, C);
   return hipGetLastError();
 }
 
-template <int C>
-static hipError_t form_c(const float2* iu, const float2* crop, const int* posx,
+template <int CT>
+static hipError_t form_c(Dim<CT> C, const float2* iu, const float2* crop, const int* posx,
                          const int* posy, int A, const double* coef, int mode,
                          const float* phu, float2* fields, hipStream_t s) {
   Timed tm("form_field", s);
-  hipLaunchKernelGGL((k_form_field<C>), dim3((C * C + 255) / 256, A), dim3(256),
-                     0, s, iu, crop, posx, posy, coef, mode, phu, fields);
+  hipLaunchKernelGGL((k_form_field<CT>), dim3((C * C + 255) / 256, A), dim3(256),
+                     0, s, iu, crop, posx, posy, coef, mode, phu, fields, C);
   return hipGetLastError();
 }
 
-template <int C>
-static hipError_t fwd_c(const float2* src, int from_iu, const float2* crop,
+template <int CT>
+static hipError_t fwd_c(Dim<CT> C, const float2* src, int from_iu, const float2* crop,
                         const double* coef, int mode, const float* phu, const int* posx,
                         const int* posy, int A, const double* win,
-                        const float2* tw, float dx2, int shift_pos, const float* beta2,
+                        const FftTab& ft, float dx2, int shift_pos, const float* beta2,
                         float2* work, float2* out, hipStream_t s) {
-  FwdRowOp<C> r{src, crop, coef, posx, posy, win, work, tw, from_iu, mode, phu,
-                make_float2(0.f, 0.f), nullptr, nullptr};
-  {
-    Timed tm(from_iu ? "fwd_rows_field" : "fwd_rows", s);
-    hipLaunchKernelGGL((k_rows<C, false, FwdRowOp<C>>), rows_grid<C>(A),
-                       rows_block<C>(), 0, s, r);
-  }
-  FwdColOp<C> c{work, out, posx, posy, tw, dx2, shift_pos, beta2, 0, 0, 0, 0};
-  Timed tm("fwd_cols", s);
-  hipLaunchKernelGGL((k_cols<C, false, FwdColOp<C>>), cols_grid<C>(A), cols_block<C>(), 0, s,
-                     c);
-  return hipGetLastError();
+  FwdRowOp<CT> r{src, crop, coef, posx, posy, win, work, ft.tw, from_iu, mode, phu,
+                 make_float2(0.f, 0.f), nullptr, nullptr};
+  r.C = C;
+  FwdColOp<CT> c{work, out, posx, posy, ft.tw, dx2, shift_pos, beta2, 0, 0, 0, 0, C};
+  return rows_cols<CT, false>(C, r, c, A, ft, from_iu ? "fwd_rows_field" : "fwd_rows",
+                              "fwd_cols", s);
+}
+
+template <int CT>
+static hipError_t synth_fields_c(Dim<CT> C, const float2* spec, const int* posx, const int* posy,
+                                 int A, const FftTab& ft, float2* work, float scale,
+                                 float2* fields, hipStream_t s) {
+  DemodRowOp<CT> r{spec, posx, posy, work, ft.tw, 0, 0, C};
+  ColCenterScaleOp<CT> c{work, fields, ft.tw, scale, C};
+  return rows_cols<CT, true>(C, r, c, A, ft, "synth_rows", "synth_cols", s);
 }
 
 #define MTD_CROP_SIZES(X) \
-  X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(256) X(288) X(320) X(352) X(384) X(512)
+  X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(250) X(256) X(288) X(320) X(352) X(384) \
+  X(512)
 
 hipError_t launch_argmax(int C, const float2* crop, int A, int* posx,
                          int* posy, float2* part, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return argmax_c<n>(crop, A, posx, posy, part, s);
+#define X(n) case n: return argmax_c<n>(Dim<n>{}, crop, A, posx, posy, part, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return argmax_c<0>(Dim<0>(C), crop, A, posx, posy, part, s);
 }
 
 hipError_t launch_argmax_fin(int C, const float2* part, int S, int A, int* posx, int* posy,
                              hipStream_t s) {
   switch (C) {
-#define X(n) case n: return argmax_fin_c<n>(part, S, A, posx, posy, s);
+#define X(n) case n: return argmax_fin_c<n>(Dim<n>{}, part, S, A, posx, posy, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
-}
-
-template <int C>
-static hipError_t synth_fields_c(const float2* spec, const int* posx, const int* posy, int A,
-                                 const float2* tw, float2* work, float scale, float2* fields,
-                                 hipStream_t s) {
-  DemodRowOp<C> r{spec, posx, posy, work, tw, 0, 0};
-  {
-    Timed tm("synth_rows", s);
-    hipLaunchKernelGGL((k_rows<C, true, DemodRowOp<C>>), rows_grid<C>(A), rows_block<C>(), 0,
-                       s, r);
-  }
-  ColCenterScaleOp<C> c{work, fields, tw, scale};
-  Timed tm("synth_cols", s);
-  hipLaunchKernelGGL((k_cols<C, true, ColCenterScaleOp<C>>), cols_grid<C>(A), cols_block<C>(),
-                     0, s, c);
-  return hipGetLastError();
+  return argmax_fin_c<0>(Dim<0>(C), part, S, A, posx, posy, s);
 }
 
 hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, const int* posy,
-                                    int A, const float2* tw, float2* work, float scale,
+                                    int A, const FftTab& ft, float2* work, float scale,
                                     float2* fields, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return synth_fields_c<n>(spec, posx, posy, A, tw, work, scale, fields, s);
+#define X(n) \
+  case n: return synth_fields_c<n>(Dim<n>{}, spec, posx, posy, A, ft, work, scale, fields, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return synth_fields_c<0>(Dim<0>(C), spec, posx, posy, A, ft, work, scale, fields, s);
 }
 
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
-                        const int* posy, int A, const float2* tw,
+                        const int* posy, int A, const FftTab& ft,
                         float2* work, float2* iu, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return demod_c<n>(crop, posx, posy, A, tw, work, iu, s);
+#define X(n) case n: return demod_c<n>(Dim<n>{}, crop, posx, posy, A, ft, work, iu, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return demod_c<0>(Dim<0>(C), crop, posx, posy, A, ft, work, iu, s);
 }
 
 hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,
                       double* coef, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return lsq_c<n>(iu, phu, A, plan, coef, s);
+#define X(n) case n: return lsq_c<n>(Dim<n>{}, iu, phu, A, plan, coef, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return lsq_c<0>(Dim<0>(C), iu, phu, A, plan, coef, s);
 }
 
 hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
@@ -572,28 +585,30 @@ hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
                              const double* coef, int mode, const float* phu, float2* fields,
                              hipStream_t s) {
   switch (C) {
-#define X(n) case n: return form_c<n>(iu, crop, posx, posy, A, coef, mode, phu, fields, s);
+#define X(n) \
+  case n: return form_c<n>(Dim<n>{}, iu, crop, posx, posy, A, coef, mode, phu, fields, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return form_c<0>(Dim<0>(C), iu, crop, posx, posy, A, coef, mode, phu, fields, s);
 }
 
 hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
                                const float2* crop, const double* coef,
                                int mode, const float* phu, const int* posx, const int* posy,
-                               int A, const double* win, const float2* tw,
+                               int A, const double* win, const FftTab& ft,
                                float dx2, int shift_pos, const float* beta2,
                                float2* work, float2* out, hipStream_t s) {
   switch (C) {
-#define X(n)                                                                         \
-  case n:                                                                            \
-    return fwd_c<n>(src, from_iu, crop, coef, mode, phu, posx, posy, A, win, tw, dx2, \
-                    shift_pos, beta2, work, out, s);
+#define X(n)                                                                          \
+  case n:                                                                             \
+    return fwd_c<n>(Dim<n>{}, src, from_iu, crop, coef, mode, phu, posx, posy, A, win, \
+                    ft, dx2, shift_pos, beta2, work, out, s);
     MTD_CROP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return fwd_c<0>(Dim<0>(C), src, from_iu, crop, coef, mode, phu, posx, posy, A, win, ft, dx2,
+                  shift_pos, beta2, work, out, s);
 }
 
 bool crop_len_supported(int C) {

@@ -16,6 +16,7 @@
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
 #include "argmax.hpp"
 #include "fft_reg.hpp"
+#include "gen_fft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
@@ -53,7 +54,7 @@ constexpr int holo_rows_waves() {
 template <int N, typename TIn>
 __global__ void __launch_bounds__(256, holo_rows_waves<N>())
 k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
-            const float2* __restrict__ tw, int x0, int C, int lw,
+            const float2* __restrict__ tw, int x0, int C, int cp, int lw,
             float2* __restrict__ t1) {
   using F = Fft2<N, false>;
   constexpr int NP = holo_pairs<N>();
@@ -88,7 +89,7 @@ k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
   // Bin d of row y0 sits at q0[tile(d)], of row y1 at q1[tile(d)].
   // W = 2^lw: shifts and masks, no integer division in the store loop
   const int W = 1 << lw;
-  float2* q0 = t1 + (size_t)a * N * C + (size_t)y0 * W;
+  float2* q0 = t1 + (size_t)a * N * cp + (size_t)y0 * W;
   float2* q1 = q0 + W;
   auto tile = [lw, W](int d) { return (((d >> lw) * N) << lw) + (d & (W - 1)); };
   constexpr int DK = (N / 2 + F::T - 1) / F::T;
@@ -154,7 +155,7 @@ constexpr int holo_cols_waves() {
 template <int N>
 __global__ void __launch_bounds__(holo_cols_threads<N>(), holo_cols_waves<N>())
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
-            int C, int W, int x0, int y0, float scale, float2* __restrict__ crop,
+            int C, int cp, int W, int x0, int y0, float scale, float2* __restrict__ crop,
             float2* __restrict__ part) {
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
@@ -164,7 +165,7 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   const int c = threadIdx.x % NC, n2 = threadIdx.x / NC;
   // this block's contiguous N x NC tile through a buffer view: one lane
   // offset, uniform row offsets
-  const BufView src(t1 + (size_t)a * N * C + (size_t)(d0 / W) * N * W + d0 % W,
+  const BufView src(t1 + (size_t)a * N * cp + (size_t)(d0 / W) * N * W + d0 % W,
                     (unsigned)((N * W - d0 % W) * sizeof(float2)));
   const unsigned lane = (unsigned)((n2 * W + c) * sizeof(float2));
   float2 v[F::L1], u[F::L2];
@@ -174,9 +175,11 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   F::run(v, u, n2, typename F::template Col<NC>{lds + c}, tw);
   // argmax partial of the crop values this block writes (coordSpec's
   // first stage, fused: the crop is not read again for it)
+  // lanes past the crop's last column (a crop side that is not a multiple
+  // of the block width) transformed padding: nothing of theirs is kept
   float best = -1.f;
   int bi = 0x7fffffff;
-  if (n2 < F::L1) {
+  if (n2 < F::L1 && d0 + c < C) {
     float2* dst = crop + (size_t)a * C * C + d0 + c;
     const int kx = x0 + d0 + c;
 #pragma unroll
@@ -197,8 +200,16 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
     __shared__ float red[2 * (holo_cols_threads<N>() / 64)];
     argmax_block(best, bi, red);
     if (threadIdx.x == 0)
-      part[(size_t)a * (C / NC) + ab.y] = make_float2(best, __int_as_float(bi));
+      part[(size_t)a * ((C + NC - 1) / NC) + ab.y] = make_float2(best, __int_as_float(bi));
   }
+}
+
+// columns of t1 per hologram row: C rounded up to whole tiles of W (a
+// multiple of the column block width)
+template <int N>
+static int t1_cols_n(int C) {
+  const int W = holo_tile_w(C, holo_cols_nc<N>());
+  return (C + W - 1) / W * W;
 }
 
 template <int N>
@@ -206,8 +217,8 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
                          const float2* tw, int x0, int C, float2* t1,
                          hipStream_t s) {
   constexpr int NP = holo_pairs<N>();
-  if (C % holo_cols_nc<N>()) return hipErrorInvalidValue;  // whole column tiles
   const int W = holo_tile_w(C, holo_cols_nc<N>());
+  const int cp = t1_cols_n<N>(C);
   int lw = 0;
   while ((1 << lw) < W) ++lw;  // W is a power of two
   dim3 grid(N / (2 * NP), A), block(NP * Fft2<N, false>::T);
@@ -215,15 +226,15 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
   switch (dtype) {
     case MTD_U8:
       hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
-                         (const uint8_t*)holos, win, tw, x0, C, lw, t1);
+                         (const uint8_t*)holos, win, tw, x0, C, cp, lw, t1);
       break;
     case MTD_U16:
       hipLaunchKernelGGL((k_holo_rows<N, uint16_t>), grid, block, 0, s,
-                         (const uint16_t*)holos, win, tw, x0, C, lw, t1);
+                         (const uint16_t*)holos, win, tw, x0, C, cp, lw, t1);
       break;
     default:
       hipLaunchKernelGGL((k_holo_rows<N, float>), grid, block, 0, s,
-                         (const float*)holos, win, tw, x0, C, lw, t1);
+                         (const float*)holos, win, tw, x0, C, cp, lw, t1);
       break;
   }
   return hipGetLastError();
@@ -233,45 +244,111 @@ template <int N>
 static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
                          int x0, int y0, float2* crop, float2* part, hipStream_t s) {
   constexpr int NC = holo_cols_nc<N>();
-  if (C % NC) return hipErrorInvalidValue;
   const float scale = (float)(0.5 / ((double)N * (double)N));
   Timed tm("holo_cols", s);
-  hipLaunchKernelGGL((k_holo_cols<N>), dim3(C / NC, A), dim3(holo_cols_threads<N>()), 0, s,
-                     t1, tw, C, holo_tile_w(C, NC), x0, y0, scale, crop, part);
+  hipLaunchKernelGGL((k_holo_cols<N>), dim3((C + NC - 1) / NC, A), dim3(holo_cols_threads<N>()),
+                     0, s, t1, tw, C, t1_cols_n<N>(C), holo_tile_w(C, NC), x0, y0, scale, crop,
+                     part);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Hologram sides without a compiled FFT: one complex transform per row of
+// the windowed, sign-modulated hologram (its C kept bins into t1[a][y][d],
+// untiled), then one per kept column (its C kept bins, / N^2 and signed).
+// The specular is then found from the crops (no argmax partials).
+// ---------------------------------------------------------------------------
+template <typename TIn>
+struct HoloRowGenOp {
+  const TIn* holos;
+  const float* ws;  // (float)tukey1d(N)[i] * (-1)^i
+  float2* t1;
+  int N, C, x0;
+  __device__ void begin(int) {}
+  __device__ float2 load(int a, int y, int x) const {
+    return make_float2((float)holos[((size_t)a * N + y) * N + x] * (ws[x] * ws[y]), 0.f);
+  }
+  __device__ void store(int a, int y, int k, float2 v) const {
+    const int d = k - x0;
+    if (d >= 0 && d < C) t1[((size_t)a * N + y) * C + d] = v;
+  }
+};
+
+struct HoloColGenOp {
+  const float2* t1;
+  float2* crop;
+  int N, C, x0, y0;
+  float scale;
+  __device__ void begin(int) {}
+  __device__ float2 load(int a, int d, int y) const { return t1[((size_t)a * N + y) * C + d]; }
+  __device__ void store(int a, int d, int ky, float2 v) const {
+    const int e = ky - y0;
+    if (e < 0 || e >= C) return;
+    const float sc = ((x0 + d + ky) & 1) ? -scale : scale;
+    crop[((size_t)a * C + e) * C + d] = make_float2(v.x * sc, v.y * sc);
+  }
+};
+
+static hipError_t rows_gen(int N, int dtype, const void* holos, int A, const float* win,
+                           const GenPlan& p, int x0, int C, float2* t1, hipStream_t s) {
+  Timed tm("holo_rows", s);
+  switch (dtype) {
+    case MTD_U8:
+      return launch_gen<false, true>(HoloRowGenOp<uint8_t>{(const uint8_t*)holos, win, t1, N, C, x0},
+                                     p, N, A, s);
+    case MTD_U16:
+      return launch_gen<false, true>(
+          HoloRowGenOp<uint16_t>{(const uint16_t*)holos, win, t1, N, C, x0}, p, N, A, s);
+    default:
+      return launch_gen<false, true>(HoloRowGenOp<float>{(const float*)holos, win, t1, N, C, x0},
+                                     p, N, A, s);
+  }
 }
 
 #define MTD_HOLO_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
-                            const float* win, const float2* tw, int x0,
+                            const float* win, const FftTab& ft, int x0,
                             int C, float2* t1, hipStream_t s) {
   switch (N) {
-#define X(n) case n: return rows_n<n>(dtype, holos, A, win, tw, x0, C, t1, s);
+#define X(n) case n: return rows_n<n>(dtype, holos, A, win, ft.tw, x0, C, t1, s);
     MTD_HOLO_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  if (!ft.gen) return hipErrorInvalidValue;
+  return rows_gen(N, dtype, holos, A, win, *ft.gen, x0, C, t1, s);
 }
 
-hipError_t launch_holo_cols(int N, const float2* t1, int A, const float2* tw,
+hipError_t launch_holo_cols(int N, const float2* t1, int A, const FftTab& ft,
                             int C, int x0, int y0, float2* crop, float2* part,
                             hipStream_t s) {
   switch (N) {
-#define X(n) case n: return cols_n<n>(t1, A, tw, C, x0, y0, crop, part, s);
+#define X(n) case n: return cols_n<n>(t1, A, ft.tw, C, x0, y0, crop, part, s);
     MTD_HOLO_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  if (!ft.gen) return hipErrorInvalidValue;
+  Timed tm("holo_cols", s);
+  const float scale = (float)(1.0 / ((double)N * (double)N));
+  return launch_gen<false, false>(HoloColGenOp{t1, crop, N, C, x0, y0, scale}, *ft.gen, C, A, s);
 }
 
 int holo_argmax_parts(int N, int C) {
   switch (N) {
-#define X(n) case n: return C / holo_cols_nc<n>();
+#define X(n) case n: return (C + holo_cols_nc<n>() - 1) / holo_cols_nc<n>();
     MTD_HOLO_SIZES(X)
 #undef X
   }
   return 0;
+}
+
+int holo_t1_cols(int N, int C) {
+  switch (N) {
+#define X(n) case n: return t1_cols_n<n>(C);
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return C;
 }
 
 bool holo_len_supported(int N) {

@@ -23,29 +23,40 @@
 // The reference's zero frequency reads one element past its array (:189);
 // it only adds a constant that the phase fit always removes; here it is 0.
 #include "dft.hpp"
+#include "gen_fft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
 namespace mtd {
 
 // source index in [0, C) of position i in [0, 2C) of the even extension
+__device__ __forceinline__ int mirror_src(int i, int C) {
+  return i < C ? C - 1 - i : i - C;
+}
 template <int C>
 __device__ __forceinline__ int mirror_src(int i) {
-  return i < C ? C - 1 - i : i - C;
+  return mirror_src(i, C);
 }
 
 // wrapped phase of U[y][x], U held in the IU frame
-template <int C>
-__device__ __forceinline__ float wrapped_phase(const float2* __restrict__ iu_a, int y, int x) {
+__device__ __forceinline__ float wrapped_phase(const float2* __restrict__ iu_a, int y, int x,
+                                               int C) {
   const float2 u = iu_a[((y + C / 2) % C) * C + (x + C / 2) % C];
   return atan2f(u.y, u.x);
+}
+template <int C>
+__device__ __forceinline__ float wrapped_phase(const float2* __restrict__ iu_a, int y, int x) {
+  return wrapped_phase(iu_a, y, x, C);
 }
 
 // frequency in cycles per pixel of FFT index k (init_kvect_shift,
 // FFT_fonctions.cpp:7-17; the Nyquist index gets -1/2)
+__device__ __forceinline__ float kfreq(int k, int D) {
+  return (float)(((k + D / 2) % D) - D / 2) / (float)D;
+}
 template <int D>
 __device__ __forceinline__ float kfreq(int k) {
-  return (float)(((k + D / 2) % D) - D / 2) / (float)D;
+  return kfreq(k, D);
 }
 
 // d/dx of one line held in the two-step layout: v (input layout) in, u
@@ -180,7 +191,7 @@ k_grad_cols(const float2* __restrict__ iu, const float2* __restrict__ tw, float*
 }
 
 // integration pass 1: rows of gx + i gy, forward
-template <int D>
+template <int DT>
 struct PackRowOp {
   const float* gx;
   const float* gy;
@@ -192,23 +203,25 @@ struct PackRowOp {
     return make_float2(gx[i], gy[i]);
   }
   __device__ void store(int a, int r, int k, float2 v) const { out[((size_t)a * D + r) * D + k] = v; }
+  Dim<DT> D{};
 };
 
 // integration pass 2: columns, forward, in place
-template <int D>
+template <int DT>
 struct InPlaceColOp {
   float2* buf;
   const float2* tw;
   __device__ void begin(int) {}
   __device__ float2 load(int a, int c, int r) const { return buf[((size_t)a * D + r) * D + c]; }
   __device__ void store(int a, int c, int r, float2 v) const { buf[((size_t)a * D + r) * D + c] = v; }
+  Dim<DT> D{};
 };
 
 // integration pass 3: P = FFT2(gx + i gy) (unnormalised). Per frequency,
 // TFx = (P[k] + conj(P[-k])) / (2 D^2), TFy = (P[k] - conj(P[-k])) / (2i D^2),
 // T = -i (TFx kx + TFy ky) / (2 pi |k|^2), T[0] = 0; backward along rows,
 // keeping x in [C, 2C): Q[a][ky][x - C].
-template <int D>
+template <int DT>
 struct PoissonRowOp {
   const float2* P;
   float2* Q;
@@ -223,20 +236,21 @@ struct PoissonRowOp {
     // TFx = (p + conj(m)) * inv, TFy = (p - conj(m)) * inv / i
     const float2 fx = make_float2((p.x + m.x) * inv, (p.y - m.y) * inv);
     const float2 fy = make_float2((p.y + m.y) * inv, -(p.x - m.x) * inv);
-    const float kxf = kfreq<D>(kx), kyf = kfreq<D>(ky);
+    const float kxf = kfreq(kx, D), kyf = kfreq(ky, D);
     const float2 num = make_float2(fx.x * kxf + fy.x * kyf, fx.y * kxf + fy.y * kyf);
     const float den = (float)(2.0 * 3.14159265358979323846) * (kxf * kxf + kyf * kyf);
     return make_float2(num.y / den, -num.x / den);  // -i num / den
   }
   __device__ void store(int a, int ky, int x, float2 v) const {
-    constexpr int C = D / 2;
+    const int C = D / 2;
     if (x >= C) Q[((size_t)a * D + ky) * C + (x - C)] = v;
   }
+  Dim<DT> D{};
 };
 
 // integration pass 4: backward along columns of Q, keeping y in [C, 2C);
 // unwrapped phase = phi + 2 pi Re(.), in the U frame: phu[a][y][x]
-template <int D>
+template <int DT>
 struct UnwrapColOp {
   const float2* Q;
   const float2* iu;
@@ -244,17 +258,18 @@ struct UnwrapColOp {
   const float2* tw;
   __device__ void begin(int) {}
   __device__ float2 load(int a, int c, int ky) const {
-    constexpr int C = D / 2;
+    const int C = D / 2;
     if (c >= C) return make_float2(0.f, 0.f);
     return Q[((size_t)a * D + ky) * C + c];
   }
   __device__ void store(int a, int c, int y, float2 v) const {
-    constexpr int C = D / 2;
+    const int C = D / 2;
     if (c >= C || y < C) return;
     const int yy = y - C;
-    const float ph = wrapped_phase<C>(iu + (size_t)a * C * C, yy, c);
+    const float ph = wrapped_phase(iu + (size_t)a * C * C, yy, c, C);
     phu[((size_t)a * C + yy) * C + c] = ph + (float)(2.0 * 3.14159265358979323846) * v.x;
   }
+  Dim<DT> D{};
 };
 
 template <int C>
@@ -288,17 +303,128 @@ static hipError_t unwrap_c(const float2* iu, int A, const float2* tw, float* gx,
   return hipGetLastError();
 }
 
-#define MTD_UNWRAP_SIZES(X) \
-  X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(256) X(288) X(320) X(352) X(384) X(512)
+// ---------------------------------------------------------------------------
+// Crop sides without compiled 2C-point FFTs: the same four steps with the
+// runtime-length FFTs. The gradient kernel holds nl lines of the extension
+// in LDS and runs both derivatives there (forward transform, times
+// 2 pi i k / D^2, backward transform); the first derivative's real part goes
+// to the output array and is read back by the same thread for the second.
+// ---------------------------------------------------------------------------
+template <bool ROWS>
+__global__ void __launch_bounds__(kGenThreads)
+k_grad_gen(const float2* __restrict__ iu, GenPlan p, int C, float* __restrict__ g, int nl) {
+  extern __shared__ float2 gen_lds[];
+  const int D = 2 * C;
+  const int a = blockIdx.y, line0 = blockIdx.x * nl;
+  const int nact = min(nl, D - line0);
+  float2* A = gen_lds;
+  float2* B = gen_lds + (size_t)nl * p.W;
+  const float2* iu_a = iu + (size_t)a * C * C;
+  const float two_pi = (float)(2.0 * 3.14159265358979323846);
+  auto at = [&](int t, int& l, int& i) {
+    if (ROWS) { l = t / D; i = t - l * D; } else { i = t / nl; l = t - i * nl; }
+  };
+  auto phase = [&](int l, int i) {
+    const int line = line0 + l;
+    return ROWS ? wrapped_phase(iu_a, mirror_src(line, C), mirror_src(i, C), C)
+                : wrapped_phase(iu_a, mirror_src(i, C), mirror_src(line, C), C);
+  };
+  auto out = [&](int l, int i) -> size_t {
+    return ROWS ? ((size_t)a * D + line0 + l) * D + i : ((size_t)a * D + i) * D + line0 + l;
+  };
+  for (int pass = 0; pass < 2; ++pass) {  // 0: phi_s, 1: Z = exp(i phi_s)
+    for (int t = threadIdx.x; t < nl * D; t += blockDim.x) {
+      int l, i;
+      at(t, l, i);
+      float2 v = make_float2(0.f, 0.f);
+      if (l < nact) {
+        const float ph = phase(l, i);
+        if (pass == 0) {
+          v.x = ph;
+        } else {
+          sincosf(ph, &v.y, &v.x);
+        }
+      }
+      A[l * p.W + i] = v;
+    }
+    __syncthreads();
+    float2* R = gen_fft<false>(A, B, nl, p);
+    for (int t = threadIdx.x; t < nl * D; t += blockDim.x) {
+      const int l = t / D, k = t - l * D;
+      const float f = two_pi * kfreq(k, D) / (float)D;
+      const float2 u = R[l * p.W + k];
+      R[l * p.W + k] = make_float2(-f * u.y, f * u.x);
+    }
+    __syncthreads();
+    const float2* R2 = gen_fft<true>(R, R == A ? B : A, nl, p);
+    for (int t = threadIdx.x; t < nl * D; t += blockDim.x) {
+      int l, i;
+      at(t, l, i);
+      if (l >= nact) continue;
+      if (pass == 0) {
+        g[out(l, i)] = R2[l * p.W + i].x;  // d phi_s
+      } else {
+        float zi, zr;
+        sincosf(phase(l, i), &zi, &zr);
+        const float2 dz = R2[l * p.W + i];
+        const float im_q = (dz.y * zr - dz.x * zi) / (zr * zr + zi * zi);
+        const size_t o = out(l, i);
+        g[o] = (im_q - g[o]) / two_pi;
+      }
+    }
+    __syncthreads();  // A and B are reused
+  }
+}
 
-hipError_t launch_unwrap(int C, const float2* iu, int A, const float2* tw2C, float* gx, float* gy,
-                         float2* P, float2* Q, float* phu, hipStream_t s) {
+static hipError_t unwrap_gen(int C, const float2* iu, int A, const GenPlan& p, float* gx,
+                             float* gy, float2* P, float2* Q, float* phu, hipStream_t s) {
+  const int D = 2 * C;
+  const Dim<0> d(D);
+  {
+    Timed tm("unwrap_grad", s);
+    const int nl = gen_lines_per_block(p.W);
+    const size_t lds = (size_t)2 * nl * p.W * sizeof(float2);
+    const dim3 grid((D + nl - 1) / nl, A);
+    hipLaunchKernelGGL((k_grad_gen<true>), grid, dim3(kGenThreads), lds, s, iu, p, C, gx, nl);
+    hipLaunchKernelGGL((k_grad_gen<false>), grid, dim3(kGenThreads), lds, s, iu, p, C, gy, nl);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e;
+  {
+    Timed tm("unwrap_fwd", s);
+    e = launch_gen<false, true>(PackRowOp<0>{gx, gy, P, nullptr, d}, p, D, A, s);
+    if (e == hipSuccess) e = launch_gen<false, false>(InPlaceColOp<0>{P, nullptr, d}, p, D, A, s);
+  }
+  if (e != hipSuccess) return e;
+  Timed tm("unwrap_inv", s);
+  e = launch_gen<true, true>(PoissonRowOp<0>{P, Q, nullptr, d}, p, D, A, s);
+  if (e != hipSuccess) return e;
+  return launch_gen<true, false>(UnwrapColOp<0>{Q, iu, phu, nullptr, d}, p, C, A, s);
+}
+
+#define MTD_UNWRAP_SIZES(X) \
+  X(16) X(32) X(64) X(128) X(160) X(192) X(224) X(250) X(256) X(288) X(320) X(352) X(384) \
+  X(512)
+
+hipError_t launch_unwrap(int C, const float2* iu, int A, const FftTab& ft2C, float* gx,
+                         float* gy, float2* P, float2* Q, float* phu, hipStream_t s) {
   switch (C) {
-#define X(n) case n: return unwrap_c<n>(iu, A, tw2C, gx, gy, P, Q, phu, s);
+#define X(n) case n: return unwrap_c<n>(iu, A, ft2C.tw, gx, gy, P, Q, phu, s);
     MTD_UNWRAP_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  if (!ft2C.gen) return hipErrorInvalidValue;
+  return unwrap_gen(C, iu, A, *ft2C.gen, gx, gy, P, Q, phu, s);
+}
+
+bool unwrap_len_supported(int C) {
+  switch (C) {
+#define X(n) case n: return true;
+    MTD_UNWRAP_SIZES(X)
+#undef X
+  }
+  return false;
 }
 
 size_t unwrap_scratch_bytes(int C, int A) {

@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include "dft.hpp"
+#include "gen_fft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
 
@@ -264,6 +265,10 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
   const int qx = X - g.V / 2, qy = Y - g.V / 2;
   const size_t plane = (size_t)g.V * g.V;
   const size_t col = (size_t)Y * g.V + X;
+  // the last tiles of a volume side that is not a multiple of the tile side
+  // hold columns past the volume: they walk the lists like the others (the
+  // ballots are per wave) but write nothing
+  const bool inb = X < g.V && Y < g.V;
   const int z0 = w * ZW, z1 = min(z0 + ZW, g.V) - 1;
   if (band) {  // banded output: windows outside the hit band are not written
     const int2 b = *band;
@@ -272,9 +277,11 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
   const int4 info = tinfo[tile];
   const bool norm = flags & 1, sgn = flags & 5;
   if (info.x == 0 || z1 < info.y || z0 > info.z) {
-    for (int z = z0; z <= z1; ++z) {
-      g.vol[z * plane + col] = make_float2(0.f, 0.f);
-      if (!sgn) g.cnt[z * plane + col] = 0;
+    if (inb) {
+      for (int z = z0; z <= z1; ++z) {
+        g.vol[z * plane + col] = make_float2(0.f, 0.f);
+        if (!sgn) g.cnt[z * plane + col] = 0;
+      }
     }
     return;
   }
@@ -360,6 +367,7 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
       }
     }
   }
+  if (!inb) return;
 #pragma unroll
   for (int k = 0; k < ZW; ++k) {
     const int z = z0 + k;
@@ -464,7 +472,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
   (void)posx;
   (void)posy;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
-  const int ntx = V / GT, tiles = ntx * ntx;
+  const int ntx = (V + GT - 1) / GT, tiles = ntx * ntx;
   // scratch: [tiles][A] entries | [tiles][A][256] height offsets | [tiles] info
   const size_t nA = (size_t)(A > 0 ? A : 1);
   TileEntry* list = (TileEntry*)scratch;
@@ -488,7 +496,7 @@ hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
 hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
                                  const double* htab, int r2max, void* scratch, hipStream_t s) {
   GatherArgs g{nullptr, 0, ageo, A, C, nxmax, V, htab, nullptr, r2max, nullptr, nullptr};
-  const int ntx = V / GT, tiles = ntx * ntx;
+  const int ntx = (V + GT - 1) / GT, tiles = ntx * ntx;
   const size_t nA = (size_t)(A > 0 ? A : 1);
   TileEntry* list = (TileEntry*)scratch;
   uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
@@ -500,7 +508,7 @@ hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, 
 }
 
 size_t gather_scratch_bytes(int V, int A) {
-  const size_t tiles = (size_t)(V / GT) * (V / GT);
+  const size_t tiles = (size_t)((V + GT - 1) / GT) * ((V + GT - 1) / GT);
   const size_t nA = (size_t)(A > 0 ? A : 1);
   return tiles * nA * (sizeof(TileEntry) + 256) + tiles * sizeof(int4);  // 16 + 256 B per entry
 }
@@ -515,7 +523,7 @@ __device__ __forceinline__ bool outside_band(const int2* band, int z) {
   return z < b.x || z > b.y;
 }
 
-template <int V>
+template <int VT>
 struct NormXOp {  // rows along x; a = z, line = y
   const float2* vol;
   const int32_t* cnt;
@@ -544,9 +552,10 @@ struct NormXOp {  // rows along x; a = z, line = y
   __device__ void store(int z, int y, int x, float2 v) const {
     out[((size_t)z * V + y) * V + x] = v;
   }
+  Dim<VT> V{};
 };
 
-template <int V>
+template <int VT>
 struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
   static constexpr int kColThreads = 512;  // complex in and out
   float2* buf;
@@ -556,9 +565,10 @@ struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
   __device__ void begin(int) {}
   __device__ float2 load(int z, int x, int y) const { return buf[((size_t)z * V + y) * V + x]; }
   __device__ void store(int z, int x, int y, float2 v) const { buf[((size_t)z * V + y) * V + x] = v; }
+  Dim<VT> V{};
 };
 
-template <int V>
+template <int VT>
 struct ZOutOp {  // lines along z; a = y, col = x, row = z
   const float2* buf;
   float* re;
@@ -587,30 +597,56 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
     re[i] = (v.x * sg * df3) * cte;
     if (im) im[i] = (v.y * sg * df3) * cte;  // uniform: NULL skips the imaginary part
   }
+  Dim<VT> V{};
 };
 
-template <int V>
-static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
+// one row pass and two column passes over V batches: compiled register FFTs
+// (VT > 0) or the runtime-length FFTs (VT == 0)
+template <int VT, bool INV, class RX, class CY, class CZ>
+static hipError_t vol_passes(Dim<VT> V, const RX& x, const CY& y, const CZ& z, const FftTab& ft,
+                             const char* nx, const char* ny, const char* nz, hipStream_t s) {
+  if constexpr (VT > 0) {
+    {
+      Timed tm(nx, s);
+      hipLaunchKernelGGL((k_rows<VT, INV, RX>), rows_grid<VT>(VT), rows_block<VT>(), 0, s, x);
+    }
+    {
+      Timed tm(ny, s);
+      hipLaunchKernelGGL((k_cols<VT, INV, CY>), (cols_grid<VT, CY>(VT)), (cols_block<VT, CY>()),
+                         0, s, y);
+    }
+    Timed tm(nz, s);
+    hipLaunchKernelGGL((k_cols<VT, INV, CZ>), (cols_grid<VT, CZ>(VT)), (cols_block<VT, CZ>()), 0,
+                       s, z);
+    return hipGetLastError();
+  } else {
+    if (!ft.gen) return hipErrorInvalidValue;
+    hipError_t e;
+    {
+      Timed tm(nx, s);
+      e = launch_gen<INV, true>(x, *ft.gen, V, V, s);
+    }
+    if (e != hipSuccess) return e;
+    {
+      Timed tm(ny, s);
+      e = launch_gen<INV, false>(y, *ft.gen, V, V, s);
+    }
+    if (e != hipSuccess) return e;
+    Timed tm(nz, s);
+    return launch_gen<INV, false>(z, *ft.gen, V, V, s);
+  }
+}
+
+template <int VT>
+static hipError_t finalize_v(Dim<VT> V, const float2* vol, const int32_t* cnt, int prenorm,
                              const int2* band, float2* work,
                              float df3, float cte, float gsign, float* re,
-                             float* im, uint8_t* otf, const float2* tw,
+                             float* im, uint8_t* otf, const FftTab& ft,
                              hipStream_t s) {
-  NormXOp<V> x{vol, cnt, work, otf, tw, prenorm, band};
-  {
-    Timed tm("vol_x", s);
-    hipLaunchKernelGGL((k_rows<V, true, NormXOp<V>>), rows_grid<V>(V), rows_block<V>(),
-                       0, s, x);
-  }
-  PlaneYOp<V> y{work, tw, band};
-  {
-    Timed tm("vol_y", s);
-    hipLaunchKernelGGL((k_cols<V, true, PlaneYOp<V>>), (cols_grid<V, PlaneYOp<V>>(V)),
-                       (cols_block<V, PlaneYOp<V>>()), 0, s, y);
-  }
-  ZOutOp<V> z{work, re, im, tw, df3, cte, gsign, band, 0, V - 1};
-  Timed tm("vol_z", s);
-  hipLaunchKernelGGL((k_cols<V, true, ZOutOp<V>>), cols_grid<V>(V), cols_block<V>(), 0, s, z);
-  return hipGetLastError();
+  NormXOp<VT> x{vol, cnt, work, otf, ft.tw, prenorm, band, V};
+  PlaneYOp<VT> y{work, ft.tw, band, V};
+  ZOutOp<VT> z{work, re, im, ft.tw, df3, cte, gsign, band, 0, V - 1, V};
+  return vol_passes<VT, true>(V, x, y, z, ft, "vol_x", "vol_y", "vol_z", s);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,7 +658,7 @@ static hipError_t finalize_v(const float2* vol, const int32_t* cnt, int prenorm,
 // output ones times (-1)^(3V/2) = gsign. Only the heights of the hit band are
 // stored, the only ones the caps read.
 // ---------------------------------------------------------------------------
-template <int V>
+template <int VT>
 struct ContrastXOp {  // rows along x of the contrast volume; a = z, line = y
   const float* re;
   const float* im;  // nullptr: real contrast
@@ -638,9 +674,10 @@ struct ContrastXOp {  // rows along x of the contrast volume; a = z, line = y
   __device__ void store(int z, int y, int x, float2 v) const {
     out[((size_t)z * V + y) * V + x] = v;
   }
+  Dim<VT> V{};
 };
 
-template <int V>
+template <int VT>
 struct SpectrumZOp {  // lines along z, in place; heights outside the band are not stored
   static constexpr int kColThreads = 512;  // complex in and out
   float2* buf;
@@ -659,29 +696,17 @@ struct SpectrumZOp {  // lines along z, in place; heights outside the band are n
     const float s = ((x + y + z) & 1) ? -scale : scale;
     buf[((size_t)z * V + y) * V + x] = make_float2(v.x * s, v.y * s);
   }
+  Dim<VT> V{};
 };
 
-template <int V>
-static hipError_t contrast_spectrum_v(const float* re, const float* im, float2* work,
-                                      float scale, const int2* band, const float2* tw,
+template <int VT>
+static hipError_t contrast_spectrum_v(Dim<VT> V, const float* re, const float* im, float2* work,
+                                      float scale, const int2* band, const FftTab& ft,
                                       hipStream_t s) {
-  ContrastXOp<V> x{re, im, work, tw};
-  {
-    Timed tm("fwd_vol_x", s);
-    hipLaunchKernelGGL((k_rows<V, false, ContrastXOp<V>>), rows_grid<V>(V), rows_block<V>(), 0,
-                       s, x);
-  }
-  PlaneYOp<V> y{work, tw, nullptr};
-  {
-    Timed tm("fwd_vol_y", s);
-    hipLaunchKernelGGL((k_cols<V, false, PlaneYOp<V>>), (cols_grid<V, PlaneYOp<V>>(V)),
-                       (cols_block<V, PlaneYOp<V>>()), 0, s, y);
-  }
-  SpectrumZOp<V> z{work, tw, scale, band, 0, V - 1};
-  Timed tm("fwd_vol_z", s);
-  hipLaunchKernelGGL((k_cols<V, false, SpectrumZOp<V>>), (cols_grid<V, SpectrumZOp<V>>(V)),
-                     (cols_block<V, SpectrumZOp<V>>()), 0, s, z);
-  return hipGetLastError();
+  ContrastXOp<VT> x{re, im, work, ft.tw, V};
+  PlaneYOp<VT> y{work, ft.tw, nullptr, V};
+  SpectrumZOp<VT> z{work, ft.tw, scale, band, 0, V - 1, V};
+  return vol_passes<VT, false>(V, x, y, z, ft, "fwd_vol_x", "fwd_vol_y", "fwd_vol_z", s);
 }
 
 // calcHolo's sampling, in the frame of the per-angle spectra (+posSpec frame:
@@ -746,34 +771,36 @@ k_cap_weights(const int32_t* __restrict__ cnt, const AngleGeo* __restrict__ ageo
   w[(size_t)a * C * C + i] = out;
 }
 
-#define MTD_VOL_SIZES(X) \
-  X(32) X(64) X(128) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(704) X(768) X(1024)
+#define MTD_VOL_SIZES(X)                                                                   \
+  X(32) X(64) X(128) X(256) X(320) X(384) X(416) X(448) X(512) X(576) X(640) X(704) X(768) \
+  X(1024)
 
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
                            int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
-                           uint8_t* otf, const float2* tw, hipStream_t s) {
+                           uint8_t* otf, const FftTab& ft, hipStream_t s) {
   switch (V) {
-#define X(n) \
-  case n:    \
-    return finalize_v<n>(vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, out_im, otf, tw, s);
+#define X(n)                                                                                   \
+  case n:                                                                                      \
+    return finalize_v<n>(Dim<n>{}, vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, \
+                         out_im, otf, ft, s);
     MTD_VOL_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return finalize_v<0>(Dim<0>(V), vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, out_im,
+                       otf, ft, s);
 }
 
 hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,
-                                   float scale, const int2* band, const float2* tw,
+                                   float scale, const int2* band, const FftTab& ft,
                                    hipStream_t s) {
   switch (V) {
 #define X(n) \
-  case n:    \
-    return contrast_spectrum_v<n>(re, im, work, scale, band, tw, s);
+  case n: return contrast_spectrum_v<n>(Dim<n>{}, re, im, work, scale, band, ft, s);
     MTD_VOL_SIZES(X)
 #undef X
   }
-  return hipErrorInvalidValue;
+  return contrast_spectrum_v<0>(Dim<0>(V), re, im, work, scale, band, ft, s);
 }
 
 hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, const double* htab,

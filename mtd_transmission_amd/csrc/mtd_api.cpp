@@ -9,12 +9,12 @@
 #include <string>
 #include <vector>
 
+#include <complex>
+
+#include "gen_fft.hpp"
 #include "internal.hpp"
 
 namespace mtd {
-bool holo_len_supported(int N);
-bool crop_len_supported(int C);
-bool vol_len_supported(int V);
 
 thread_local mtd_ctx* tl_ctx = nullptr;
 
@@ -103,38 +103,161 @@ static uint32_t bits_of(float v) {
   return u;
 }
 
+float2 twiddle_at(int L, int k) {
+  // exp(-2 pi i k / L) from an exactly reduced angle
+  const long double ang =
+      2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+  return make_float2((float)std::cos(ang), (float)-std::sin(ang));
+}
+
+void twiddle_host(int L, float2* out) {
+  for (int k = 0; k < L; ++k) out[k] = twiddle_at(L, k);
+}
+
 const float2* twiddle(mtd_ctx* ctx, int L) {
   auto it = ctx->twiddles.find(L);
   if (it != ctx->twiddles.end()) return it->second;
   std::vector<float2> h(L);
-  for (int k = 0; k < L; ++k) {
-    // exp(-2 pi i k / L) from an exactly reduced angle
-    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
-    h[k] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
-  }
+  twiddle_host(L, h.data());
   float2* d = upload(h.data(), h.size());
   if (!d) return nullptr;
   ctx->twiddles[L] = d;
   return d;
 }
 
+// ---------------------------------------------------------------------------
+// runtime-length FFT plans (gen_fft.hpp)
+// ---------------------------------------------------------------------------
+// radices of L from {8, 4, 2, 3, 5, 7, 11, 13}; false if another prime divides L
+static bool gen_factor(int L, std::vector<int>& f) {
+  f.clear();
+  int n = L;
+  while (n % 8 == 0) { f.push_back(8); n /= 8; }
+  while (n % 4 == 0) { f.push_back(4); n /= 4; }
+  while (n % 2 == 0) { f.push_back(2); n /= 2; }
+  for (int r : {3, 5, 7, 11, 13})
+    while (n % r == 0) { f.push_back(r); n /= r; }
+  return n == 1 && (int)f.size() <= kGenMaxPasses;
+}
+
+static int bluestein_len(int L) {
+  int W = 1;
+  while (W < 2 * L - 1) W *= 2;
+  return W;
+}
+
+bool gen_len_ok(int L) {
+  if (L < 2) return false;
+  std::vector<int> f;
+  const int W = gen_factor(L, f) ? L : bluestein_len(L);
+  return gen_lines_per_block(W) >= 1 && (size_t)2 * W * sizeof(float2) <= (size_t)kGenLdsBytes;
+}
+
+// in-place radix-2 FFT (forward, unnormalised) of a power-of-two length, in
+// long double: the host side of the Bluestein tables
+static void fft_pow2_host(std::vector<std::complex<long double>>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const long double ang = -2 * pi * (long double)k / (long double)len;
+        const std::complex<long double> w(std::cos(ang), std::sin(ang));
+        const std::complex<long double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+static const GenPlan* gen_plan(mtd_ctx* ctx, int L) {
+  auto it = ctx->gen_plans.find(L);
+  if (it != ctx->gen_plans.end()) return it->second;
+  if (!gen_len_ok(L)) return nullptr;
+  GenPlan p{};
+  p.L = L;
+  std::vector<int> f;
+  const bool direct = gen_factor(L, f);
+  p.W = direct ? L : bluestein_len(L);
+  if (!direct) gen_factor(p.W, f);
+  p.nf = (int)f.size();
+  for (int i = 0; i < p.nf; ++i) p.f[i] = f[i];
+  p.tw = twiddle(ctx, p.W);
+  if (!p.tw) return nullptr;
+  if (!direct) {
+    // chirp w_n = exp(-i pi n^2 / L) with n^2 reduced mod 2L exactly; the
+    // convolution kernel b = conj(w) wrapped into length W, its spectrum / W
+    const long double pi = 3.14159265358979323846264338327950288L;
+    std::vector<float2> ch(L);
+    std::vector<std::complex<long double>> b(p.W, 0.0L);
+    for (int n = 0; n < L; ++n) {
+      const long long m = ((long long)n * n) % (2LL * L);
+      const long double ang = -pi * (long double)m / (long double)L;
+      const std::complex<long double> w(std::cos(ang), std::sin(ang));
+      ch[n] = make_float2((float)w.real(), (float)w.imag());
+      b[n] = std::conj(w);
+      if (n > 0) b[p.W - n] = std::conj(w);
+    }
+    fft_pow2_host(b);
+    std::vector<float2> bk(p.W);
+    for (int k = 0; k < p.W; ++k)
+      bk[k] = make_float2((float)(b[k].real() / p.W), (float)(b[k].imag() / p.W));
+    float2* dch = upload(ch.data(), ch.size());
+    float2* dbk = upload(bk.data(), bk.size());
+    if (!dch || !dbk) {
+      if (dch) (void)hipFree(dch);
+      if (dbk) (void)hipFree(dbk);
+      return nullptr;
+    }
+    ctx->bluestein[L] = std::make_pair(dch, dbk);
+    p.chirp = dch;
+    p.bk = dbk;
+  }
+  GenPlan* h = new GenPlan(p);
+  ctx->gen_plans[L] = h;
+  return h;
+}
+
+bool fft_tab(mtd_ctx* ctx, int L, bool compiled, FftTab* out) {
+  out->tw = twiddle(ctx, L);
+  out->gen = nullptr;
+  if (!out->tw) return false;
+  if (compiled) return true;
+  out->gen = gen_plan(ctx, L);
+  return out->gen != nullptr;
+}
+
 // tukey2D 1-D factor (pretraitement_hors_axe_CPU/FFT_fonctions.cpp:61-87):
 // alpha is a C float, 2*cpt/(alpha*(N-1)) a float division, pi = 3.1415.
-void tukey1d_host(int N, double alpha_d, double* out) {
-  const float alpha = (float)alpha_d;
+static double tukey1d_at(int N, float alpha, int cpt) {
   const int borne1 = (int)std::round(alpha * (float)(N - 1) / 2.0f);
   const int borne2 = (int)std::round((float)(N - 1) * (1.0f - alpha / 2.0f));
-  for (int cpt = 0; cpt < N; ++cpt) {
-    if (cpt < borne1 + 1) {
-      const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 1.0f;
-      out[cpt] = 0.5 * (1 + std::cos(3.1415 * (double)arg));
-    } else if (cpt < borne2 + 1) {
-      out[cpt] = 1.0;
-    } else {
-      const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 2.0f / alpha + 1.0f;
-      out[cpt] = 0.5 * (1 + std::cos(3.1415 * (double)arg));
-    }
+  if (cpt < borne1 + 1) {
+    const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 1.0f;
+    return 0.5 * (1 + std::cos(3.1415 * (double)arg));
   }
+  if (cpt < borne2 + 1) return 1.0;
+  const float arg = (float)(2 * cpt) / (alpha * (float)(N - 1)) - 2.0f / alpha + 1.0f;
+  return 0.5 * (1 + std::cos(3.1415 * (double)arg));
+}
+
+void tukey1d_host(int N, double alpha_d, double* out) {
+  for (int cpt = 0; cpt < N; ++cpt) out[cpt] = tukey1d_at(N, (float)alpha_d, cpt);
+}
+
+float tukey_signed_at(int n, double alpha, int i) {
+  const float w = (float)tukey1d_at(n, (float)alpha, i);
+  return (i & 1) ? -w : w;
+}
+
+void tukey_signed_host(int n, double alpha, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = tukey_signed_at(n, alpha, i);
 }
 
 const double* tukey_table(mtd_ctx* ctx, int n, double alpha) {
@@ -154,10 +277,8 @@ const float* tukey_signed_table(mtd_ctx* ctx, int n, double alpha) {
   const auto key = std::make_pair(n, bits_of((float)alpha));
   auto it = ctx->tukey_signed.find(key);
   if (it != ctx->tukey_signed.end()) return it->second;
-  std::vector<double> h(n);
-  tukey1d_host(n, alpha, h.data());
   std::vector<float> f(n);
-  for (int i = 0; i < n; ++i) f[i] = (i & 1) ? -(float)h[i] : (float)h[i];
+  tukey_signed_host(n, alpha, f.data());
   float* d = upload(f.data(), f.size());
   if (!d) return nullptr;
   ctx->tukey_signed[key] = d;
@@ -339,24 +460,35 @@ static int ewald_tables(mtd_ctx* ctx, const mtd_geom* g, EwaldTables* t) {
 // ---------------------------------------------------------------------------
 // validation
 // ---------------------------------------------------------------------------
-enum { NEED_HOLO = 1, NEED_VOL = 2 };
-
-static int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
+int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
   if (!ctx) return MTD_ERR_ARG;
   if (!g) return set_err(ctx, MTD_ERR_ARG, "null geometry");
   if (A < 0) return set_err(ctx, MTD_ERR_ARG, "negative angle count");
   const int C = 2 * g->nxmax;
   if (g->nxmax <= 0) return set_err(ctx, MTD_ERR_ARG, "NXMAX must be positive");
-  if (!crop_len_supported(C))
-    return set_err(ctx, MTD_ERR_UNSUPPORTED, "crop side 2*NXMAX=" + std::to_string(C) + " not built");
+  // compiled register FFT lengths run fast; any other length the
+  // runtime-length FFT takes (direct up to 4096, Bluestein up to 2048) works
+  if (!crop_len_supported(C) && !gen_len_ok(C))
+    return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                   "crop side 2*NXMAX=" + std::to_string(C) + " beyond the FFT lengths built");
+  if (g->c_aber && g->deroul == MTD_DEROUL_VOLKOV && !unwrap_len_supported(C) &&
+      !gen_len_ok(2 * C))
+    return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                   "unwrapping length 4*NXMAX=" + std::to_string(2 * C) +
+                       " beyond the FFT lengths built");
   if (g->deroul == MTD_DEROUL_HERRAEZ)
     return set_err(ctx, MTD_ERR_UNSUPPORTED,
                    "DEROUL 1 with VOLKOV 0 (Herraez unwrapping) is not implemented");
   if (g->deroul != MTD_DEROUL_OFF && g->deroul != MTD_DEROUL_VOLKOV)
     return set_err(ctx, MTD_ERR_ARG, "deroul must be MTD_DEROUL_OFF, _VOLKOV or _HERRAEZ");
   if (need & NEED_HOLO) {
-    if (!holo_len_supported(g->n))
-      return set_err(ctx, MTD_ERR_UNSUPPORTED, "hologram side " + std::to_string(g->n) + " not built");
+    if (g->n <= 0 || g->n % 2)
+      return set_err(ctx, MTD_ERR_ARG,
+                     "hologram side CCD_ROIX=" + std::to_string(g->n) +
+                         " must be even and positive (the centred transform's fftshift)");
+    if (!holo_len_supported(g->n) && !gen_len_ok(g->n))
+      return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                     "hologram side " + std::to_string(g->n) + " beyond the FFT lengths built");
     const int x0 = g->circle_cx - g->nxmax, y0 = g->circle_cy - g->nxmax;
     if (x0 < 0 || y0 < 0 || x0 + C > g->n || y0 + C > g->n)
       return set_err(ctx, MTD_ERR_ARG,
@@ -364,8 +496,12 @@ static int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
   }
   if (need & NEED_VOL) {
     const int V = g->dim_final;
-    if (!vol_len_supported(V))
-      return set_err(ctx, MTD_ERR_UNSUPPORTED, "DIM_FINAL " + std::to_string(V) + " not built");
+    if (V <= 0 || V % 2)
+      return set_err(ctx, MTD_ERR_ARG,
+                     "DIM_FINAL " + std::to_string(V) + " must be even and positive (fftshift3D)");
+    if (!vol_len_supported(V) && !gen_len_ok(V))
+      return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                     "DIM_FINAL " + std::to_string(V) + " beyond the FFT lengths built");
     if (!(g->rayon > 0) || g->rayon != std::floor(g->rayon))
       return set_err(ctx, MTD_ERR_ARG, "rayon must be a positive integer value");
     // every voxel index must stay inside V^3 (the reference does not check,
@@ -393,27 +529,38 @@ static int chunk_of(size_t bytes_per_angle, size_t budget, int A) {
   return (int)c;
 }
 
-static size_t dtype_bytes(int dtype) {
+size_t dtype_bytes(int dtype) {
   return dtype == MTD_U8 ? 1 : dtype == MTD_U16 ? 2 : 4;
 }
 
 // holograms [a0, a0+n) -> crops
 // part_out (nullable): argmax partials of these crops, holo_argmax_parts per angle
-static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
-                      int dtype, int a0, int n, float2* crop_out, float2* part_out = nullptr) {
+size_t holo_t1_bytes(const mtd_geom* g) {
+  return (size_t)g->n * holo_t1_cols(g->n, 2 * g->nxmax) * sizeof(float2);
+}
+
+int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int a0, int n,
+                    float2* crop_out, float2* part_out, const float* win, const FftTab& ftN,
+                    float2* t1) {
   const int N = g->n, C = 2 * g->nxmax;
-  const float* win = tukey_signed_table(ctx, N, g->alpha_holo);
-  const float2* twN = twiddle(ctx, N);
-  float2* t1 = (float2*)ws_get(ctx, WS_T1, (size_t)n * N * C * sizeof(float2));
-  if (!win || !twN || !t1) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram stage)");
   const char* src = (const char*)holos + (size_t)a0 * N * N * dtype_bytes(dtype);
-  MTD_TRY(hip_check(ctx, launch_holo_rows(N, dtype, src, n, win, twN, g->circle_cx - g->nxmax,
+  MTD_TRY(hip_check(ctx, launch_holo_rows(N, dtype, src, n, win, ftN, g->circle_cx - g->nxmax,
                                           C, t1, ctx->stream), "hologram row pass"));
-  MTD_TRY(hip_check(ctx, launch_holo_cols(N, t1, n, twN, C, g->circle_cx - g->nxmax,
+  MTD_TRY(hip_check(ctx, launch_holo_cols(N, t1, n, ftN, C, g->circle_cx - g->nxmax,
                                           g->circle_cy - g->nxmax, crop_out, part_out,
                                           ctx->stream),
                     "hologram column pass"));
   return MTD_OK;
+}
+
+static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                      int dtype, int a0, int n, float2* crop_out, float2* part_out = nullptr) {
+  const float* win = tukey_signed_table(ctx, g->n, g->alpha_holo);
+  FftTab ftN;
+  const bool tabs = fft_tab(ctx, g->n, holo_len_supported(g->n), &ftN);
+  float2* t1 = (float2*)ws_get(ctx, WS_T1, (size_t)n * holo_t1_bytes(g));
+  if (!win || !tabs || !t1) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram stage)");
+  return holo_chunk_with(ctx, g, holos, dtype, a0, n, crop_out, part_out, win, ftN, t1);
 }
 
 static int field_mode(const mtd_geom* g) { return (g->born ? 1 : 0) | (g->c_aber ? 2 : 0); }
@@ -425,13 +572,14 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
                        int* posx, int* posy, float2* out_fields, float2* out_F,
                        const float2* part_in = nullptr, int S = 0) {
   const int C = 2 * g->nxmax;
-  const float2* twC = twiddle(ctx, C);
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, crop_len_supported(C), &twC);
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)n * C * C * sizeof(float2));
   float2* iu = (float2*)ws_get(ctx, WS_IU, (size_t)n * C * C * sizeof(float2));
   double* coef = (double*)ws_get(ctx, WS_COEF, (size_t)n * 30 * sizeof(double));
-  if (!twC || !work || !iu || !coef)
+  if (!tabs || !work || !iu || !coef)
     return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
-  if (part_in) {
+  if (part_in && S > 0) {
     MTD_TRY(hip_check(ctx, launch_argmax_fin(C, part_in, S, n, posx, posy, ctx->stream), "argmax"));
   } else {
     float2* part = (float2*)ws_get(ctx, WS_ARGMAX, argmax_scratch_bytes(n));
@@ -444,10 +592,12 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
   // reference only unwraps when the aberration correction runs (main.cpp:253-277)
   float* phu = nullptr;
   if (g->c_aber && g->deroul == MTD_DEROUL_VOLKOV) {
-    const float2* tw2 = twiddle(ctx, 2 * C);
+    FftTab tw2;
+    const bool tabs2 = fft_tab(ctx, 2 * C, unwrap_len_supported(C), &tw2);
     phu = (float*)ws_get(ctx, WS_PHU, (size_t)n * C * C * sizeof(float));
     char* us = (char*)ws_get(ctx, WS_UNWRAP, unwrap_scratch_bytes(C, n));
-    if (!tw2 || !phu || !us) return set_err(ctx, MTD_ERR_HIP, "out of device memory (unwrapping)");
+    if (!tabs2 || !phu || !us)
+      return set_err(ctx, MTD_ERR_HIP, "out of device memory (unwrapping)");
     const size_t D2 = (size_t)4 * C * C * n;  // (2C)^2 per angle
     float* gx = (float*)us;
     float* gy = gx + D2;
@@ -613,6 +763,11 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   timing_collect(ctx);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (auto& kv : ctx->gen_plans) delete kv.second;
+  for (auto& kv : ctx->bluestein) {
+    (void)hipFree(kv.second.first);
+    (void)hipFree(kv.second.second);
+  }
   for (auto& kv : ctx->tukey) (void)hipFree(kv.second);
   for (auto& kv : ctx->tukey_signed) (void)hipFree(kv.second);
   for (auto& kv : ctx->htab) (void)hipFree(kv.second);
@@ -782,7 +937,8 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
   const int S = holo_argmax_parts(g->n, C);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
-  float2* part = (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * S * sizeof(float2));
+  float2* part =
+      (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * std::max(S, 1) * sizeof(float2));
   if (!crop || !part) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
@@ -800,11 +956,12 @@ int mtd_field_to_spectrum(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
   if (A == 0) return MTD_OK;
   if (!fields || !posspec || !spectra) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int C = 2 * g->nxmax;
-  const float2* twC = twiddle(ctx, C);
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, crop_len_supported(C), &twC);
   const double* winC = tukey_table(ctx, C, g->alpha_field);
   const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, A);
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
-  if (!twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!tabs || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
@@ -954,11 +1111,12 @@ int mtd_ewald_accumulate(mtd_ctx* ctx, const mtd_geom* g, const float* fields,
     return set_err(ctx, MTD_ERR_ARG, "null array");
   const int C = 2 * g->nxmax;
   float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
-  const float2* twC = twiddle(ctx, C);
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, crop_len_supported(C), &twC);
   const double* winC = tukey_table(ctx, C, g->alpha_field);
   const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, std::max(A, 1));
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
-  if (!F || !twC || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!F || !tabs || !winC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
   EwaldTables t;
   MTD_TRY(ewald_tables(ctx, g, &t));
@@ -980,8 +1138,9 @@ static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
                          float* index_im, uint8_t* otf, double scale = 0) {
   const int V = g->dim_final;
   float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
-  const float2* twV = twiddle(ctx, V);
-  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory (volume work)");
+  FftTab twV;
+  const bool tabs = fft_tab(ctx, V, vol_len_supported(V), &twV);
+  if (!work || !tabs) return set_err(ctx, MTD_ERR_HIP, "out of device memory (volume work)");
   // Reconstruction/main.cpp:346-347 (PI = M_PI there) and the delta_f^3 of
   // TF3Dcplx_INV (src/FFT_fonctions.cpp:399)
   const double kv = 2 * M_PI / g->lambda;
@@ -1074,8 +1233,9 @@ int mtd_forward_spectra(mtd_ctx* ctx, const mtd_geom* g, const float* dn_re, con
   bool planned;
   MTD_TRY(call_geometry(ctx, g, posspec, A, t, true, &geo, &planned));
   float2* work = (float2*)ws_get(ctx, WS_VWORK, (size_t)V * V * V * sizeof(float2));
-  const float2* twV = twiddle(ctx, V);
-  if (!work || !twV) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  FftTab twV;
+  const bool tabs = fft_tab(ctx, V, vol_len_supported(V), &twV);
+  if (!work || !tabs) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   const double tp = tomo_pixel(g);
   const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
   MTD_TRY(hip_check(ctx, launch_contrast_spectrum(V, dn_re, dn_im, work,
@@ -1165,10 +1325,11 @@ int mtd_spectra_to_fields(mtd_ctx* ctx, const mtd_geom* g, const float* spectra,
   if (A == 0) return MTD_OK;
   if (!spectra || !posspec || !fields) return set_err(ctx, MTD_ERR_ARG, "null array");
   const int C = 2 * g->nxmax;
-  const float2* twC = twiddle(ctx, C);
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, crop_len_supported(C), &twC);
   const int chunk = chunk_of((size_t)C * C * sizeof(float2), kFieldBudget, A);
   float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
-  if (!twC || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  if (!tabs || !work) return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   // TF2Dcplx_INV's factor delta_f^2 (Simul_holo/src/FFT_fonctions.cpp:280-299)
   const float scale = (float)(g->delta_f * g->delta_f);
   for (int a0 = 0; a0 < A; a0 += chunk) {
@@ -1306,7 +1467,8 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, std::max(A, 1));
   const int S = holo_argmax_parts(g->n, C);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
-  float2* part = (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * S * sizeof(float2));
+  float2* part =
+      (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * std::max(S, 1) * sizeof(float2));
   float2* vol = (float2*)ws_get(ctx, WS_VOL, (size_t)V * V * V * sizeof(float2));
   int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
   if (!F || !crop || !part || !vol || !band)

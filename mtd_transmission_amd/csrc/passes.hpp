@@ -14,6 +14,21 @@
 
 namespace mtd {
 
+// A side length fixed at compile time (N > 0: folds into the index
+// arithmetic of the compiled lengths) or held at run time (N == 0: the
+// runtime-length path). Converts to int.
+template <int N>
+struct Dim {
+  __host__ __device__ constexpr Dim(int = N) {}
+  __host__ __device__ constexpr operator int() const { return N; }
+};
+template <>
+struct Dim<0> {
+  int v = 0;
+  __host__ __device__ constexpr Dim(int x = 0) : v(x) {}
+  __host__ __device__ constexpr operator int() const { return v; }
+};
+
 // Raw buffer view of one device array (at most 2 GiB). A load takes a 32-bit
 // per-lane byte offset and a uniform byte offset held in a scalar register,
 // so a run of loads at uniform strides keeps one address register in place of

@@ -16,10 +16,11 @@ from cases import LAM, N0, NA, rel_l2
 from oracle import ref
 
 
-def acquisition(n, nx, V, A, seed, dtype="u16"):
+def acquisition(n, nx, V, A, seed, deroul=0):
     """Seeded holograms (torch, on the GPU) of a small ellipsoid phantom,
     Fermat-spiral illumination, with the geometry of the reference's
-    conventions: pix = NXMAX lambda / (NA N), carrier at 0.6 Nyquist."""
+    conventions: pix = NXMAX lambda / (NA N), carrier at 0.6 Nyquist.
+    deroul 1: Volkov unwrapping (DEROUL 1, VOLKOV 1) with Born fields."""
     import mtd_transmission_amd as m
     from mtd_transmission_amd import synth
     pix = nx * LAM / (NA * n)
@@ -27,8 +28,8 @@ def acquisition(n, nx, V, A, seed, dtype="u16"):
     ph = synth.scale_phantom(synth.make_phantom(seed, 3), 0.45 * n * pix / 33.8e-6)
     kx, ky = synth.scan_fermat(A, nx, 0.95)
     holos, mask = synth.simulate_torch(n, nx, cx, cy, pix, LAM, N0, kx, ky, ph, seed=seed + 1)
-    gl = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix)
-    go = ref.derive_geom(n, nx, V, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    gl = m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix, 1.0, 1.0, 1.0, 1.0, 1, 1, deroul)
+    go = ref.derive_geom(n, nx, V, cx, cy, NA, N0, LAM, pix, 1, 1, 1, deroul=deroul)
     return holos, mask, gl, go
 
 

@@ -50,12 +50,32 @@ def _check_index(re, im, fields32, fields64, pos, go, seed, dense_max=320, sampl
     return e32, e64, bound
 
 
+# sides without a compiled register FFT (the runtime-length FFTs), as
+# (N, NXMAX, V, unwrapping): direct mixed-radix (600 = 8*3*25, 182 = 2*7*13,
+# 364 = 4*7*13, 104 = 8*13, 208 = 16*13) and Bluestein (crop 122 = 2*61 and
+# volume 254 = 2*127: chirp-z over 256 and 512 points; hologram 526 = 2*263:
+# over 2048), with and without Volkov unwrapping (its 2C-point transforms);
+# crop 250 (the shipped Tomo_config's NXMAX 125) and volume 416 (the shipped
+# simulator configuration's DIM_FINAL) are compiled lengths
+GEN_CASES = [(512, 61, 254, 0), (512, 61, 254, 1), (600, 91, 364, 0), (526, 52, 208, 1),
+             (1024, 125, 512, 1), (1024, 61, 416, 0)]
+
+
+@pytest.mark.parametrize("n,nx,V,deroul", GEN_CASES)
+def test_plain_runtime_lengths(gpu_ctx, n, nx, V, deroul):
+    _plain_case(gpu_ctx, n, nx, V, 3, deroul)
+
+
 @pytest.mark.parametrize("n,nx", CASES)
 def test_plain_at_length(gpu_ctx, n, nx):
+    _plain_case(gpu_ctx, n, nx, 4 * nx, 3, 0)
+
+
+def _plain_case(gpu_ctx, n, nx, V, A, deroul):
     import torch
-    C, V, A = 2 * nx, 4 * nx, 3
-    _say(f"plain method N={n} crop={C} V={V} A={A}")
-    holos, mask, gl, go = L.acquisition(n, nx, V, A, seed=100 + nx)
+    C = 2 * nx
+    _say(f"plain method N={n} crop={C} V={V} A={A} deroul={deroul}")
+    holos, mask, gl, go = L.acquisition(n, nx, V, A, seed=100 + nx, deroul=deroul)
     re = torch.empty((V, V, V), device="cuda")
     im = torch.empty_like(re)
     pos = torch.empty((2 * A,), dtype=torch.int32, device="cuda")
