@@ -151,6 +151,19 @@ int mtd_write_tiff_f32(const char* path, const float* data, int w, int h,
 int mtd_holo_to_crop(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
                      int dtype, int A, float* crops);
 
+/* Allocation-free variant of mtd_holo_to_crop. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same kernels and results; the twiddles, the window and the row
+ * intermediate live in one static device array of 64 MiB per GPU (held until
+ * the process ends), so one hologram's N x C row intermediate must fit 63 MiB
+ * (any compiled hologram side does; sides without a compiled FFT are
+ * refused) and holos must be device memory. It calls no hipMalloc/hipFree.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, threads or contexts) race. */
+int mtd_holo_to_crop_noalloc(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
+                             int dtype, int A, float* crops);
+
 /* Field formation from crops (main.cpp:230-342): coordSpec, calc_Uborn2,
  * atan2 phase, aberCorr2/ampliCorr2 polynomial fits (FP64), amplitude clamp,
  * Born/Rytov (or the C_ABER 0 branch). mask: host uint8 [C][C]

@@ -73,6 +73,7 @@ SIGNATURES = {
     "mtd_tukey1d": (_I, [_I, _D, ctypes.POINTER(_D)]),
     "mtd_write_tiff_f32": (_I, [ctypes.c_char_p, _P, _I, _I, _I, _D, _I, _I]),
     "mtd_holo_to_crop": (_I, [_P, _GP, _P, _I, _I, _P]),
+    "mtd_holo_to_crop_noalloc": (_I, [_P, _GP, _P, _I, _I, _P]),
     "mtd_crop_to_field": (_I, [_P, _GP, _P, _I, _P, _P, _P]),
     "mtd_preprocess": (_I, [_P, _GP, _P, _I, _I, _P, _P, _P]),
     "mtd_field_to_spectrum": (_I, [_P, _GP, _P, _P, _I, _P]),
@@ -334,6 +335,17 @@ class Context:
                                            _dtype_code(holos), A,
                                            _dev(crops, "c32", A * g.C * g.C, "crops")),
                     "holo_to_crop")
+
+    def holo_to_crop_noalloc(self, g: Geom, holos, crops):
+        """holo_to_crop with no device allocation: the library's per-device
+        scratch array holds its tables and row intermediate (holograms in
+        device memory; calls overlapping on one GPU race)."""
+        self._stream()
+        A = _holo_count(holos, g)
+        self._check(lib().mtd_holo_to_crop_noalloc(self.h, ctypes.byref(g), _ptr(holos),
+                                                   _dtype_code(holos), A,
+                                                   _dev(crops, "c32", A * g.C * g.C, "crops")),
+                    "holo_to_crop_noalloc")
 
     def crop_to_field(self, g: Geom, crops, mask, posspec, fields):
         self._stream()

@@ -177,6 +177,9 @@ size_t holo_t1_bytes(const mtd_geom* g);
 int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int a0, int n,
                     float2* crop_out, float2* part_out, const float* win, const FftTab& ftN,
                     float2* t1);
+// allocation-free entry points: the per-device scratch address, looked up
+// once per device when a context is created on it
+int noalloc_init(int device, std::string* err);
 int hip_check(mtd_ctx* ctx, hipError_t e, const char* what);
 void* ws_get(mtd_ctx* ctx, int slot, size_t bytes);
 const float2* twiddle(mtd_ctx* ctx, int L);

@@ -748,6 +748,12 @@ int mtd_ctx_create(int device, mtd_ctx** out) {
                     " (" + hipGetErrorString(e) + ")";
     return MTD_ERR_HIP;
   }
+  // the allocation-free entry points' scratch address on this device
+  std::string nerr;
+  if (noalloc_init(device, &nerr) != MTD_OK) {
+    tl_create_err = nerr;
+    return MTD_ERR_HIP;
+  }
   mtd_ctx* c = new mtd_ctx();
   c->device = device;
   c->stream = nullptr;
