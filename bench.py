@@ -3,9 +3,10 @@
 Workload (the full-size single-GPU configuration): 600 off-axis holograms
 1024x1024 uint16 of a seeded 5-12 ellipsoid phantom, Fermat-spiral
 illumination filling 0.95 NA, Poisson shot noise at 20k photons peak,
-sideband crop 256x256, 512^3 complex64 spectrum + int32 hit counts, 3-D
-inverse FFT to a float32 index volume. One step = holograms already in HBM ->
-index volume (mtd_reconstruct_plain: both programs' compute).
+sideband crop 256x256, the count-normalised 512^3 complex64 spectrum (its
+hit band; counts are not stored), 3-D inverse FFT to float32 index and
+absorption volumes. One step = holograms already in HBM -> index volume
+(mtd_reconstruct_plain: both programs' compute).
 
 Prints one JSON line. value = holograms/s of the end-to-end step
 (A / step time); ms_per_step = ms per 512^3 volume.
@@ -75,27 +76,42 @@ def hit_band(pos, nx, V, R, coef_na):
 
 
 def kernel_bytes(name, n, C, V, A, P, M, B, b_in=2):
-    """Algorithmic (compulsory) bytes of one step, per kernel: each stage
-    interface read once and written once. B = planes in the hit band: the
-    fused plain method writes and transforms only those before the last
-    (height) pass, which reads them and writes the whole volume."""
-    c2 = C * C * 8
+    """Algorithmic (compulsory) bytes of one step per kernel, from the
+    per-hologram stage bytes of the measurement definition: S1 = b_in N^2 +
+    8 C^2 (the hologram read by the row pass, the crop written by the column
+    pass; the N x C row intermediate between them is not compulsory), S2 =
+    16 C^2 (the crop read by the demodulation's row pass, the spectrum
+    written by the forward column pass; the passes in between move
+    intermediates), S3 = 8 P (pupil samples read by the gather). Per volume
+    the fused plain method writes only the B planes of the hit band of the
+    count-normalised spectrum (8 B V^2; no counts), and the finalisation
+    reads that band and writes the index and absorption volumes (8 V^3).
+    Kernels that only move intermediates count 0 (their rate is reported from
+    the counters instead)."""
     table = {
-        "holo_rows": A * (n * n * b_in + n * C * 8),
-        "holo_cols": A * (n * C * 8 + c2),
-        "argmax": A * c2,
-        "demod_rows": A * 2 * c2,
-        "demod_cols": A * 2 * c2,
-        "lsq": A * M * 8,
-        "fwd_rows_field": A * 2 * c2,
-        "fwd_cols": A * 2 * c2,
-        "angle_geo": A * 32,
-        "ewald_gather": A * P * 8 + B * V * V * 8,
-        "vol_x": B * V * V * 16,
-        "vol_y": B * V * V * 16,
-        "vol_z": B * V * V * 8 + V ** 3 * 8,
+        "holo_rows": A * b_in * n * n,
+        "holo_cols": A * 8 * C * C,
+        "demod_rows": A * 8 * C * C,
+        "fwd_cols": A * 8 * C * C,
+        "ewald_gather": A * 8 * P + 8 * B * V * V,
+        "vol_x": 8 * B * V * V,
+        "vol_z": 8 * V ** 3,
     }
     return table.get(name)
+
+
+def stage_bytes(n, C, V, A, P, B, b_in=2):
+    """Per stage: the measurement definition's bytes (S1, S2, S3 per
+    hologram; accumulation 12 V^3 of spectrum + counts; S4 = 16 V^3) and the
+    bytes this design must move (banded spectrum, no counts, index and
+    absorption written)."""
+    s1, s2, s3 = A * (b_in * n * n + 8 * C * C), A * 16 * C * C, A * 8 * P
+    return {
+        "S1": {"definition": s1, "design": s1},
+        "S2": {"definition": s2, "design": s2},
+        "S3_acc": {"definition": s3 + 12 * V ** 3, "design": s3 + 8 * B * V * V},
+        "S4": {"definition": 16 * V ** 3, "design": 8 * B * V * V + 8 * V ** 3},
+    }
 
 
 def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
@@ -216,44 +232,72 @@ def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
             "weights": "1/(q^2 hits)", "start": "plain"}
 
 
-def cpu_baseline_child(path_holos, path_out):
-    """Runs in a child with one thread: the oracle (FP64, the reference's
-    arithmetic) on a sample of holograms, plus one full-volume finalize."""
+def cpu_baseline_child(path_holos, path_out, threads):
+    """Runs in a child process: the reference CPU path's FP64 arithmetic at
+    its stage boundaries over every hologram (oracle/cpu_pipeline.py), the
+    FFTs on `threads` threads, the host loops serial."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle import ref
+    from oracle import cpu_pipeline, ref
     d = np.load(path_holos)
     holos, mask = d["holos"], d["mask"]
     n, nx, v, A, pix, cx, cy = geometry(WORKLOADS["cfg2"])
     g = ref.derive_geom(n, nx, v, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
-    S = holos.shape[0]
     t0 = time.perf_counter()
-    fields, pos = ref.preprocess(holos, g, mask, "f64")
-    t1 = time.perf_counter()
-    spectra = np.stack([ref.field_to_spectrum(fields[a], int(pos[a]), int(pos[a + S]), g, "f64")
-                        for a in range(S)])
-    vol, cnt = ref.ewald_scatter(spectra, pos, g, "f64")
-    t2 = time.perf_counter()
-    ref.finalize(vol, cnt, g, "f64")
-    t3 = time.perf_counter()
-    json.dump({"S": S, "pre": t1 - t0, "rec": t2 - t1, "fin": t3 - t2}, open(path_out, "w"))
+    _, _, t, name = cpu_pipeline.run(holos, g, mask, threads)
+    t["total"] = time.perf_counter() - t0
+    json.dump({"A": int(holos.shape[0]), "stages_s": t, "fft": name}, open(path_out, "w"))
 
 
-def cpu_baseline(holos_dev, mask, sample):
+def host_cpu():
+    """nproc and the CPU model of this host (lscpu), and the threads the
+    baseline may use: the host's share of one GPU box is 16 cores."""
+    import shutil
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    model = ""
+    if shutil.which("lscpu"):
+        r = subprocess.run(["lscpu"], capture_output=True, text=True)
+        for line in r.stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    return nproc, model
+
+
+def cpu_baseline(holos_dev, mask, A):
+    """The reference CPU path on this host over all A holograms, at the
+    reference's setting (NB_THREADS 4 for the FFTs) and with every core of
+    the host's share (min(nproc, 16))."""
     import numpy as np
     import tempfile
+    nproc, model = host_cpu()
     tmp = tempfile.mkdtemp(prefix="mtd_cpu_")
-    ph, po = os.path.join(tmp, "in.npz"), os.path.join(tmp, "out.json")
-    np.savez(ph, holos=holos_dev[:sample].cpu().numpy(), mask=mask)
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1",
-               MKL_NUM_THREADS="1", HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
-    subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", ph, po],
-                   check=True, env=env)
-    r = json.load(open(po))
+    ph = os.path.join(tmp, "in.npz")
+    np.savez(ph, holos=holos_dev[:A].cpu().numpy(), mask=mask)
+    runs = {}
+    for label, th in (("NB_THREADS_4", 4), ("nproc", min(nproc, 16))):
+        po = os.path.join(tmp, f"out_{th}.json")
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1",
+                   MKL_NUM_THREADS="1", HIP_VISIBLE_DEVICES="", CUDA_VISIBLE_DEVICES="")
+        print(f"[bench] CPU baseline: {A} holograms, FFT threads {th}", file=sys.stderr,
+              flush=True)
+        subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", ph, po,
+                        str(th)], check=True, env=env)
+        r = json.load(open(po))
+        os.remove(po)
+        st = r["stages_s"]
+        compute = st["T_pre1"] + st["T_pre2"] + st["T_rec1"] + st["T_rec2"]
+        runs[label] = {"threads": th, "fft": r["fft"],
+                       "stages_s": {k: round(v, 3) for k, v in st.items()},
+                       "ms_per_volume": round(compute * 1e3, 1),
+                       "holograms_per_s": round(r["A"] / compute, 3),
+                       "holograms_per_s_s1_to_s3": round(
+                           r["A"] / (st["T_pre1"] + st["T_pre2"] + st["T_rec1"]), 3)}
     os.remove(ph)
-    os.remove(po)
     os.rmdir(tmp)
-    return r
+    return runs, nproc, model
 
 
 def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
@@ -378,9 +422,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cfg2")
-    ap.add_argument("--cpu-sample", type=int, default=16,
-                    help="holograms timed for the CPU baseline (0 = skip)")
-    ap.add_argument("--cpu-child", nargs=2, help=argparse.SUPPRESS)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline (the reference CPU path over every hologram)")
+    ap.add_argument("--cpu-child", nargs=3, help=argparse.SUPPRESS)
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the plain vs adjoint-TV comparison (differentiation form)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -388,7 +432,7 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
     if args.cpu_child:
-        cpu_baseline_child(*args.cpu_child)
+        cpu_baseline_child(args.cpu_child[0], args.cpu_child[1], int(args.cpu_child[2]))
         return
     if args.copy_probe:
         import torch
@@ -448,6 +492,10 @@ def main():
     torch.cuda.synchronize()
     ctx.set_timing(False)
     kt_all = ctx.timing()
+    counters = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and args.workload == "cfg2":  # measured on that workload
+        counters = json.load(open(tpath))
     per_kernel = {}
     for name, (launches, tot) in kt_all.items():
         b = kernel_bytes(name, n, C, V, A, P, M, B)
@@ -455,22 +503,37 @@ def main():
         avg = tot / launches
         per_kernel[name] = {"ms_per_launch": round(avg, 5), "launches_per_step": lps,
                             "ms_per_step": round(tot / args.steps, 4)}
-        if b is not None:
-            per_kernel[name]["GBps"] = round(b / lps / (avg * 1e-3) / 1e9, 1)
+        if b:
+            per_kernel[name]["GBps_algorithmic"] = round(b / lps / (avg * 1e-3) / 1e9, 1)
+        if isinstance(counters.get(name), (int, float)):
+            per_kernel[name]["GBps_counters"] = round(counters[name] / (avg * 1e-3) / 1e9, 1)
     # stage times from the breakdown: S1 hologram stage, S2 fields and
     # per-angle spectra, S3 + accumulation (Ewald gather and its geometry),
     # S4 finalisation (3-D inverse FFT to the index volume)
-    stage_of = {"holo_rows": "S1", "holo_cols": "S1", "argmax_fin": "S2", "demod_rows": "S2",
-                "demod_cols": "S2", "lsq": "S2", "fwd_rows_field": "S2", "fwd_cols": "S2",
+    stage_of = {"holo_rows": "S1", "holo_cols": "S1", "argmax": "S2", "argmax_fin": "S2",
+                "demod_rows": "S2", "demod_cols": "S2", "lsq": "S2", "fwd_rows_field": "S2",
+                "fwd_cols": "S2",
                 "angle_geo": "S3_acc", "ewald_band": "S3_acc", "ewald_tiles": "S3_acc",
                 "ewald_gather": "S3_acc", "vol_x": "S4", "vol_y": "S4", "vol_z": "S4"}
     stages = {}
     for name, (launches, tot) in kt_all.items():
         st = stage_of.get(name, "other")
         stages[st] = round(stages.get(st, 0.0) + tot / args.steps, 4)
+    # each stage's bytes over its kernels' time, against the 8 TB/s peak:
+    # the measurement definition's bytes and this design's (banded spectrum)
+    sb = stage_bytes(n, C, V, A, P, B)
+    stage_roofline = {}
+    for st, by in sb.items():
+        t_ms = stages.get(st, 0.0)
+        if t_ms > 0:
+            stage_roofline[st] = {
+                "ms": t_ms,
+                "bytes_definition": by["definition"], "bytes_design": by["design"],
+                "frac_definition": round(by["definition"] / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                "frac_design": round(by["design"] / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     # the kernel with the largest total time among those with an algorithmic
     # byte count (under a counter-collecting profiler the order can change)
-    dom = max((k for k in kt_all if kernel_bytes(k, n, C, V, A, P, M, B) is not None),
+    dom = max((k for k in kt_all if kernel_bytes(k, n, C, V, A, P, M, B)),
               key=lambda k: kt_all[k][1])
 
     # timed region: K steps, events around the dominant kernel's launches only
@@ -511,27 +574,35 @@ def main():
     bpl = kernel_bytes(dom, n, C, V, A, P, M, B) / lps
     avg_s = tot / launches * 1e-3
     achieved = bpl / avg_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath) and args.workload == "cfg2":  # measured on that workload
-        traffic = json.load(open(tpath)).get(dom)
+    traffic = counters.get(dom) if isinstance(counters.get(dom), (int, float)) else None
+    step_def = sum(v["definition"] for v in sb.values())
+    step_des = sum(v["design"] for v in sb.values())
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes_per_launch": int(bpl),
-                "ms_per_launch": round(avg_s * 1e3, 5)}
+                "ms_per_launch": round(avg_s * 1e3, 5),
+                "stages": stage_roofline,
+                "step": {"bytes_definition": step_def, "bytes_design": step_des,
+                         "frac_definition": round(step_def / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                         "frac_design": round(step_des / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+                "note": ("achieved = the kernel's share of the per-hologram stage bytes "
+                         "(hologram rows: the b_in N^2 hologram read) per launch over its "
+                         "event-timed mean duration; stages and step: their bytes over their "
+                         "time")}
 
     cpu = None
-    if args.cpu_sample > 0 and args.workload == "cfg2":
-        r = cpu_baseline(holos, mask, args.cpu_sample)
-        per_holo = (r["pre"] + r["rec"]) / r["S"]
-        cpu_ms_volume = (per_holo * A + r["fin"]) * 1e3
-        cpu = {"value": round(A / (cpu_ms_volume / 1e3), 3), "unit": "holograms/s",
-               "cores": 1, "kind": "port",
-               "sample": (f"FP64 oracle on {r['S']} of the {A} holograms (preprocess "
-                          f"{r['pre']:.2f}s + spectra/scatter {r['rec']:.2f}s) extrapolated "
-                          f"to {A}, plus one full {V}^3 finalize ({r['fin']:.2f}s); numpy "
-                          f"pocketfft, 1 thread"),
-               "ms_per_volume": round(cpu_ms_volume, 1)}
+    if not args.no_cpu and args.workload == "cfg2":
+        runs, nproc, model = cpu_baseline(holos, mask, A)
+        ref4 = runs["NB_THREADS_4"]
+        cpu = {"value": ref4["holograms_per_s"], "unit": "holograms/s", "cores": 4,
+               "kind": "port",
+               "sample": (f"all {A} holograms of the workload, the reference's FP64 "
+                          f"arithmetic at its stage boundaries (oracle/cpu_pipeline.py), "
+                          f"FFTs on NB_THREADS=4 threads as the shipped recon.txt, host "
+                          f"loops serial; {ref4['fft']}"),
+               "ms_per_volume": ref4["ms_per_volume"],
+               "nproc": nproc, "cpu_model": model, "settings": runs,
+               "gpu_speedup": {k: round(r["ms_per_volume"] / ms, 1) for k, r in runs.items()}}
 
     hbm = copy_bandwidth(torch)
     if args.no_quality:
@@ -547,7 +618,9 @@ def main():
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain method: {A} x {n}^2 uint16 holograms -> crop {C}^2 -> "
-                               f"{V}^3 complex64 spectrum + int32 counts -> {V}^3 float32 index",
+                               f"count-normalised {V}^3 complex64 spectrum (only its hit band "
+                               f"materialised; counts not stored) -> {V}^3 float32 index and "
+                               f"absorption",
                    "holograms": A, "hologram_side": n, "crop": C, "volume": V,
                    "hit_band": [lo, hi]},
         "ms_per_volume": round(ms, 4),

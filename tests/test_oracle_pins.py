@@ -471,3 +471,19 @@ def test_sampled_index_equals_dense_oracle():
         f, p = ref.preprocess(case.acq.holos, g, case.acq.mask, prec)
         dense = ref.reconstruct(f, p, g, prec)[0].ravel()[idx]
         assert rel_l2(L.index_at(f, p, g, prec, idx), dense) < tol
+
+
+def test_cpu_baseline_pipeline_equals_oracle():
+    """The timed CPU baseline (oracle/cpu_pipeline.py: the FP64 arithmetic at
+    the reference's stage boundaries, FFTs threaded) gives the FP64 oracle's
+    speculars and index volume."""
+    from oracle import cpu_pipeline
+    from cases import make_case, rel_l2
+    case = make_case("small")
+    idx, pos, t, name = cpu_pipeline.run(case.acq.holos, case.g, case.acq.mask, 2)
+    f64, p64 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f64")
+    want = ref.reconstruct(f64, p64, case.g, "f64")[0]
+    assert np.array_equal(pos, p64)
+    # FP64 rounding of another FFT library, through the normal-equation fit
+    assert rel_l2(idx, want) < 1e-9
+    assert set(t) == {"T_pre1", "T_pre2", "T_rec1", "T_rec2"} and name
