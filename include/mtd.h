@@ -42,7 +42,7 @@ extern "C" {
 enum mtd_status {
   MTD_OK = 0,
   MTD_ERR_ARG = 1,          /* invalid argument or geometry            */
-  MTD_ERR_UNSUPPORTED = 2,  /* size/mode not built (e.g. Herraez)      */
+  MTD_ERR_UNSUPPORTED = 2,  /* size/mode beyond what is built         */
   MTD_ERR_HIP = 3,          /* HIP runtime error                       */
   MTD_ERR_IO = 4            /* file could not be read or written       */
 };
@@ -52,8 +52,11 @@ enum mtd_dtype { MTD_U8 = 0, MTD_U16 = 1, MTD_F32 = 2 };
 /* Phase unwrapping (pretraitement_hors_axe_CPU/main.cpp:260-277; only applied
  * with C_ABER 1): DEROUL 0 -> OFF; DEROUL 1 with VOLKOV 1 -> VOLKOV
  * (deroul_volkov4_total_sym_paire, src/deroulement_volkov4.cpp:14-92);
- * DEROUL 1 with VOLKOV 0 -> HERRAEZ (phaseUnwrapping_Mat, not built:
- * MTD_ERR_UNSUPPORTED). */
+ * DEROUL 1 with VOLKOV 0 -> HERRAEZ (phaseUnwrapping_Mat,
+ * deroulement_herraez.cpp:32-75: the reference's edge order and grouping
+ * tree; the border reliabilities come from rand()'s stream, in which angle
+ * a of a call's stack is the a-th image unwrapped, as in one run of the
+ * program). */
 enum mtd_deroul { MTD_DEROUL_OFF = 0, MTD_DEROUL_VOLKOV = 1, MTD_DEROUL_HERRAEZ = 2 };
 
 typedef struct mtd_ctx mtd_ctx;

@@ -79,6 +79,8 @@ struct mtd_ctx {
   // tables (chirp, chirp spectrum)
   std::map<int, mtd::GenPlan*> gen_plans;
   std::map<int, std::pair<float2*, float2*>> bluestein;
+  // Herraez border reliabilities per (crop side, first angle, angles)
+  std::map<std::tuple<int, int, int>, float*> herraez_border;
   std::map<std::pair<int, uint32_t>, double*> tukey;      // (n, bits of (float)alpha)
   std::map<std::pair<int, uint32_t>, float*> tukey_signed;
   std::map<std::pair<uint64_t, int>, double*> htab;       // heights per (bits of R, table size)
@@ -246,6 +248,13 @@ hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
 hipError_t launch_unwrap(int C, const float2* iu, int A, const FftTab& ft2C, float* gx,
                          float* gy, float2* P, float2* Q, float* phu, hipStream_t s);
 size_t unwrap_scratch_bytes(int C, int A);
+// Herraez unwrapping of the wrapped phase of IU into phu (U frame);
+// border: the angles' border reliabilities (herraez_border_table), scratch
+// of herraez_scratch_bytes(C, A)
+hipError_t launch_herraez(int C, const float2* iu, int A, const float* border, void* scratch,
+                          float* phu, hipStream_t s);
+size_t herraez_scratch_bytes(int C, int A);
+const float* herraez_border_table(mtd_ctx* ctx, int C, int a0, int n);
 // Per-angle forward spectrum. from_iu=0: src are fields (U frame);
 // from_iu=1: src is IU and the field formation runs inside the load.
 // Output frame: shift_pos=0 -> unshifted F, 1 -> shifted by +posSpec.

@@ -476,10 +476,8 @@ int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
     return set_err(ctx, MTD_ERR_UNSUPPORTED,
                    "unwrapping length 4*NXMAX=" + std::to_string(2 * C) +
                        " beyond the FFT lengths built");
-  if (g->deroul == MTD_DEROUL_HERRAEZ)
-    return set_err(ctx, MTD_ERR_UNSUPPORTED,
-                   "DEROUL 1 with VOLKOV 0 (Herraez unwrapping) is not implemented");
-  if (g->deroul != MTD_DEROUL_OFF && g->deroul != MTD_DEROUL_VOLKOV)
+  if (g->deroul != MTD_DEROUL_OFF && g->deroul != MTD_DEROUL_VOLKOV &&
+      g->deroul != MTD_DEROUL_HERRAEZ)
     return set_err(ctx, MTD_ERR_ARG, "deroul must be MTD_DEROUL_OFF, _VOLKOV or _HERRAEZ");
   if (need & NEED_HOLO) {
     if (g->n <= 0 || g->n % 2)
@@ -568,7 +566,77 @@ static int field_mode(const mtd_geom* g) { return (g->born ? 1 : 0) | (g->c_aber
 // crops [n] (angles a0..) -> fields (out_fields) or unshifted spectra F (out_F)
 // part_in (nullable): argmax partials already computed by the hologram
 // column pass, S per angle; otherwise the argmax reads the crops
-static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int n,
+// rand() of glibc with its default seed (the reference never seeds it),
+// restated from glibc's published random_r, TYPE_3: a 31-word state seeded by
+// the Park-Miller recurrence 16807 x mod (2^31 - 1), r[i] = r[i-31] + r[i-3]
+// (mod 2^32), the first 310 words discarded, output r >> 1.
+struct GlibcRand {
+  uint32_t ring[34];
+  uint64_t i = 0;  // index of the next word
+  GlibcRand() {
+    int32_t r0 = 1;
+    ring[0] = 1;
+    for (int k = 1; k < 31; ++k) {
+      const int32_t hi = r0 / 127773, lo = r0 % 127773;
+      int32_t w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      ring[k] = (uint32_t)w;
+      r0 = w;
+    }
+    for (int k = 31; k < 34; ++k) ring[k] = ring[k - 31];
+    i = 34;
+    for (int k = 0; k < 310; ++k) word();
+  }
+  uint32_t word() {
+    const uint32_t v = ring[(i - 31) % 34] + ring[(i - 3) % 34];
+    ring[i % 34] = v;
+    ++i;
+    return v;
+  }
+  int32_t next() { return (int32_t)(word() >> 1); }
+};
+
+// Herraez border reliabilities of angles [a0, a0 + n) of a C x C stack:
+// initialisePIXELs (deroulement_herraez.cpp:348) gives every pixel
+// 9999999 + rand() (a 32-bit int sum, wrapping past INT_MAX) and only the
+// border pixels keep it; the program unwrapped a0 images before, each
+// drawing C*C numbers. Packed per angle as row 0, row C-1, then columns 0
+// and C-1 of rows 1 .. C-2. Cached on the device per (C, a0, n).
+const float* herraez_border_table(mtd_ctx* ctx, int C, int a0, int n) {
+  const auto key = std::make_tuple(C, a0, n);
+  auto it = ctx->herraez_border.find(key);
+  if (it != ctx->herraez_border.end()) return it->second;
+  const size_t per = 4 * (size_t)C - 4, npix = (size_t)C * C;
+  std::vector<float> t((size_t)n * per);
+  GlibcRand rng;
+  for (size_t k = 0; k < (size_t)a0 * npix; ++k) rng.word();
+  for (int a = 0; a < n; ++a) {
+    float* o = t.data() + (size_t)a * per;
+    for (int y = 0; y < C; ++y)
+      for (int x = 0; x < C; ++x) {
+        const int32_t r = rng.next();
+        const float v = (float)(int32_t)((uint32_t)r + 9999999u);
+        if (y == 0) o[x] = v;
+        else if (y == C - 1) o[C + x] = v;
+        else if (x == 0) o[2 * C + 2 * (y - 1)] = v;
+        else if (x == C - 1) o[2 * C + 2 * (y - 1) + 1] = v;
+      }
+  }
+  float* d = upload(t.data(), t.size());
+  if (!d) return nullptr;
+  if (ctx->herraez_border.size() >= 64) {  // bounded cache: drop all, after queued work
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->herraez_border) (void)hipFree(kv.second);
+    ctx->herraez_border.clear();
+    ++ctx->ws_gen;
+  }
+  ctx->herraez_border[key] = d;
+  return d;
+}
+
+// a0: index of the chunk's first angle in the caller's stack (Herraez
+// unwrapping's border reliabilities depend on it)
+static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int n, int a0,
                        int* posx, int* posy, float2* out_fields, float2* out_F,
                        const float2* part_in = nullptr, int S = 0) {
   const int C = 2 * g->nxmax;
@@ -605,6 +673,14 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
     float2* Q = P + D2;
     MTD_TRY(hip_check(ctx, launch_unwrap(C, iu, n, tw2, gx, gy, P, Q, phu, ctx->stream),
                       "phase unwrapping"));
+  } else if (g->c_aber && g->deroul == MTD_DEROUL_HERRAEZ) {
+    const float* border = herraez_border_table(ctx, C, a0, n);
+    phu = (float*)ws_get(ctx, WS_PHU, (size_t)n * C * C * sizeof(float));
+    void* hs = ws_get(ctx, WS_UNWRAP, herraez_scratch_bytes(C, n));
+    if (!border || !phu || !hs)
+      return set_err(ctx, MTD_ERR_HIP, "out of device memory (unwrapping)");
+    MTD_TRY(hip_check(ctx, launch_herraez(C, iu, n, border, hs, phu, ctx->stream),
+                      "phase unwrapping"));
   }
   if (g->c_aber)
     MTD_TRY(hip_check(ctx, launch_lsq(C, iu, phu, n, *ctx->lsq, coef, ctx->stream), "aberration fit"));
@@ -638,6 +714,8 @@ static size_t field_bytes(const mtd_geom* g) {
   size_t b = 3 * C * C * sizeof(float2);
   if (g->c_aber && g->deroul == MTD_DEROUL_VOLKOV)
     b += unwrap_scratch_bytes((int)C, 1) + C * C * sizeof(float);
+  if (g->c_aber && g->deroul == MTD_DEROUL_HERRAEZ)
+    b += herraez_scratch_bytes((int)C, 1) + C * C * sizeof(float);
   return b;
 }
 
@@ -770,6 +848,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
   for (auto& kv : ctx->gen_plans) delete kv.second;
+  for (auto& kv : ctx->herraez_border) (void)hipFree(kv.second);
   for (auto& kv : ctx->bluestein) {
     (void)hipFree(kv.second.first);
     (void)hipFree(kv.second.second);
@@ -926,7 +1005,7 @@ int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(front_chunk(ctx, g, (const float2*)crops + (size_t)a0 * C * C, n, posspec + a0,
+    MTD_TRY(front_chunk(ctx, g, (const float2*)crops + (size_t)a0 * C * C, n, a0, posspec + a0,
                         posspec + A + a0, (float2*)fields + (size_t)a0 * C * C, nullptr));
   }
   return MTD_OK;
@@ -949,7 +1028,7 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
-    MTD_TRY(front_chunk(ctx, g, crop, n, posspec + a0, posspec + A + a0,
+    MTD_TRY(front_chunk(ctx, g, crop, n, a0, posspec + a0, posspec + A + a0,
                         (float2*)fields + (size_t)a0 * C * C, nullptr, part, S));
   }
   return MTD_OK;
@@ -1482,7 +1561,7 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
     MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
-    MTD_TRY(front_chunk(ctx, g, crop, n, pos + a0, pos + A + a0, nullptr,
+    MTD_TRY(front_chunk(ctx, g, crop, n, a0, pos + a0, pos + A + a0, nullptr,
                         F + (size_t)a0 * C * C, part, S));
   }
   // The gather writes the count-normalised, centring-signed spectrum, and

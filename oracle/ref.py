@@ -378,7 +378,7 @@ def lsq_background(values, plan: LsqPlan):
 # --------------------------------------------------------------------------
 # A7, A10-A13: field formation
 # --------------------------------------------------------------------------
-def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
+def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None, angle=0):
     """main.cpp:254-341 for one angle, from the demodulated field U.
 
     C_ABER 1: phase = atan2 (fonctions.cpp:39-43); DEROUL 0 keeps it wrapped
@@ -396,8 +396,11 @@ def form_field(U, plan: LsqPlan, g: Geom, prec="f64", spec_value=None):
     phi = np.arctan2(U.imag, U.real).astype(rt)
     if g.deroul == DEROUL_VOLKOV:
         phi = volkov_unwrap(phi, prec)
+    elif g.deroul == DEROUL_HERRAEZ:
+        # float32 values (the reference's PIXEL.value), used as doubles next
+        phi = herraez_unwrap(phi, angle).astype(rt)
     elif g.deroul:
-        raise NotImplementedError("DEROUL 1 with VOLKOV 0 (Herraez) is not restated")
+        raise ValueError("deroul must be 0, 1 (Volkov) or 2 (Herraez)")
     amp = np.abs(U).astype(rt)
     bg_phi, _ = lsq_background(phi, plan)
     bg_amp, _ = lsq_background(amp, plan)
@@ -516,6 +519,176 @@ def volkov_unwrap(phi, prec="f64"):
     return (np.asarray(phi, dtype=rt) + two_pi * intm).astype(rt)
 
 
+# --------------------------------------------------------------------------
+# A8: phase unwrapping, DEROUL 1 with VOLKOV 0 (Herraez: phaseUnwrapping_Mat,
+# pretraitement_hors_axe_CPU/deroulement_herraez.cpp:32-75)
+# --------------------------------------------------------------------------
+HERR_PI = np.float32(3.141592654)      # deroulement_herraez.cpp:9 (float)
+HERR_TWOPI = np.float32(6.283185307)   # :10
+
+
+def glibc_rand(start, count):
+    """rand() of glibc with its default seed 1 (the reference never calls
+    srand): outputs start .. start+count-1 of the TYPE_3 additive generator,
+    restated from glibc's published random_r: a 31-word state seeded by the
+    Park-Miller recurrence 16807*x mod (2^31-1), r[i] = r[i-31] + r[i-3]
+    (mod 2^32) with the first 310 words discarded, output r >> 1."""
+    total = start + count
+    r = [0] * (344 + total)
+    r[0] = 1
+    for i in range(1, 31):
+        hi, lo = divmod(r[i - 1], 127773)
+        w = 16807 * lo - 2836 * hi
+        r[i] = w + 2147483647 if w < 0 else w
+    for i in range(31, 34):
+        r[i] = r[i - 31]
+    for i in range(34, 344 + total):
+        r[i] = (r[i - 31] + r[i - 3]) & 0xFFFFFFFF
+    return np.array([x >> 1 for x in r[344 + start:344 + total]], dtype=np.int64)
+
+
+def herraez_border_reliability(rand_vals):
+    """initialisePIXELs (deroulement_herraez.cpp:335-356): every pixel starts
+    with reliability 9999999 + rand() as a C int (wrapping past INT_MAX, as
+    the 32-bit addition does), stored in a float; only border pixels keep it."""
+    v = (rand_vals + 9999999) & 0xFFFFFFFF
+    v = np.where(v >= 2 ** 31, v - 2 ** 32, v)
+    return v.astype(np.float32)
+
+
+def _herr_wrap(x):
+    """gamma (deroulement_herraez.cpp:360-367), float arithmetic."""
+    x = x.astype(np.float32)
+    return np.where(x > HERR_PI, x - HERR_TWOPI,
+                    np.where(x < -HERR_PI, x + HERR_TWOPI, x)).astype(np.float32)
+
+
+def herraez_reliability(W, border):
+    """calculate_reliability (:382-404): interior pixels get H^2 + V^2 + D1^2
+    + D2^2 in float, each second difference from double differences of the
+    wrapped image rounded to float; border pixels keep `border` (row-major
+    image of the initial values)."""
+    Wd = np.asarray(W, np.float64)
+    c = Wd[1:-1, 1:-1]
+    d = lambda a, b: _herr_wrap((a - b).astype(np.float32))
+    H = d(Wd[1:-1, :-2], c) - d(c, Wd[1:-1, 2:])
+    V = d(Wd[:-2, 1:-1], c) - d(c, Wd[2:, 1:-1])
+    D1 = d(Wd[:-2, :-2], c) - d(c, Wd[2:, 2:])
+    D2 = d(Wd[:-2, 2:], c) - d(c, Wd[2:, :-2])
+    rel = np.array(border, dtype=np.float32).copy()
+    rel[1:-1, 1:-1] = ((H * H + V * V) + D1 * D1) + D2 * D2
+    return rel
+
+
+def herraez_edges(v, rel):
+    """horizentalEDGEs / verticalEDGEs (:409-452): all horizontal edges in
+    row-major order, then all vertical ones; reliab = float sum of the two
+    pixels' reliabilities; increment = find_wrap (:370-380) of the float
+    values. Returns (p1, p2, reliab, increment)."""
+    Hh, Ww = v.shape
+    idx = np.arange(Hh * Ww).reshape(Hh, Ww)
+    p1 = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    p2 = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    vf, rf = v.ravel().astype(np.float32), rel.ravel().astype(np.float32)
+    reliab = (rf[p1] + rf[p2]).astype(np.float32)
+    diff = (vf[p1] - vf[p2]).astype(np.float32)
+    inc = np.where(diff > HERR_PI, -1, np.where(diff < -HERR_PI, 1, 0)).astype(np.int64)
+    return p1, p2, reliab, inc
+
+
+def _herr_mix(keys, run_l, run_r):
+    """Mix (:113-157): the two-pointer merge of two runs (index arrays) by
+    key, the left element first on ties. The runs need not be sorted (odd
+    sizes leave elements out of the recursion, below); such a merge orders
+    elements exactly as a stable merge of the runs' running maxima."""
+    kl = np.maximum.accumulate(keys[run_l]) if len(run_l) else keys[run_l]
+    kr = np.maximum.accumulate(keys[run_r]) if len(run_r) else keys[run_r]
+    order = np.argsort(np.concatenate([kl, kr]), kind="stable")
+    return np.concatenate([run_l, run_r])[order]
+
+
+def herraez_sort(keys):
+    """quick_sort (:176-195) -> sort (:160-174): a recursive merge sort of
+    the edge indices that splits a range of `size` into two halves of
+    size // 2, so an odd range leaves its last element where it is, and size
+    2 is a compare-and-swap. Returns the processed order (edge indices)."""
+    keys = np.asarray(keys)
+    idx = np.arange(len(keys))
+
+    def rec(start, size):
+        if size == 2:
+            if keys[idx[start]] > keys[idx[start + 1]]:
+                idx[start], idx[start + 1] = idx[start + 1], idx[start]
+        elif size > 2:
+            h = size // 2
+            rec(start, h)
+            rec(start + h, h)
+            idx[start:start + 2 * h] = _herr_mix(keys, idx[start:start + h].copy(),
+                                                 idx[start + h:start + 2 * h].copy())
+
+    rec(0, len(keys))
+    return idx
+
+
+def herraez_gather(order, p1, p2, inc, npix):
+    """gatherPIXELs (:455-552): edges in the sorted order; an edge joining two
+    groups merges them and fixes the number of 2 pi between them: a pixel
+    alone joins the other's group (PIXEL 2 tested first), else the group
+    with more pixels absorbs the other (PIXEL 2's on equal counts), its
+    members' increments shifted. Returns the increments."""
+    head = list(range(npix))
+    members = [[p] for p in range(npix)]
+    incr = [0] * npix
+    for e in order:
+        a, b, w = int(p1[e]), int(p2[e]), int(inc[e])
+        ha, hb = head[a], head[b]
+        if ha == hb:
+            continue
+        if len(members[hb]) == 1:                  # PIXEL 2 alone
+            members[ha].append(b)
+            members[hb] = None
+            head[b] = ha
+            incr[b] = incr[a] - w
+        elif len(members[ha]) == 1:                # PIXEL 1 alone
+            members[hb].append(a)
+            members[ha] = None
+            head[a] = hb
+            incr[a] = incr[b] + w
+        elif len(members[ha]) > len(members[hb]):  # group 2 into group 1
+            d = incr[a] - w - incr[b]
+            for q in members[hb]:
+                head[q] = ha
+                incr[q] += d
+            members[ha].extend(members[hb])
+            members[hb] = None
+        else:                                      # group 1 into group 2
+            d = incr[b] + w - incr[a]
+            for q in members[ha]:
+                head[q] = hb
+                incr[q] += d
+            members[hb].extend(members[ha])
+            members[ha] = None
+    return np.asarray(incr, dtype=np.int64)
+
+
+def herraez_unwrap(phi, angle=0):
+    """phaseUnwrapping_Mat (:32-75) of one C x C wrapped phase: reliability,
+    edges, the reference's sort, the grouping, then value + TWOPI * increment
+    in float (unwrapImage :555-566). `angle`: how many images the program
+    unwrapped before this one (each consumed one rand() per pixel). Returns
+    float32."""
+    Hh, Ww = phi.shape
+    npix = Hh * Ww
+    border = herraez_border_reliability(glibc_rand(angle * npix, npix)).reshape(Hh, Ww)
+    v = np.asarray(phi).astype(np.float32)
+    rel = herraez_reliability(phi, border)
+    p1, p2, reliab, inc = herraez_edges(v, rel)
+    order = herraez_sort(reliab)
+    incr = herraez_gather(order, p1, p2, inc, npix)
+    out = v.ravel() + HERR_TWOPI * incr.astype(np.float32)
+    return out.astype(np.float32).reshape(Hh, Ww)
+
+
 def load_mask(C, mask=None):
     """init_mask_aber (Correction_aberration.cpp:43-70): absent mask => 255."""
     if mask is None:
@@ -542,7 +715,7 @@ def preprocess(holos, g: Geom, mask=None, prec="f64", return_crops=False):
         px, py = i % C, i // C
         pos[a], pos[a + A] = px, py
         U = demod(crop, px, py, prec)
-        fields[a] = form_field(U, plan, g, prec, spec_value=crop[py, px])
+        fields[a] = form_field(U, plan, g, prec, spec_value=crop[py, px], angle=a)
     if return_crops:
         return fields, pos, crops
     return fields, pos

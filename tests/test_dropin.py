@@ -31,7 +31,7 @@ def write_pgm8(path, img):
         f.write(img.tobytes())
 
 
-def setup_home(tmp_path, case, drop=(), deroul=0, born=1):
+def setup_home(tmp_path, case, drop=(), deroul=0, born=1, volkov=1):
     home, acq, res, cfg = (tmp_path / d for d in ("home", "acquis", "result", "config"))
     for d in (home / ".config", acq, res, cfg):
         d.mkdir(parents=True)
@@ -44,7 +44,7 @@ def setup_home(tmp_path, case, drop=(), deroul=0, born=1):
         f"NXMAX {p['nxmax']}\nCIRCLE_CX {p['cx']}\nCIRCLE_CY {p['cy']}\n"
         f"NB_HOLO {len(case.acq.holos)}\n")
     (cfg / "recon.txt").write_text(
-        f"CCD_ROIX {p['n']}\nDEROUL {deroul}\nVOLKOV 1\nC_ABER 1\nBORN {born}\nNB_THREADS 4\n"
+        f"CCD_ROIX {p['n']}\nDEROUL {deroul}\nVOLKOV {volkov}\nC_ABER 1\nBORN {born}\nNB_THREADS 4\n"
         f"PREMIER_ANGLE 0\nDIM_FINAL {p['v']}\nEXPORT_OTF 1\n")
     for k, h in enumerate(case.acq.holos):
         if k not in drop:
@@ -66,12 +66,14 @@ def test_help_and_missing_inputs(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("deroul,born", [(0, 1), (1, 0)])  # (1, 0): the shipped recon.txt
-def test_programs_round_trip(tmp_path, deroul, born):
-    case = make_case("small", born=born, deroul=deroul)
+@pytest.mark.parametrize("deroul,born,volkov", [(0, 1, 1), (1, 0, 1), (1, 0, 0)])
+def test_programs_round_trip(tmp_path, deroul, born, volkov):
+    """(1, 0, 1): the shipped recon.txt (Volkov); (1, 0, 0): Herraez."""
+    case = make_case("small", born=born, deroul=0 if not deroul else (1 if volkov else 2))
     drop = (3,)  # a missing hologram is skipped and the angles compacted
-    print(f"drop-in round trip case=small deroul={deroul} born={born}", flush=True)
-    env, res = setup_home(tmp_path, case, drop, deroul, born)
+    print(f"drop-in round trip case=small deroul={deroul} born={born} volkov={volkov}",
+          flush=True)
+    env, res = setup_home(tmp_path, case, drop, deroul, born, volkov)
     r = subprocess.run([PRE], env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     keep = [k for k in range(len(case.acq.holos)) if k not in drop]

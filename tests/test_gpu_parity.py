@@ -130,19 +130,55 @@ def test_crop_to_field_volkov(gpu_ctx, name, born):
     assert e32 < TOL and e64 <= fp64_bound(fields32, fields64)
 
 
-def test_herraez_is_unsupported(gpu_ctx):
-    import mtd_transmission_amd as m
+@pytest.mark.parametrize("name,born", [("small", 1), ("small", 0), ("u16", 1), ("u16", 0),
+                                       ("cfg1", 1), ("cfg1", 0)])
+def test_crop_to_field_herraez(gpu_ctx, name, born):
+    """DEROUL 1 / VOLKOV 0: Herraez unwrapping (phaseUnwrapping_Mat) before
+    the fit, against the oracle's literal restatement (the reference's sort,
+    grouping and border rand() values): the GPU builds the same spanning
+    tree (Boruvka on the sort positions); the unwrapped phase may differ by
+    one global 2 pi k, which the fit's constant term removes. At most 20
+    angles (the oracle's grouping is a Python loop)."""
     import torch
-    case = make_case("small")
-    p = case.params
-    g = m.geom_derive(p["n"], p["nxmax"], p["v"], p["cx"], p["cy"], 1.2, 1.333, 6.33e-7,
-                      p["pix"], deroul=2)
-    A, C = 2, g.C
-    crops = _empty((A, C, C), torch.complex64)
+    case = make_case(name, born=born, deroul=2)
+    _say(f"crop_to_field herraez case={name} born={born}")
+    holos = case.acq.holos[:20]
+    g = case.lib_geom()
+    f32, p32, crops32 = ref.preprocess(holos, case.g, case.acq.mask, "f32", return_crops=True)
+    f64, _ = ref.preprocess(holos, case.g, case.acq.mask, "f64")
+    A, C = crops32.shape[0], g.C
     pos = _empty((2 * A,), torch.int32)
     fields = _empty((A, C, C), torch.complex64)
-    with pytest.raises(m.MTDError, match="Herraez"):
-        gpu_ctx.crop_to_field(g, crops, case.acq.mask, pos, fields)
+    gpu_ctx.crop_to_field(g, _t(crops32), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), p32)
+    got = fields.cpu().numpy()
+    e32, e64 = rel_l2(got, f32), rel_l2(got, f64)
+    _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}  (f32 oracle vs f64 "
+         f"{rel_l2(f32, f64):.3e})")
+    assert e32 < TOL and e64 <= fp64_bound(f32, f64)
+
+
+def test_reconstruct_plain_herraez(gpu_ctx):
+    """The whole chain with Herraez unwrapping (20 angles of the 100 x 512^2
+    case, Born): speculars bit-exact, index within 1e-5 of the FP32 oracle."""
+    import torch
+    case = make_case("cfg1", deroul=2)
+    _say("reconstruct_plain herraez case=cfg1[:20]")
+    holos = case.acq.holos[:20]
+    g = case.lib_geom()
+    f32, p32 = ref.preprocess(holos, case.g, case.acq.mask, "f32")
+    idx32 = ref.reconstruct(f32, p32, case.g, "f32")[0]
+    V, A = g.dim_final, holos.shape[0]
+    re = _empty((V, V, V), torch.float32)
+    im = _empty((V, V, V), torch.float32)
+    pos = _empty((2 * A,), torch.int32)
+    gpu_ctx.reconstruct_plain(g, _t(holos), case.acq.mask, re, im, pos)
+    torch.cuda.synchronize()
+    assert np.array_equal(pos.cpu().numpy(), p32)
+    e32 = rel_l2(re.cpu().numpy() + 1j * im.cpu().numpy(), idx32)
+    _say(f"  index rel-L2 vs f32 {e32:.3e}")
+    assert e32 < TOL
 
 
 def test_schedules_bit_identical(gpu_ctx):

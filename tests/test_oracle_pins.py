@@ -487,3 +487,94 @@ def test_cpu_baseline_pipeline_equals_oracle():
     # FP64 rounding of another FFT library, through the normal-equation fit
     assert rel_l2(idx, want) < 1e-9
     assert set(t) == {"T_pre1", "T_pre2", "T_rec1", "T_rec2"} and name
+
+
+# --------------------------------------------------------------------------
+# Herraez unwrapping (DEROUL 1, VOLKOV 0)
+# --------------------------------------------------------------------------
+def test_glibc_rand_known_answers():
+    """rand() with glibc's default seed: the first outputs every glibc
+    program prints (1804289383, 846930886, ...); any start offset continues
+    the same stream."""
+    assert list(ref.glibc_rand(0, 5)) == [1804289383, 846930886, 1681692777, 1714636915,
+                                          1957747793]
+    assert list(ref.glibc_rand(3, 4)) == list(ref.glibc_rand(0, 7)[3:])
+
+
+def test_herraez_border_reliability_wraps_like_int():
+    """9999999 + rand() is a 32-bit int sum: past INT_MAX it wraps negative."""
+    r = ref.herraez_border_reliability(np.array([0, 2137483647, 2137483649, 2147483647]))
+    assert list(r) == [np.float32(9999999), np.float32(2147483646),
+                       np.float32(-2147483648), np.float32(-2137483650)]
+
+
+def _literal_mix(keys, L, R):
+    i = j = 0
+    out = []
+    while i < len(L) and j < len(R):
+        if keys[L[i]] <= keys[R[j]]:
+            out.append(L[i]); i += 1
+        else:
+            out.append(R[j]); j += 1
+    return out + list(L[i:]) + list(R[j:])
+
+
+def test_herraez_mix_equals_two_pointer_merge():
+    """The oracle's merge (stable merge of running maxima) against the
+    literal two-pointer merge on unsorted runs with ties."""
+    rng = np.random.default_rng(3)
+    for _ in range(3000):
+        h = int(rng.integers(1, 9))
+        keys = rng.integers(0, 4, 2 * h).astype(np.float32)
+        L, R = np.arange(h), np.arange(h, 2 * h)
+        assert list(ref._herr_mix(keys, L, R)) == _literal_mix(keys, L, R)
+
+
+def test_herraez_sort_known_answers():
+    """Hand-traced runs of the reference's sort: an odd range leaves its last
+    element in place and merges unsorted halves."""
+    f = lambda k: list(ref.herraez_sort(np.array(k, np.float32)))
+    assert f([3, 1, 2]) == [1, 0, 2]
+    assert f([5, 4, 3, 2, 1]) == [3, 2, 1, 0, 4]
+    assert f([6, 5, 4, 3, 2, 1]) == [4, 3, 5, 1, 0, 2]
+    assert f([1, 2, 3, 4]) == [0, 1, 2, 3]
+    assert f([2, 2, 1, 1]) == [2, 3, 0, 1]  # ties: left run first
+
+
+def test_herraez_tree_is_the_rank_mst():
+    """The grouping merges exactly the edges of the minimum spanning tree
+    with the sort position as weight (scipy's MST on the same graph), and
+    the increments satisfy inc(p2) = inc(p1) - w on every tree edge."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    rng = np.random.default_rng(8)
+    C = 12
+    phi = np.angle(np.exp(1j * rng.normal(scale=2.0, size=(C, C))))  # residues
+    v = phi.astype(np.float32)
+    rel = ref.herraez_reliability(phi, np.full((C, C), 1e7, np.float32))
+    p1, p2, reliab, inc = ref.herraez_edges(v, rel)
+    order = ref.herraez_sort(reliab)
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    mst = minimum_spanning_tree(coo_matrix((rank + 1.0, (p1, p2)), shape=(C * C, C * C))).tocoo()
+    tree = {(min(a, b), max(a, b)) for a, b in zip(mst.row, mst.col)}
+    incr = ref.herraez_gather(order, p1, p2, inc, C * C)
+    in_tree = [e for e in range(len(p1)) if (p1[e], p2[e]) in tree]
+    assert len(in_tree) == C * C - 1
+    for e in in_tree:
+        assert incr[p2[e]] == incr[p1[e]] - inc[e]
+    assert (incr == 0).any()  # the last surviving group head
+
+
+def test_herraez_smooth_phase_recovery():
+    """Smooth phases spanning several wraps (bowl + ramp, Gaussian bump) come
+    back to float precision, up to one multiple of 2 pi."""
+    C = 48
+    y, x = np.mgrid[0:C, 0:C].astype(np.float64)
+    for true in (0.004 * ((x - 17) ** 2 + (y - 30) ** 2) + 0.35 * x,
+                 14.0 * np.exp(-((x - 24) ** 2 + (y - 20) ** 2) / 150.0)):
+        for prec in (np.float64, np.float32):
+            wr = np.angle(np.exp(1j * true)).astype(prec)
+            u = ref.herraez_unwrap(wr, 3).astype(np.float64)
+            k = np.round(np.mean(u - true) / (2 * np.pi))
+            assert np.abs(u - true - 2 * np.pi * k).max() < 2e-5
