@@ -83,6 +83,10 @@ typedef struct mtd_geom {
   double coef_na;      /* COEF_NA_OBJ_LIMIT (default 1)                    */
   double alpha_holo;   /* Tukey alpha of the hologram window (0.05)        */
   double alpha_field;  /* Tukey alpha of the field window (0.1)            */
+  double alpha_crop;   /* ArrayFire variant (pretraitement_hors_axe_GPU/
+                          main.cpp:74-79, src/GPU_fonctions.cpp:122): the crop
+                          times (1+i) tukey2D(C, alpha_crop); 0 = off (the
+                          CPU program, mtd_geom_derive's default)          */
 } mtd_geom;
 
 /* ---- context ---------------------------------------------------------- */

@@ -36,6 +36,7 @@ class Geom(ctypes.Structure):
         ("rayon", ctypes.c_double), ("pix_uborn", ctypes.c_double),
         ("delta_f", ctypes.c_double), ("coef_na", ctypes.c_double),
         ("alpha_holo", ctypes.c_double), ("alpha_field", ctypes.c_double),
+        ("alpha_crop", ctypes.c_double),
     ]
 
     @property

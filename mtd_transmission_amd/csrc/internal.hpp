@@ -213,6 +213,8 @@ hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const FftTab& ft,
                             int C, int x0, int y0, float2* crop, float2* part,
                             hipStream_t s);
+// the ArrayFire variant's crop window (1+i) win[x] win[y] on A crops
+hipError_t launch_crop_window(int C, float2* crop, int A, const double* win, hipStream_t s);
 // argmax partials per angle the column pass writes (0: none, the field
 // stage then finds the specular from the crops)
 int holo_argmax_parts(int N, int C);

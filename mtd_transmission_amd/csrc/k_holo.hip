@@ -305,6 +305,27 @@ static hipError_t rows_gen(int N, int dtype, const void* holos, int A, const flo
   }
 }
 
+// The ArrayFire variant's crop window (pretraitement_hors_axe_GPU
+// src/GPU_fonctions.cpp:122): crop * af::complex(T, T), T(x, y) the product
+// of the float-rounded 1-D factors, i.e. (re - im, re + im) * T.
+__global__ void __launch_bounds__(256)
+k_crop_window(float2* __restrict__ crop, int C, const double* __restrict__ win) {
+  const int a = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= C * C) return;
+  const int y = i / C, x = i - y * C;
+  const float w = (float)win[x] * (float)win[y];
+  float2* c = crop + (size_t)a * C * C + i;
+  const float2 v = *c;
+  *c = make_float2((v.x - v.y) * w, (v.x + v.y) * w);
+}
+
+hipError_t launch_crop_window(int C, float2* crop, int A, const double* win, hipStream_t s) {
+  if (A <= 0) return hipSuccess;
+  Timed tm("crop_window", s);
+  hipLaunchKernelGGL(k_crop_window, dim3((C * C + 255) / 256, A), dim3(256), 0, s, crop, C, win);
+  return hipGetLastError();
+}
+
 #define MTD_HOLO_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
 hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,

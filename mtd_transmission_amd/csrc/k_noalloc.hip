@@ -105,6 +105,9 @@ extern "C" int mtd_holo_to_crop_noalloc(mtd_ctx* ctx, const mtd_geom* g, const v
                      "allocation-free hologram stage: the stack must be in device memory");
   }
   const int N = g->n;
+  if (g->alpha_crop > 0)
+    return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                   "allocation-free hologram stage: no crop window (alpha_crop 0)");
   if (!holo_len_supported(N))
     return set_err(ctx, MTD_ERR_UNSUPPORTED,
                    "allocation-free hologram stage: hologram side " + std::to_string(N) +

@@ -476,6 +476,8 @@ int validate(mtd_ctx* ctx, const mtd_geom* g, int A, int need) {
     return set_err(ctx, MTD_ERR_UNSUPPORTED,
                    "unwrapping length 4*NXMAX=" + std::to_string(2 * C) +
                        " beyond the FFT lengths built");
+  if (!(g->alpha_crop >= 0) || g->alpha_crop > 1)
+    return set_err(ctx, MTD_ERR_ARG, "alpha_crop must be in [0, 1]");
   if (g->deroul != MTD_DEROUL_OFF && g->deroul != MTD_DEROUL_VOLKOV &&
       g->deroul != MTD_DEROUL_HERRAEZ)
     return set_err(ctx, MTD_ERR_ARG, "deroul must be MTD_DEROUL_OFF, _VOLKOV or _HERRAEZ");
@@ -548,7 +550,19 @@ int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtyp
                                           g->circle_cy - g->nxmax, crop_out, part_out,
                                           ctx->stream),
                     "hologram column pass"));
+  if (g->alpha_crop > 0) {
+    const double* wc = tukey_table(ctx, C, g->alpha_crop);
+    if (!wc) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop window)");
+    MTD_TRY(hip_check(ctx, launch_crop_window(C, crop_out, n, wc, ctx->stream), "crop window"));
+  }
   return MTD_OK;
+}
+
+// argmax partials the hologram stage can hand to the field stage: none when
+// a crop window follows the column pass (the specular is then found on the
+// windowed crops, as the ArrayFire variant's coordSpec sees them)
+static int fused_argmax_parts(const mtd_geom* g) {
+  return g->alpha_crop > 0 ? 0 : holo_argmax_parts(g->n, 2 * g->nxmax);
 }
 
 static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
@@ -981,6 +995,7 @@ int mtd_geom_derive(double ccd_roix, double nxmax, double dim_final, double circ
   g.coef_na = (float)coef_na;
   g.alpha_holo = 0.05;
   g.alpha_field = 0.1;
+  g.alpha_crop = 0.0;
   *out = g;
   return NX ? MTD_OK : MTD_ERR_ARG;
 }
@@ -1020,14 +1035,14 @@ int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype
   const int C = 2 * g->nxmax;
   if (g->c_aber) MTD_TRY(lsq_prepare(ctx, C, mask));
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, A);
-  const int S = holo_argmax_parts(g->n, C);
+  const int S = fused_argmax_parts(g);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   float2* part =
       (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * std::max(S, 1) * sizeof(float2));
   if (!crop || !part) return set_err(ctx, MTD_ERR_HIP, "out of device memory (crop stack)");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, S > 0 ? part : nullptr, S));
     MTD_TRY(front_chunk(ctx, g, crop, n, a0, posspec + a0, posspec + A + a0,
                         (float2*)fields + (size_t)a0 * C * C, nullptr, part, S));
   }
@@ -1550,7 +1565,7 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   }
   float2* F = (float2*)ws_get(ctx, WS_F, (size_t)std::max(A, 1) * C * C * sizeof(float2));
   const int chunk = chunk_of(field_bytes(g), kFieldBudget, std::max(A, 1));
-  const int S = holo_argmax_parts(g->n, C);
+  const int S = fused_argmax_parts(g);
   float2* crop = (float2*)ws_get(ctx, WS_CROP, (size_t)chunk * C * C * sizeof(float2));
   float2* part =
       (float2*)ws_get(ctx, WS_HOLOPART, (size_t)chunk * std::max(S, 1) * sizeof(float2));
@@ -1560,7 +1575,7 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
     return set_err(ctx, MTD_ERR_HIP, "out of device memory");
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, part, S));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, S > 0 ? part : nullptr, S));
     MTD_TRY(front_chunk(ctx, g, crop, n, a0, pos + a0, pos + A + a0, nullptr,
                         F + (size_t)a0 * C * C, part, S));
   }

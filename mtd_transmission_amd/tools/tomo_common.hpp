@@ -20,6 +20,10 @@ struct Args {
   std::string etat;
   bool polar = false;
   bool help = false;
+  // -arrayfire (no reference counterpart): the windows of the ArrayFire
+  // variant pretraitement_hors_axe_GPU (main.cpp:74-79): alpha 0.025 for the
+  // hologram and a (1+i) tukey2D(C, 0.025) crop window
+  bool arrayfire = false;
 };
 
 // Order-sensitive argv loop of pretraitement_hors_axe_CPU/main.cpp:37-61 and
@@ -38,6 +42,8 @@ inline Args parse_args(int argc, char** argv) {
     } else if (arg == "-etat" && i + 1 < argc) {
       a.etat = argv[++i];
       a.conf = "gui_tomo_polar.conf";
+    } else if (arg == "-arrayfire") {
+      a.arrayfire = true;
     } else {
       a.conf = "gui_tomo.conf";
     }
@@ -47,7 +53,8 @@ inline Args parse_args(int argc, char** argv) {
 
 inline void print_help() {
   std::cout << "-polar : mode polar\n-etat : indiquez un des 4 états :\n"
-               "/LC/polar0/\n/LC/polar90/\n/RC/polar0/\n/RC/polar90/\n";
+               "/LC/polar0/\n/LC/polar90/\n/RC/polar0/\n/RC/polar90/\n"
+               "-arrayfire : fenêtres de la variante ArrayFire (alpha 0.025, découpe (1+i))\n";
 }
 
 inline std::string gui_conf_path(const std::string& name) {

@@ -37,7 +37,7 @@ int main(int argc, char** argv) {
   const long nb_holo = (long)(size_t)val(cfg, "NB_HOLO");
   const int c_aber = val(recon, "C_ABER") != 0;
   // DEROUL / VOLKOV are read as booleans (manip.cpp:77-79); VOLKOV 0 selects
-  // Herraez, which the library reports as not built
+  // Herraez
   const int deroul = val(recon, "DEROUL") == 0 ? MTD_DEROUL_OFF
                      : val(recon, "VOLKOV") != 0 ? MTD_DEROUL_VOLKOV : MTD_DEROUL_HERRAEZ;
   const int born = val(recon, "BORN") != 0;
@@ -57,6 +57,10 @@ int main(int argc, char** argv) {
                       1.0, born, c_aber, deroul, &g) != MTD_OK) {
     std::cerr << "invalid geometry (NXMAX)\n";
     return 1;
+  }
+  if (args.arrayfire) {  // pretraitement_hors_axe_GPU/main.cpp:74-79
+    g.alpha_holo = 0.025;
+    g.alpha_crop = 0.025;
   }
   const int N = g.n, C = 2 * g.nxmax;
   const std::string sav_param = chemin_result + "/SAV_param_manip.txt";

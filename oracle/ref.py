@@ -229,6 +229,10 @@ class Geom:
     deroul: int = 0
     alpha_holo: float = 0.05   # main.cpp:105
     alpha_field: float = 0.1   # main.cpp:180, Reconstruction/main.cpp:185
+    # the ArrayFire variant's crop window (1+i) tukey2D(C, alpha_crop)
+    # (pretraitement_hors_axe_GPU/main.cpp:74-79, src/GPU_fonctions.cpp:122);
+    # 0: none (the CPU program)
+    alpha_crop: float = 0.0
 
     @property
     def C(self):
@@ -281,7 +285,15 @@ def holo_to_crop(holo, g: Geom, prec="f64", window=None):
     spec = (spec / rt(N * N)).astype(ct)
     spec = fftshift2(spec)
     x0, y0 = g.coin
-    return spec[y0:y0 + C, x0:x0 + C].copy()
+    crop = spec[y0:y0 + C, x0:x0 + C].copy()
+    if g.alpha_crop:
+        # crop * af::complex(T, T): (re - im, re + im) * T, T the separable
+        # window (products of the float-rounded factors in FP32)
+        t = tukey1d(C, g.alpha_crop)[0]
+        w = (t.astype(np.float32)[None, :] * t.astype(np.float32)[:, None]).astype(rt) \
+            if prec == "f32" else (t[None, :] * t[:, None])
+        crop = _mk((crop.real - crop.imag) * w, (crop.real + crop.imag) * w, ct)
+    return crop
 
 
 # --------------------------------------------------------------------------

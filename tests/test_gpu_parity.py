@@ -159,6 +159,34 @@ def test_crop_to_field_herraez(gpu_ctx, name, born):
     assert e32 < TOL and e64 <= fp64_bound(f32, f64)
 
 
+@pytest.mark.parametrize("name", ["small", "u16"])
+def test_arrayfire_windows(gpu_ctx, name):
+    """The ArrayFire variant's windows (alpha 0.025 for the hologram, the
+    (1+i) tukey2D(C, 0.025) crop window; pretraitement_hors_axe_GPU
+    main.cpp:74-79): crops within 1e-5 of the FP32 oracle, speculars
+    bit-exact (found on the windowed crops), fields within 1e-5."""
+    import dataclasses
+    import torch
+    case = make_case(name)
+    _say(f"arrayfire windows case={name}")
+    g = case.lib_geom()
+    g.alpha_holo = 0.025
+    g.alpha_crop = 0.025
+    go = dataclasses.replace(case.g, alpha_holo=0.025, alpha_crop=0.025)
+    A, C = case.acq.holos.shape[0], g.C
+    crops = _empty((A, C, C), torch.complex64)
+    gpu_ctx.holo_to_crop(g, _t(case.acq.holos), crops)
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.preprocess(g, _t(case.acq.holos), case.acq.mask, pos, fields)
+    torch.cuda.synchronize()
+    f32, p32, c32 = ref.preprocess(case.acq.holos, go, case.acq.mask, "f32", return_crops=True)
+    ec, ef = rel_l2(crops.cpu().numpy(), c32), rel_l2(fields.cpu().numpy(), f32)
+    _say(f"  crops rel-L2 {ec:.3e}, fields {ef:.3e}")
+    assert np.array_equal(pos.cpu().numpy(), p32)
+    assert ec < TOL and ef < TOL
+
+
 def test_reconstruct_plain_herraez(gpu_ctx):
     """The whole chain with Herraez unwrapping (20 angles of the 100 x 512^2
     case, Born): speculars bit-exact, index within 1e-5 of the FP32 oracle."""

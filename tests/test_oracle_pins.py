@@ -578,3 +578,48 @@ def test_herraez_smooth_phase_recovery():
             u = ref.herraez_unwrap(wr, 3).astype(np.float64)
             k = np.round(np.mean(u - true) / (2 * np.pi))
             assert np.abs(u - true - 2 * np.pi * k).max() < 2e-5
+
+
+def test_c_aber0_single_bin_closed_form():
+    """C_ABER 0 (main.cpp:328-341): a crop whose only nonzero bin is the
+    specular s demodulates to the constant s (the -posSpec shift moves it to
+    bin 0, the unnormalised inverse FFT spreads it, the NXMAX shift keeps it
+    constant); times conj(s)/|s| that is the constant |s|, real."""
+    from cases import make_case
+    case = make_case("small", c_aber=0)
+    g = case.g
+    C = g.C
+    px, py = 19, 7
+    s = 3.0 - 4.0j
+    crop = np.zeros((C, C), np.complex128)
+    crop[py, px] = s
+    assert ref.coord_spec(crop) == py * C + px
+    U = ref.demod(crop, px, py)
+    assert np.allclose(U, s, rtol=0, atol=1e-12)
+    plan = ref.LsqPlan.build(ref.load_mask(C, None))
+    f = ref.form_field(U, plan, g, "f64", spec_value=crop[py, px])
+    assert np.allclose(f, abs(s), rtol=0, atol=1e-12)
+
+
+def test_arrayfire_crop_window():
+    """The ArrayFire variant's crop (pretraitement_hors_axe_GPU
+    src/GPU_fonctions.cpp:122): the plain crop times (1+i) T(x) T(y), T the
+    pinned tukey1d factor: for a pure carrier the single peak becomes
+    (1+i) T(bx) T(by) times the plain peak, every other bin stays 0."""
+    import dataclasses
+    from cases import make_case
+    case = make_case("small")
+    N, C = case.g.n, case.g.C
+    ga = dataclasses.replace(case.g, alpha_holo=0.025, alpha_crop=0.025)
+    gp = dataclasses.replace(case.g, alpha_holo=0.025)
+    x0, y0 = ga.coin
+    bx, by = 5, 9  # crop bin of the carrier
+    kx, ky = x0 + bx - N // 2, y0 + by - N // 2
+    y, x = np.mgrid[0:N, 0:N]
+    h = 100 + np.cos(2 * np.pi * (kx * x + ky * y) / N)
+    plain = ref.holo_to_crop(h, gp)
+    af = ref.holo_to_crop(h, ga)
+    T = ref.tukey1d(C, 0.025)[0]
+    assert np.isclose(af[by, bx], (1 + 1j) * T[bx] * T[by] * plain[by, bx], rtol=1e-12)
+    w = np.outer(T, T)
+    assert np.allclose(af, (1 + 1j) * w * plain, rtol=0, atol=1e-12 * abs(plain).max())
