@@ -80,9 +80,10 @@ def kernel_bytes(name, n, C, V, A, P, M, B, b_in=2):
     per-hologram stage bytes of the measurement definition: S1 = b_in N^2 +
     8 C^2 (the hologram read by the row pass, the crop written by the column
     pass; the N x C row intermediate between them is not compulsory), S2 =
-    16 C^2 (the crop read by the demodulation's row pass, the spectrum
-    written by the forward column pass; the passes in between move
-    intermediates), S3 = 8 P (pupil samples read by the gather). Per volume
+    16 C^2 (the crop read by the demodulation's row pass -- in the fused
+    field stage the crop's column transform Q, the same size, read by the
+    one row kernel -- and the spectrum written by the forward column pass;
+    the passes in between move intermediates), S3 = 8 P (pupil samples read by the gather). Per volume
     the fused plain method writes only the B planes of the hit band of the
     count-normalised spectrum (8 B V^2; no counts), and the finalisation
     reads that band and writes the index and absorption volumes (8 V^3).
@@ -92,6 +93,7 @@ def kernel_bytes(name, n, C, V, A, P, M, B, b_in=2):
         "holo_rows": A * b_in * n * n,
         "holo_cols": A * 8 * C * C,
         "demod_rows": A * 8 * C * C,
+        "fwd_rows_field_q": A * 8 * C * C,
         "fwd_cols": A * 8 * C * C,
         "ewald_gather": A * 8 * P + 8 * B * V * V,
         "vol_x": 8 * B * V * V,
@@ -512,7 +514,7 @@ def main():
     # S4 finalisation (3-D inverse FFT to the index volume)
     stage_of = {"holo_rows": "S1", "holo_cols": "S1", "argmax": "S2", "argmax_fin": "S2",
                 "demod_rows": "S2", "demod_cols": "S2", "lsq": "S2", "fwd_rows_field": "S2",
-                "fwd_cols": "S2",
+                "fwd_cols": "S2", "sample_rows": "S2",
                 "angle_geo": "S3_acc", "ewald_band": "S3_acc", "ewald_tiles": "S3_acc",
                 "ewald_gather": "S3_acc", "vol_x": "S4", "vol_y": "S4", "vol_z": "S4"}
     stages = {}

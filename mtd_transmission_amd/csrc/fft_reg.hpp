@@ -221,6 +221,42 @@ struct Fft2 {
       RegFFT<L2, INV>::run(u);
     }
   }
+
+  // run with the transform's work confined to the threads where act holds
+  // (the barriers are still every thread's): for a block whose lines have
+  // more threads than this length's transform needs
+  template <class X>
+  __device__ __forceinline__ static void run_masked(float2 (&v)[L1], float2 (&u)[L2], int n2,
+                                                    X x, const float2* __restrict__ tw,
+                                                    bool act) {
+    if (act) {
+      if constexpr (L1 > 1) RegFFT<L1, INV>::run(v);
+#pragma unroll
+      for (int k1 = 1; k1 < L1; ++k1) {
+        float2 w = tw[n2 * k1];
+        if (INV) w.y = -w.y;
+        v[k1] = cmul(v[k1], w);
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < L1; ++k1) *x.at(k1, n2) = v[k1].x;
+    }
+    __syncthreads();
+    if (act && n2 < L1) {
+#pragma unroll
+      for (int n = 0; n < L2; ++n) u[n].x = *x.at(n2, n);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k1 = 0; k1 < L1; ++k1) *x.at(k1, n2) = v[k1].y;
+    }
+    __syncthreads();
+    if (act && n2 < L1) {
+#pragma unroll
+      for (int n = 0; n < L2; ++n) u[n].y = *x.at(n2, n);
+      RegFFT<L2, INV>::run(u);
+    }
+  }
 };
 
 // Moves a line from the output layout of Fft2::run (threads k1 < L1 hold

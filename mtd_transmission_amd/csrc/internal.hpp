@@ -178,7 +178,7 @@ void tukey_signed_host(int n, double alpha, float* out);
 size_t holo_t1_bytes(const mtd_geom* g);
 int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int a0, int n,
                     float2* crop_out, float2* part_out, const float* win, const FftTab& ftN,
-                    float2* t1);
+                    float2* t1, const float2* qtw = nullptr);
 // allocation-free entry points: the per-device scratch address, looked up
 // once per device when a context is created on it
 int noalloc_init(int device, std::string* err);
@@ -210,9 +210,12 @@ hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
                             int C, float2* t1, hipStream_t s);
 // part != nullptr: also writes per-block argmax partials of the crops
 // (holo_argmax_parts(N, C) per angle) for launch_argmax_fin
+// qtw != nullptr (the C-point twiddle table; holo_q_supported(N, C)): crop
+// receives Q, the crops' column inverse DFTs (see k_holo_cols)
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const FftTab& ft,
                             int C, int x0, int y0, float2* crop, float2* part,
-                            hipStream_t s);
+                            hipStream_t s, const float2* qtw = nullptr);
+bool holo_q_supported(int N, int C);
 // the ArrayFire variant's crop window (1+i) win[x] win[y] on A crops
 hipError_t launch_crop_window(int C, float2* crop, int A, const double* win, hipStream_t s);
 // argmax partials per angle the column pass writes (0: none, the field
@@ -239,6 +242,15 @@ hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, 
 // atan2 of IU (DEROUL 0)
 hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,
                       double* coef, hipStream_t s);
+// The field stage from Q, the crops' column inverse DFTs (compiled crop
+// sides): IU's least-squares sample rows into ius, the fit into coef, then
+// one row kernel (row inverse DFT with the specular's shift, field, window,
+// forward row FFT) into work and the forward column pass into out, the
+// spectra pre-weighted by beta2 (as launch_fwd_spectrum with from_iu = 1).
+hipError_t launch_fused_field(int C, const float2* q, const int* posx, const int* posy, int A,
+                              const LsqPlan& plan, double* coef, float2* ius, int mode,
+                              const double* win, const FftTab& ft, float dx2,
+                              const float* beta2, float2* work, float2* out, hipStream_t s);
 // mode: bit0 born, bit1 c_aber
 hipError_t launch_form_field(int C, const float2* iu, const float2* crop,
                              const int* posx, const int* posy, int A,

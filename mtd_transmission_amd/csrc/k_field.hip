@@ -120,6 +120,9 @@ struct ColCenterScaleOp {
 // ---------------------------------------------------------------------------
 constexpr int NCOEF = 15;
 
+// rows of the U frame that hold least-squares samples (y = 0, 5, 10, ...)
+__host__ __device__ inline int lsq_sample_rows(int C) { return (C + 4) / 5; }
+
 __device__ __forceinline__ void monomials(double X, double Y, double* mono) {
   double xp[5], yp[5];
   xp[0] = 1.0; yp[0] = 1.0;
@@ -155,9 +158,10 @@ template <int CT>
 __global__ void __launch_bounds__(256)
 k_lsq(const float2* __restrict__ iu, const float* __restrict__ phu,
       const int2* __restrict__ samples, int M, const double* __restrict__ lu,
-      const int* __restrict__ perm, double* __restrict__ coef, Dim<CT> C) {
+      const int* __restrict__ perm, double* __restrict__ coef, Dim<CT> C, int sample_rows) {
   const int a = blockIdx.x;
-  const float2* u = iu + (size_t)a * C * C;
+  const int Ci = C;
+  const float2* u = iu + (size_t)a * (sample_rows ? lsq_sample_rows(Ci) : Ci) * Ci;
   double rp[NCOEF], ra[NCOEF];
 #pragma unroll
   for (int k = 0; k < NCOEF; ++k) { rp[k] = 0.0; ra[k] = 0.0; }
@@ -165,7 +169,10 @@ k_lsq(const float2* __restrict__ iu, const float* __restrict__ phu,
 #pragma unroll 4
   for (int m = threadIdx.x; m < M; m += 256) {
     const int2 s = samples[m];
-    const float2 v = u[((s.y + C / 2) % C) * C + (s.x + C / 2) % C];
+    // sample_rows: iu holds only the IU rows of the sample rows (U-frame y a
+    // multiple of 5), row y / 5
+    const int row = sample_rows ? s.y / 5 : (s.y + C / 2) % C;
+    const float2 v = u[row * C + (s.x + C / 2) % C];
     // phase: wrapped (atan2) or, with unwrapping, the unwrapped U-frame phase
     const double ph = phu ? (double)phu[((size_t)a * C + s.y) * C + s.x] : (double)atan2f(v.y, v.x);
     const double am = (double)hypotf(v.x, v.y);
@@ -354,25 +361,28 @@ struct FwdRowOp {
       pa = row_poly(cph + NCOEF, (double)Y);
     }
   }
-  __device__ float2 load(int a, int y, int x) const {
+  // windowed field of the IU value u at column x of the line set up by line()
+  __device__ float2 form_iu(int x, float2 u) const {
     const int X = (x + C / 2) % C;
     const float w = (float)(win[X] * wy);
     float2 f;
-    if (from_iu) {
-      const float2 u = src[((size_t)a * C + y) * C + x];
-      if (mode & 2) {
-        if (!phu_a && (mode & 1)) {
-          f = form_born_wrapped(u, row_poly_at(pp, (double)X), row_poly_at(pa, (double)X));
-        } else {
-          const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
-          f = form_from(u, phf, row_poly_at(pp, (double)X), row_poly_at(pa, (double)X), sn, mode);
-        }
+    if (mode & 2) {
+      if (!phu_a && (mode & 1)) {
+        f = form_born_wrapped(u, row_poly_at(pp, (double)X), row_poly_at(pa, (double)X));
       } else {
-        f = form_from(u, 0.f, 0.0, 0.0, sn, mode);
+        const float phf = phu_a ? phu_a[Y * C + X] : atan2f(u.y, u.x);
+        f = form_from(u, phf, row_poly_at(pp, (double)X), row_poly_at(pa, (double)X), sn, mode);
       }
     } else {
-      f = src[((size_t)a * C + Y) * C + X];
+      f = form_from(u, 0.f, 0.0, 0.0, sn, mode);
     }
+    return make_float2(f.x * w, f.y * w);
+  }
+  __device__ float2 load(int a, int y, int x) const {
+    if (from_iu) return form_iu(x, src[((size_t)a * C + y) * C + x]);
+    const int X = (x + C / 2) % C;
+    const float w = (float)(win[X] * wy);
+    const float2 f = src[((size_t)a * C + Y) * C + X];
     return make_float2(f.x * w, f.y * w);
   }
   __device__ void store(int a, int y, int x, float2 v) const {
@@ -416,6 +426,77 @@ struct FwdColOp {
     out[((size_t)a * C + (row + py) % C) * C + (col + px) % C] = s;
   }
   Dim<CT> C{};
+};
+
+// ---------------------------------------------------------------------------
+// Fused field stage from Q, the crops' column inverse DFTs (k_holo_cols's
+// QOUT form). The demodulation's -posSpec shift (calc_Uborn2) is a shift by
+// px along each row of Q on load and, by the shift theorem, the phase ramp
+// exp(-2 pi i py k / C) on row k; the row inverse DFT then gives row k of IU.
+// QRowOp goes on through the field formation, the alpha = 0.1 window and
+// the forward row FFT in the same kernel (k_rows2); SampleRowOp only writes
+// the IU rows that hold least-squares samples (U-frame y = 5 j), for the fit
+// that the field formation needs first.
+// ---------------------------------------------------------------------------
+template <int CT>
+struct QLoad {
+  const float2* q;
+  const int* posx;
+  const int* posy;
+  const float2* tw;
+  int px;
+  float2 ramp;
+  __device__ void begin(int a, int C) {
+    (void)C;
+    px = posx[a];
+  }
+  __device__ void set_row(int a, int k, int C) {
+    const float2 w = tw[(posy[a] * k) % C];  // exp(-2 pi i py k / C)
+    ramp = w;
+  }
+  __device__ float2 at(int a, int k, int u, int C) const {
+    return cmul(q[((size_t)a * C + k) * C + (u + px) % C], ramp);
+  }
+};
+
+template <int CT>
+struct QRowOp {
+  QLoad<CT> ql;
+  FwdRowOp<CT> f;  // field formation, window and output (from_iu = 1)
+  const float2* tw;
+  __device__ void begin(int a) {
+    ql.begin(a, f.C);
+    f.begin(a);
+  }
+  __device__ void line(int a, int k) {
+    ql.set_row(a, k, f.C);
+    f.line(a, k);
+  }
+  __device__ float2 load(int a, int k, int u) const { return ql.at(a, k, u, f.C); }
+  __device__ float2 mid(int, int, int l, float2 iu) const { return f.form_iu(l, iu); }
+  __device__ void store(int a, int k, int j, float2 v) const { f.store(a, k, j, v); }
+};
+
+template <int CT>
+struct SampleRowOp {
+  QLoad<CT> ql;
+  float2* ius;  // [A][lsq_sample_rows(C)][C]
+  const float2* tw;
+  int ns;
+  Dim<CT> C{};
+  int j;
+  __device__ void begin(int a) { ql.begin(a, C); }
+  __device__ void line(int a, int jj) {
+    j = jj;
+    ql.set_row(a, (5 * (jj < ns ? jj : 0) + C / 2) % C, C);
+  }
+  __device__ float2 load(int a, int, int u) const {
+    if (j >= ns) return make_float2(0.f, 0.f);
+    return ql.at(a, (5 * j + C / 2) % C, u, C);
+  }
+  __device__ void store(int a, int, int l, float2 v) const {
+    if (j < ns) ius[((size_t)a * ns + j) * C + l] = v;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -483,10 +564,10 @@ static hipError_t demod_c(Dim<CT> C, const float2* crop, const int* posx, const 
 
 template <int CT>
 static hipError_t lsq_c(Dim<CT> C, const float2* iu, const float* phu, int A,
-                        const LsqPlan& plan, double* coef, hipStream_t s) {
+                        const LsqPlan& plan, double* coef, hipStream_t s, int sample_rows = 0) {
   Timed tm("lsq", s);
   hipLaunchKernelGGL((k_lsq<CT>), dim3(A), dim3(256), 0, s, iu, phu, plan.d_samples,
-                     plan.M, plan.d_lu, plan.d_perm, coef, C);
+                     plan.M, plan.d_lu, plan.d_perm, coef, C, sample_rows);
   return hipGetLastError();
 }
 
@@ -609,6 +690,53 @@ hipError_t launch_fwd_spectrum(int C, const float2* src, int from_iu,
   }
   return fwd_c<0>(Dim<0>(C), src, from_iu, crop, coef, mode, phu, posx, posy, A, win, ft, dx2,
                   shift_pos, beta2, work, out, s);
+}
+
+// the fused field stage from Q (compiled crop sides): the sample rows, the
+// fit, then one row kernel to the forward row FFT and the column pass
+template <int C>
+static hipError_t fused_c(const float2* q, const int* posx, const int* posy, int A,
+                          const LsqPlan& plan, double* coef, float2* ius, int mode,
+                          const double* win, const float2* tw, float dx2, const float* beta2,
+                          float2* work, float2* out, hipStream_t s) {
+  const int ns = lsq_sample_rows(C);
+  SampleRowOp<C> sr{QLoad<C>{q, posx, posy, tw, 0, make_float2(0.f, 0.f)}, ius, tw, ns};
+  {
+    Timed tm("sample_rows", s);
+    constexpr int NT = rows_nt<C>();
+    hipLaunchKernelGGL((k_rows<C, true, SampleRowOp<C>>), dim3((ns + NT - 1) / NT, A),
+                       rows_block<C>(), 0, s, sr);
+  }
+  hipError_t e = lsq_c<C>(Dim<C>{}, ius, nullptr, A, plan, coef, s, 1);
+  if (e != hipSuccess) return e;
+  FwdRowOp<C> f{nullptr, nullptr, coef, posx, posy, win, work, tw, 1, mode, nullptr,
+                make_float2(0.f, 0.f), nullptr, nullptr};
+  QRowOp<C> r{QLoad<C>{q, posx, posy, tw, 0, make_float2(0.f, 0.f)}, f, tw};
+  {
+    Timed tm("fwd_rows_field", s);
+    hipLaunchKernelGGL((k_rows2<C, true, false, QRowOp<C>>), rows_grid<C>(A), rows_block<C>(), 0,
+                       s, r);
+  }
+  FwdColOp<C> c{work, out, posx, posy, tw, dx2, 0, beta2, 0, 0, 0, 0};
+  Timed tm("fwd_cols", s);
+  hipLaunchKernelGGL((k_cols<C, false, FwdColOp<C>>), (cols_grid<C, FwdColOp<C>>(A)),
+                     (cols_block<C, FwdColOp<C>>()), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_field(int C, const float2* q, const int* posx, const int* posy, int A,
+                              const LsqPlan& plan, double* coef, float2* ius, int mode,
+                              const double* win, const FftTab& ft, float dx2,
+                              const float* beta2, float2* work, float2* out, hipStream_t s) {
+  switch (C) {
+#define X(n)                                                                                \
+  case n:                                                                                   \
+    return fused_c<n>(q, posx, posy, A, plan, coef, ius, mode, win, ft.tw, dx2, beta2, work, \
+                      out, s);
+    MTD_CROP_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
 }
 
 bool crop_len_supported(int C) {

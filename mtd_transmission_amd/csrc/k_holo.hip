@@ -152,11 +152,51 @@ constexpr int holo_cols_waves() {
   return N >= 1024 ? 2 : col_waves_per_simd<N>();
 }
 
-template <int N>
+// Crop sides whose column inverse DFT the column pass can produce (QOUT):
+// compiled register transforms, for hologram sides of at least twice the
+// crop (the kept column is staged in the exchange storage as complex values)
+#define MTD_QCOL_SIZES(X) X(128) X(192) X(250) X(256) X(384) X(512)
+
+template <int N, int CQ>
+constexpr bool qcol_fits() {
+  return N >= 512 && 2 * CQ <= N && Fft2<CQ, true>::T <= Fft2<N, false>::T;
+}
+
+// Column c's inverse DFT from the staging st ([CQ][NC], column fastest)
+// through the exchange storage x (which aliases st: read first) to Q's
+// rows; threads n2 < Fft2<CQ>::T of the column work, every thread passes the
+// barriers.
+template <int CQ, int NC>
+__device__ __forceinline__ void q_column(const float2* st, float* x, int c, int n2,
+                                         const float2* __restrict__ tq, float2* __restrict__ q,
+                                         bool keep) {
+  using G = Fft2<CQ, true>;
+  const bool act = n2 < G::T;
+  float2 v[G::L1], u[G::L2];
+  if (act) {
+#pragma unroll
+    for (int n1 = 0; n1 < G::L1; ++n1) v[n1] = st[(G::L2 * n1 + n2) * NC + c];
+  }
+  __syncthreads();
+  G::run_masked(v, u, n2, typename G::template Col<NC>{x + c}, tq, act);
+  if (act && keep && n2 < G::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < G::L2; ++k2) q[(size_t)(n2 + G::L1 * k2) * CQ] = u[k2];
+  }
+}
+
+// QOUT: instead of the crop, each column's unnormalised C-point inverse DFT
+// along y, Q[a][k][x] = sum_e crop[a][e][x] exp(+2 pi i e k / C) (the column
+// half of the demodulation's 2-D inverse transform, calc_Uborn2,
+// fonctions.cpp:348-374; its -posSpec shift becomes a phase ramp on Q's
+// rows by the shift theorem, applied by the row pass once the specular is
+// known). tq: the C-point twiddle table; C one of MTD_QCOL_SIZES with
+// qcol_fits<N, C>.
+template <int N, bool QOUT = false>
 __global__ void __launch_bounds__(holo_cols_threads<N>(), holo_cols_waves<N>())
 k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
             int C, int cp, int W, int x0, int y0, float scale, float2* __restrict__ crop,
-            float2* __restrict__ part) {
+            float2* __restrict__ part, const float2* __restrict__ tq) {
   using F = Fft2<N, false>;
   constexpr int NC = holo_cols_nc<N>();
   __shared__ float lds[NC * F::LDS_COL];
@@ -179,6 +219,10 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
   // of the block width) transformed padding: nothing of theirs is kept
   float best = -1.f;
   int bi = 0x7fffffff;
+  // QOUT: the kept column is staged as complex values in the exchange
+  // storage, free once every thread is past the transform
+  float2* st = (float2*)lds;
+  if constexpr (QOUT) __syncthreads();
   if (n2 < F::L1 && d0 + c < C) {
     float2* dst = crop + (size_t)a * C * C + d0 + c;
     const int kx = x0 + d0 + c;
@@ -189,7 +233,11 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
       if (e >= 0 && e < C) {
         const float s = ((kx + ky) & 1) ? -scale : scale;
         const float2 val = make_float2(u[k2].x * s, u[k2].y * s);
-        dst[(size_t)e * C] = val;
+        if constexpr (QOUT) {
+          st[e * NC + c] = val;
+        } else {
+          dst[(size_t)e * C] = val;
+        }
         const int i = e * C + d0 + c;
         const float m = argmax_key(val, i == 0);
         if (argmax_better(m, i, best, bi)) { best = m; bi = i; }
@@ -201,6 +249,19 @@ k_holo_cols(const float2* __restrict__ t1, const float2* __restrict__ tw,
     argmax_block(best, bi, red);
     if (threadIdx.x == 0)
       part[(size_t)a * ((C + NC - 1) / NC) + ab.y] = make_float2(best, __int_as_float(bi));
+  }
+  if constexpr (QOUT) {
+    __syncthreads();
+    float2* q = crop + (size_t)a * C * C + d0 + c;
+    const bool keep = d0 + c < C;
+    switch (C) {  // uniform
+#define X(n)                                                         \
+  case n:                                                            \
+    if constexpr (qcol_fits<N, n>()) q_column<n, NC>(st, lds, c, n2, tq, q, keep); \
+    break;
+      MTD_QCOL_SIZES(X)
+#undef X
+    }
   }
 }
 
@@ -241,14 +302,31 @@ static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
 }
 
 template <int N>
+static bool qcol_n(int C) {
+  switch (C) {
+#define X(n) case n: return qcol_fits<N, n>();
+    MTD_QCOL_SIZES(X)
+#undef X
+  }
+  return false;
+}
+
+template <int N>
 static hipError_t cols_n(const float2* t1, int A, const float2* tw, int C,
-                         int x0, int y0, float2* crop, float2* part, hipStream_t s) {
+                         int x0, int y0, float2* crop, float2* part, const float2* tq,
+                         hipStream_t s) {
   constexpr int NC = holo_cols_nc<N>();
   const float scale = (float)(0.5 / ((double)N * (double)N));
+  if (tq && !qcol_n<N>(C)) return hipErrorInvalidValue;
   Timed tm("holo_cols", s);
-  hipLaunchKernelGGL((k_holo_cols<N>), dim3((C + NC - 1) / NC, A), dim3(holo_cols_threads<N>()),
-                     0, s, t1, tw, C, t1_cols_n<N>(C), holo_tile_w(C, NC), x0, y0, scale, crop,
-                     part);
+  const dim3 grid((C + NC - 1) / NC, A), block(holo_cols_threads<N>());
+  if (tq) {
+    hipLaunchKernelGGL((k_holo_cols<N, true>), grid, block, 0, s, t1, tw, C, t1_cols_n<N>(C),
+                       holo_tile_w(C, NC), x0, y0, scale, crop, part, tq);
+  } else {
+    hipLaunchKernelGGL((k_holo_cols<N, false>), grid, block, 0, s, t1, tw, C, t1_cols_n<N>(C),
+                       holo_tile_w(C, NC), x0, y0, scale, crop, part, nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -342,12 +420,13 @@ hipError_t launch_holo_rows(int N, int dtype, const void* holos, int A,
 
 hipError_t launch_holo_cols(int N, const float2* t1, int A, const FftTab& ft,
                             int C, int x0, int y0, float2* crop, float2* part,
-                            hipStream_t s) {
+                            hipStream_t s, const float2* qtw) {
   switch (N) {
-#define X(n) case n: return cols_n<n>(t1, A, ft.tw, C, x0, y0, crop, part, s);
+#define X(n) case n: return cols_n<n>(t1, A, ft.tw, C, x0, y0, crop, part, qtw, s);
     MTD_HOLO_SIZES(X)
 #undef X
   }
+  if (qtw) return hipErrorInvalidValue;  // the fused column IDFT is for compiled sides
   if (!ft.gen) return hipErrorInvalidValue;
   Timed tm("holo_cols", s);
   const float scale = (float)(1.0 / ((double)N * (double)N));
@@ -370,6 +449,15 @@ int holo_t1_cols(int N, int C) {
 #undef X
   }
   return C;
+}
+
+bool holo_q_supported(int N, int C) {
+  switch (N) {
+#define X(n) case n: return qcol_n<n>(C);
+    MTD_HOLO_SIZES(X)
+#undef X
+  }
+  return false;
 }
 
 bool holo_len_supported(int N) {

@@ -541,14 +541,14 @@ size_t holo_t1_bytes(const mtd_geom* g) {
 
 int holo_chunk_with(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int a0, int n,
                     float2* crop_out, float2* part_out, const float* win, const FftTab& ftN,
-                    float2* t1) {
+                    float2* t1, const float2* qtw) {
   const int N = g->n, C = 2 * g->nxmax;
   const char* src = (const char*)holos + (size_t)a0 * N * N * dtype_bytes(dtype);
   MTD_TRY(hip_check(ctx, launch_holo_rows(N, dtype, src, n, win, ftN, g->circle_cx - g->nxmax,
                                           C, t1, ctx->stream), "hologram row pass"));
   MTD_TRY(hip_check(ctx, launch_holo_cols(N, t1, n, ftN, C, g->circle_cx - g->nxmax,
                                           g->circle_cy - g->nxmax, crop_out, part_out,
-                                          ctx->stream),
+                                          ctx->stream, qtw),
                     "hologram column pass"));
   if (g->alpha_crop > 0) {
     const double* wc = tukey_table(ctx, C, g->alpha_crop);
@@ -566,13 +566,14 @@ static int fused_argmax_parts(const mtd_geom* g) {
 }
 
 static int holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
-                      int dtype, int a0, int n, float2* crop_out, float2* part_out = nullptr) {
+                      int dtype, int a0, int n, float2* crop_out, float2* part_out = nullptr,
+                      const float2* qtw = nullptr) {
   const float* win = tukey_signed_table(ctx, g->n, g->alpha_holo);
   FftTab ftN;
   const bool tabs = fft_tab(ctx, g->n, holo_len_supported(g->n), &ftN);
   float2* t1 = (float2*)ws_get(ctx, WS_T1, (size_t)n * holo_t1_bytes(g));
   if (!win || !tabs || !t1) return set_err(ctx, MTD_ERR_HIP, "out of device memory (hologram stage)");
-  return holo_chunk_with(ctx, g, holos, dtype, a0, n, crop_out, part_out, win, ftN, t1);
+  return holo_chunk_with(ctx, g, holos, dtype, a0, n, crop_out, part_out, win, ftN, t1, qtw);
 }
 
 static int field_mode(const mtd_geom* g) { return (g->born ? 1 : 0) | (g->c_aber ? 2 : 0); }
@@ -715,6 +716,43 @@ static int front_chunk(mtd_ctx* ctx, const mtd_geom* g, const float2* crop, int 
   return MTD_OK;
 }
 
+// The fused field stage of the plain method: the hologram column pass
+// writes Q, the crops' column inverse DFTs, instead of the crops, and the
+// field stage runs from Q (launch_fused_field): the wrapped-phase fit
+// (C_ABER 1, DEROUL 0), no crop window, compiled crop side whose column
+// transform the hologram column kernel holds (holo_q_supported). Returns
+// the C-point twiddle table for launch_holo_cols, nullptr where it does not
+// apply.
+static const float2* fused_qtw(mtd_ctx* ctx, const mtd_geom* g) {
+  const int C = 2 * g->nxmax;
+  if (!g->c_aber || g->deroul != MTD_DEROUL_OFF || g->alpha_crop > 0) return nullptr;
+  if (!crop_len_supported(C) || !holo_q_supported(g->n, C)) return nullptr;
+  return twiddle(ctx, C);
+}
+
+// argmax from the column pass's partials, then the fused field stage from
+// Q to the Ewald-weighted spectra
+static int front_fused(mtd_ctx* ctx, const mtd_geom* g, const float2* q, int n, int* posx,
+                       int* posy, float2* out_F, const float2* part, int S) {
+  const int C = 2 * g->nxmax;
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, true, &twC);
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)n * C * C * sizeof(float2));
+  float2* ius = (float2*)ws_get(ctx, WS_IU, (size_t)n * C * C * sizeof(float2));
+  double* coef = (double*)ws_get(ctx, WS_COEF, (size_t)n * 30 * sizeof(double));
+  const double* winC = tukey_table(ctx, C, g->alpha_field);
+  if (!tabs || !work || !ius || !coef || !winC)
+    return set_err(ctx, MTD_ERR_HIP, "out of device memory (field stage)");
+  MTD_TRY(hip_check(ctx, launch_argmax_fin(C, part, S, n, posx, posy, ctx->stream), "argmax"));
+  EwaldTables t;
+  MTD_TRY(ewald_tables(ctx, g, &t));
+  const float dx2 = (float)(g->pix_uborn * g->pix_uborn);
+  return hip_check(ctx, launch_fused_field(C, q, posx, posy, n, *ctx->lsq, coef, ius,
+                                           field_mode(g), winC, twC, dx2, t.beta2, work, out_F,
+                                           ctx->stream),
+                   "field stage");
+}
+
 // Hologram stage: the N x C row-pass intermediate of a chunk stays within
 // the 256 MiB Infinity Cache. Field stage: about 1 GiB of per-angle buffers
 // per launch, so that its small kernels get hundreds of angles at once.
@@ -767,7 +805,7 @@ static int copy_engine(mtd_ctx* ctx) {
 // i is transformed; the copy into a slot waits for the row pass that last
 // read it, the row pass waits for its copy.
 static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
-                      int n, float2* crop, float2* part, int S) {
+                      int n, float2* crop, float2* part, int S, const float2* qtw = nullptr) {
   const int N = g->n, C = 2 * g->nxmax;
   const int hc = ctx->holo_chunk_override > 0
                      ? std::min(ctx->holo_chunk_override, std::max(n, 1))
@@ -776,7 +814,7 @@ static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dt
   if (!is_host_ptr(holos)) {
     for (int h0 = 0; h0 < n; h0 += hc)
       MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
-                         crop + (size_t)h0 * C * C, part_at(h0)));
+                         crop + (size_t)h0 * C * C, part_at(h0), qtw));
     return MTD_OK;
   }
   MTD_TRY(copy_engine(ctx));
@@ -813,7 +851,7 @@ static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dt
     const int k = i % 2, h0 = i * hc, m = std::min(hc, n - h0);
     MTD_TRY(hip_check(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_ready[k], 0), "compute wait"));
     MTD_TRY(holo_chunk(ctx, g, slots + (size_t)k * hc * hb, dtype, 0, m,
-                       crop + (size_t)h0 * C * C, part_at(h0)));
+                       crop + (size_t)h0 * C * C, part_at(h0), qtw));
     MTD_TRY(hip_check(ctx, hipEventRecord(ctx->ev_free[k], ctx->stream), "slot event"));
     ctx->slot_used[k] = true;
   }
@@ -1573,11 +1611,16 @@ int mtd_reconstruct_plain(mtd_ctx* ctx, const mtd_geom* g, const void* holos, in
   int2* band = (int2*)ws_get(ctx, WS_BAND, sizeof(int2));
   if (!F || !crop || !part || !vol || !band)
     return set_err(ctx, MTD_ERR_HIP, "out of device memory");
+  const float2* qp = S > 0 ? fused_qtw(ctx, g) : nullptr;
   for (int a0 = 0; a0 < A; a0 += chunk) {
     const int n = std::min(chunk, A - a0);
-    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, S > 0 ? part : nullptr, S));
-    MTD_TRY(front_chunk(ctx, g, crop, n, a0, pos + a0, pos + A + a0, nullptr,
-                        F + (size_t)a0 * C * C, part, S));
+    MTD_TRY(holo_range(ctx, g, holos, dtype, a0, n, crop, S > 0 ? part : nullptr, S, qp));
+    if (qp)
+      MTD_TRY(front_fused(ctx, g, crop, n, pos + a0, pos + A + a0, F + (size_t)a0 * C * C, part,
+                          S));
+    else
+      MTD_TRY(front_chunk(ctx, g, crop, n, a0, pos + a0, pos + A + a0, nullptr,
+                          F + (size_t)a0 * C * C, part, S));
   }
   // The gather writes the count-normalised, centring-signed spectrum, and
   // only in the height band that holds hits; the inverse transform treats the

@@ -170,6 +170,40 @@ __global__ void __launch_bounds__((cols_threads<L, op_col_threads<Op>::value>())
   }
 }
 
+// Two chained transforms per row: the INV1 transform of the loaded line,
+// op.mid(batch, line, element, value) on every output element, the values
+// moved back to the input layout through the exchange storage, the INV2
+// transform, op.store. The line set up by op.line(batch, line) is kept for
+// the whole kernel (one line per NT-th of the block).
+template <int L, bool INV1, bool INV2, class Op>
+__global__ void __launch_bounds__(256) k_rows2(Op op) {
+  using F1 = Fft2<L, INV1>;
+  using F2 = Fft2<L, INV2>;
+  constexpr int NT = rows_nt<L>();
+  __shared__ float lds[NT * F1::LDS_ROW];
+  const int2 ab = xcd_block();
+  const int a = ab.x;
+  const int r = threadIdx.x / F1::T, n2 = threadIdx.x % F1::T;
+  const int line = ab.y * NT + r;
+  op.begin(a);
+  op.line(a, line);
+  float2 v[F1::L1], u[F1::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F1::L1; ++n1) v[n1] = op.load(a, line, F1::L2 * n1 + n2);
+  const typename F1::Row x{lds + r * F1::LDS_ROW};
+  F1::run(v, u, n2, x, op.tw);
+  if (n2 < F1::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < F1::L2; ++k2) u[k2] = op.mid(a, line, n2 + F1::L1 * k2, u[k2]);
+  }
+  fft2_relayout<L>(u, v, n2, x);
+  F2::run(v, u, n2, x, op.tw);
+  if (n2 < F2::L1) {
+#pragma unroll
+    for (int k2 = 0; k2 < F2::L2; ++k2) op.store(a, line, n2 + F2::L1 * k2, u[k2]);
+  }
+}
+
 template <int L>
 inline dim3 rows_grid(int A) {
   return dim3(L / rows_nt<L>(), A);

@@ -8,9 +8,10 @@ bit-exact, fields within rel-L2 1e-5. The Ewald accumulation of all 600
 angles' spectra into the 512^3 spectrum and counts is bit-exact against the
 oracle's, the hit total equals the closed form sum of P over non-skipped
 angles, the finalisation of that spectrum is within rel-L2 1e-5 of the
-oracle's, and the fused plain method (normalisation in the gather, spectra
-pre-weighted in the forward pass) equals the stage-by-stage chain bit for
-bit."""
+oracle's, and the fused plain method (the demodulation's column transform in
+the hologram column pass, one row kernel from the row inverse DFT to the
+forward row FFT, normalisation in the gather, spectra pre-weighted in the
+forward pass) is within rel-L2 1e-5 of the oracle's finalised index too."""
 import numpy as np
 import pytest
 
@@ -105,14 +106,19 @@ def test_full_scatter_finalize_and_fused(full, gpu_ctx):
     e = rel_l2(got, idx32)
     _say(f"  index rel-L2 vs f32 {e:.3e}")
     assert e < TOL
-    del got, idx32
-    # the fused plain method (one call from the holograms) against the chain
+    del got
+    # the fused plain method (one call from the holograms) against the same
+    # oracle index and against the stage chain
     re2 = torch.empty_like(re)
     im2 = torch.empty_like(re)
     pos2 = torch.empty_like(full["pos"])
     gpu_ctx.reconstruct_plain(gl, full["holos"], full["mask"], re2, im2, pos2)
     torch.cuda.synchronize()
     assert torch.equal(pos2, full["pos"])
-    same = torch.equal(re2, re) and torch.equal(im2, im)
-    _say(f"  fused plain method bit-identical to the stage chain: {same}")
-    assert same
+    got2 = re2.cpu().numpy() + 1j * im2.cpu().numpy()
+    e2 = rel_l2(got2, idx32)
+    del idx32
+    chain = re.cpu().numpy() + 1j * im.cpu().numpy()
+    ec = rel_l2(got2, chain)
+    _say(f"  fused plain method: index rel-L2 vs f32 {e2:.3e}, vs the stage chain {ec:.3e}")
+    assert e2 < TOL and ec < TOL
