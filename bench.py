@@ -205,6 +205,31 @@ def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
             sweep[f"{lam:g}"] = {"iterations": 100, "index_rmse": round(rmse(r.dn), 6),
                                  "rmse_reduction": round(1 - rmse(r.dn) / r_plain, 4),
                                  "ms_solver": round((time.perf_counter() - t0) * 1e3, 2)}
+    # The configuration's objective ||A(n) - y||^2 + lam_cfg TV(n) in the
+    # solver's normalisation: 1/2 ||A dn - y||_W^2 / kappa + lam TV(dn) equals
+    # (||A dn - y||_W^2 + 2 kappa lam TV(dn)) / (2 kappa), so lam_cfg = 2 kappa lam
+    # (W = 1, kappa = ||A^H A||, for the unweighted objective the
+    # configuration writes; kappa = Tp^6 V^3 with the count weights). The
+    # library's spectra are in SI units, so lam_cfg applied literally is
+    # lam = lam_cfg / (2 kappa): the unweighted objective run at both values.
+    cfg_obj = {}
+    if wname == "diff":
+        kappa1 = adj.normal_norm(ctx, g, pos, None)
+        cfg_obj = {"kappa_weighted": kappa, "kappa_unweighted": kappa1,
+                   "lam_solver_of_headline_in_config_units_weighted": 2 * kappa * TV_LAM,
+                   "runs_unweighted_literal": {}}
+        for lam_cfg in (1e-4, 1e-3):
+            lam_s = lam_cfg / (2 * kappa1)
+            cur = {}
+            r = adj.reconstruct_tv(ctx, g, pos, y, n_iter=100, lam=min(lam_s, 3.0e38),
+                                   eps=TV_EPS, dn0=re, kappa=kappa1, weights=None,
+                                   monitor=lambda k, x: cur.__setitem__(k, rmse(x)))
+            cfg_obj["runs_unweighted_literal"][f"{lam_cfg:g}"] = {
+                "lam_solver": lam_s, "iterations": 100,
+                "index_rmse": round(rmse(r.dn), 6),
+                "rmse_reduction": round(1 - rmse(r.dn) / r_plain, 4),
+                "rmse_by_iteration": {str(k): round(cur[k], 6) for k in (10, 20, 50, 100)
+                                      if k in cur}}
     setup_ms, _ = timed(lambda: tv_setup(), 3)
     per_iter = max((ms_tv - setup_ms) / n_iter, 1e-6)
     k10 = int(max(0, min(curve_iters, (10 * ms_plain - setup_ms) // per_iter)))
@@ -231,6 +256,7 @@ def adjoint_quality(ctx, torch, np, m, synth, wname="diff", n_iter=TV_ITERS,
             "rmse_reduction_within_10x_plain": round(1 - r10 / r_plain, 4),
             "lam_tv": TV_LAM, "eps_tv": TV_EPS,
             "lam_sweep_100_iterations": sweep,
+            "config_objective": cfg_obj,
             "weights": "1/(q^2 hits)", "start": "plain"}
 
 

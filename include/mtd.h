@@ -110,6 +110,13 @@ int mtd_ctx_set_timing(mtd_ctx* ctx, int on);
  * gather_window = height of the Ewald gather's windows, 8, 16 or 24 voxels
  * (0: 16). The batched form's autotuner picks both per shape. */
 int mtd_ctx_set_schedule(mtd_ctx* ctx, int holo_chunk, int gather_window);
+/* Named schedule parameters of the plain path (no reference counterpart;
+ * results are the same for every value): "holo_chunk" (as above),
+ * "gather_window" (8, 16, 24), "gather_batch" (list entries whose spectrum
+ * loads the gather issues together: 8, 16). 0 restores the default. An
+ * unknown name is MTD_ERR_ARG, a value not built MTD_ERR_UNSUPPORTED. */
+int mtd_ctx_set_schedule_param(mtd_ctx* ctx, const char* name, int value);
+int mtd_ctx_get_schedule_param(mtd_ctx* ctx, const char* name, int* value);
 /* Restricts the timing to the kernel of that report name (NULL or "": all
  * kernels). Each timed launch adds a pair of event markers to the stream,
  * which delays the next dispatch by a few microseconds; timing one kernel

@@ -64,6 +64,8 @@ SIGNATURES = {
     "mtd_ctx_set_timing": (_I, [_P, _I]),
     "mtd_ctx_set_timing_only": (_I, [_P, ctypes.c_char_p]),
     "mtd_ctx_set_schedule": (_I, [_P, _I, _I]),
+    "mtd_ctx_set_schedule_param": (_I, [_P, ctypes.c_char_p, _I]),
+    "mtd_ctx_get_schedule_param": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_I)]),
     "mtd_ctx_timing_report": (_I, [_P, ctypes.c_char_p, ctypes.c_size_t]),
     "mtd_ctx_timing_reset": (_I, [_P]),
     "mtd_config_extract_val": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _D,
@@ -312,6 +314,18 @@ class Context:
         window height (0: defaults); results do not depend on it."""
         self._check(lib().mtd_ctx_set_schedule(self.h, int(holo_chunk), int(gather_window)),
                     "set_schedule")
+
+    def set_schedule_param(self, name, value):
+        """One named schedule parameter ("holo_chunk", "gather_window",
+        "gather_batch"; 0: its default); results do not depend on it."""
+        self._check(lib().mtd_ctx_set_schedule_param(self.h, name.encode(), int(value)),
+                    "set_schedule_param")
+
+    def schedule_param(self, name):
+        v = _I(0)
+        self._check(lib().mtd_ctx_get_schedule_param(self.h, name.encode(), ctypes.byref(v)),
+                    "schedule_param")
+        return v.value
 
     def timing_reset(self):
         lib().mtd_ctx_timing_reset(self.h)

@@ -110,6 +110,8 @@ struct mtd_ctx {
   std::map<std::tuple<int, int, int>, std::pair<int, int>> batch_tuned;
   int holo_chunk_override = 0;  // > 0: holograms per hologram-stage chunk
   int gather_zw_override = 0;   // > 0: gather height window (8, 16, 24)
+  int gather_kb_override = 0;   // > 0: gather entries per load batch (8, 16)
+  std::map<int, int*> tile_orders;  // the gather's tile dispatch order per volume side
   // geometry plan of the forward model / adjoint (mtd_born_plan): speculars,
   // angle geometry, hit band and the gather's tile lists, in WS_PLAN* slots
   bool plan_valid = false;
@@ -292,17 +294,20 @@ hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, i
 // band == nullptr: every voxel of the volume (and counts) is written;
 // otherwise only the height windows that meet the band are, and [zlo, zhi]
 // is a host-side bound of the band (only windows meeting it are launched).
-hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
-                         const int* posy, const AngleGeo* ageo, int A, int C,
-                         int nxmax, int V, const double* htab,
-                         const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, bool prepass, int zw, hipStream_t s);
-// gather height windows built (8, 16, 24; others fall back to 16)
+// order: the tile dispatch order (gather_tile_order); zw: height window
+// (8, 16, 24; others fall back to 16); kb: list entries per load batch (8,
+// 16). The crop side must keep one spectrum below 2 GiB.
+hipError_t launch_gather(const float2* spec, int frame_pos, const AngleGeo* ageo, int A, int C,
+                         int nxmax, int V, const double* htab, const float* beta, int r2max,
+                         float2* vol, int32_t* cnt, void* scratch, const int* order, int flags,
+                         const int2* band, int zlo, int zhi, bool prepass, int zw, int kb,
+                         hipStream_t s);
 bool gather_window_supported(int zw);
 // the gather's prepass alone (per-tile angle lists into scratch)
 hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
                                  const double* htab, int r2max, void* scratch, hipStream_t s);
+// the gather's tile dispatch order for a V^3 volume ((V/16)^2 tiles, rounded up)
+void gather_tile_order_host(int V, int* order);
 size_t gather_scratch_bytes(int V, int A);
 // Born forward model: work = centred 3-D spectrum of the contrast (re, im;
 // im may be nullptr) times scale (Tp^3 * gsign), stored only on the heights

@@ -71,30 +71,48 @@ constexpr int GT = 16;  // tile side (columns per block = 256)
 // Windowed gather. A prepass lists, per 16x16 column tile and in angle order,
 // the angles whose pupil disc touches the tile, with a conservative height
 // range derived from the tile box's nearest and farthest pupil radii (h is
-// decreasing in r^2 and round is monotonic), and each column's exact hit
-// height per listed angle as one byte. The main kernel gives each block one
-// (tile, ZW-height window); a wave tests 64 list entries against the window
-// at once, so each hit is visited in about 1 + span/ZW windows. Near the
-// lateral origin nearly every angle hits the same few voxels, and those
-// sums are serial by definition (angle order): the loads of kBatch hits are
-// issued together so that only the additions stay serial.
+// decreasing in r^2 and round is monotonic). The main kernel gives each block
+// one (tile, ZW-height window); a wave tests 64 list entries against the
+// window at once and takes the overlapping ones in angle order, KB at a
+// time. Each column's exact hit height is computed in registers, and only
+// the hits' spectrum values are loaded (a miss's buffer-load offset is out
+// of range, which fetches nothing). Near the lateral origin nearly every
+// angle hits the same few voxels, and those sums are serial by definition
+// (angle order): the loads of KB entries are issued together so that only
+// the additions stay serial, and those tiles are dispatched first (tiles in
+// order of distance from the lateral origin).
 // ---------------------------------------------------------------------------
 struct TileEntry {
   int a;
   int zpack;  // zlo | zhi << 16, absolute heights
   int fxy;    // fxm0 & 0xffff | fym0 << 16 (signed 16-bit each)
-  int pad;
+  float fz;   // fzm0 rounded to float (the registers' height path)
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-constexpr int kBatch = 8;  // hits whose loads are issued together
-constexpr int kPrepBatch = 8;  // prepass: entries whose heights are computed together
-constexpr int kTileChunk = 64;  // prepass: angles per block
-
 // h(r2) = sqrt(R^2 - r2) for 0 <= r2 <= R^2: IEEE sqrt of the same exact
 // double, so the host height table's value bit for bit
 __device__ __forceinline__ double height(double R2, int r2) { return sqrt(R2 - (double)r2); }
+
+// Height offset round(h(r2) - fz) of a hit (before + V/2), as the reference
+// computes it: FP64, round half away from zero. First in FP32: with R^2 <
+// 2^24, R^2 - r2 is an exact float, sqrtf is correctly rounded, fz32 is fz
+// rounded once and the difference is rounded once, each within half an ulp
+// of R (h, fz and |h - fz| are at most R): the FP32 difference is within
+// 1.5 ulp(R) of the FP64 one. Wherever the fractional part of |difference|
+// is farther than thr = 4 ulp(R) from 1/2 both round alike; near 1/2
+// (*near is set) the caller takes the FP64 path.
+__device__ __forceinline__ int height_off_f32(float R2f, int r2, float fz32, float thr,
+                                              bool* near) {
+  const float d = sqrtf(R2f - (float)r2) - fz32;
+  const float ad = fabsf(d);
+  const float fl = floorf(ad);
+  const float fr = ad - fl;  // exact (Sterbenz)
+  *near = fabsf(fr - 0.5f) < thr;
+  const int zr = (int)fl + (fr >= 0.5f ? 1 : 0);
+  return d < 0.f ? -zr : zr;
+}
 
 // Does angle a's pupil disc touch the 16x16 column tile at (qx0, qy0)? If so,
 // the nearest and farthest pupil radii^2 of the tile box (the farthest
@@ -113,72 +131,21 @@ __device__ __forceinline__ bool tile_touch(const GatherArgs& g, const AngleGeo& 
   return dmin <= g.r2max;
 }
 
-__device__ __forceinline__ int block_sum256(int v, int* wsum) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  __syncthreads();  // wsum is reused
-  return t;
-}
-
-// Prepass: the angles whose pupil disc touches each 16x16 column tile, in
-// angle order, with a conservative height range from the tile box's nearest
-// and farthest pupil radii (h decreases with r^2, round is monotonic), and
-// every column's exact hit height per listed angle as a byte offset from the
-// entry's lower bound (255 = no hit): FP64 heights and round-half-away, as
-// the reference. One block per (tile, chunk of kTileChunk angles), so that
-// the tiles near the lateral origin, which nearly every angle touches, are
-// spread over many blocks: a block first counts the tile's entries of the
-// earlier chunks (its offset in the tile's list; the chunk-0 block counts all
-// of them and writes the tile's total and height range), then lists its own
-// chunk and computes the heights of those entries, one column per thread.
-__global__ void __launch_bounds__(256)
-k_tile_angles(GatherArgs g, int ntx, int nchunk, TileEntry* __restrict__ list,
-              int4* __restrict__ tinfo, uint8_t* __restrict__ zoff) {
-  __shared__ int wsum[4];
-  __shared__ int zr[2][4];
-  __shared__ int nown;
-  __shared__ int4 sent[kTileChunk];  // a, zl, fx, fy of this chunk's entries
-  __shared__ double sfz[kTileChunk];
-  const int tile = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
-  const int tid = threadIdx.x, lane = tid & 63;
+// Prepass: one wave per 16x16 column tile lists, in angle order, the angles
+// whose pupil disc touches the tile, 64 angles per ballot, with a
+// conservative height range from the tile box's nearest and farthest pupil
+// radii (h decreases with r^2, round is monotonic; FP64 heights and
+// round-half-away, as the reference), and writes the tile's entry count and
+// height range.
+__global__ void __launch_bounds__(64)
+k_tile_angles(GatherArgs g, int ntx, TileEntry* __restrict__ list, int4* __restrict__ tinfo) {
+  const int tile = blockIdx.x, lane = threadIdx.x;
   const int qx0 = (tile % ntx) * GT - g.V / 2, qy0 = (tile / ntx) * GT - g.V / 2;
-  const int qx = qx0 + tid % GT, qy = qy0 + tid / GT;  // this thread's column
   const double R2 = g.htab[0] * g.htab[0];  // h(0) = R, an integer: exact
-  const int a_own = chunk * kTileChunk;
-  // entries of the earlier chunks; chunk 0: all entries and the height range
-  const int lim = chunk == 0 ? g.A : a_own;
-  int cnt = 0, zmin = 0x7fffffff, zmax = -1;
-  for (int a = tid; a < lim; a += 256) {
-    const AngleGeo ag = g.ageo[a];
-    int dmin, dmax;
-    if (tile_touch(g, ag, qx0, qy0, dmin, dmax)) {
-      ++cnt;
-      if (chunk == 0) {
-        zmin = min(zmin, clampi((int)round_away(height(R2, dmax) - ag.fz) + g.V / 2, 0, g.V - 1));
-        zmax = max(zmax, clampi((int)round_away(height(R2, dmin) - ag.fz) + g.V / 2, 0, g.V - 1));
-      }
-    }
-  }
-  const int total = block_sum256(cnt, wsum);
-  if (chunk == 0) {
-    for (int o = 32; o > 0; o >>= 1) {
-      zmin = min(zmin, __shfl_xor(zmin, o, 64));
-      zmax = max(zmax, __shfl_xor(zmax, o, 64));
-    }
-    if (lane == 0) { zr[0][tid >> 6] = zmin; zr[1][tid >> 6] = zmax; }
-    __syncthreads();
-    if (tid == 0) {
-      const int lo = min(min(zr[0][0], zr[0][1]), min(zr[0][2], zr[0][3]));
-      const int hi = max(max(zr[1][0], zr[1][1]), max(zr[1][2], zr[1][3]));
-      tinfo[tile] = make_int4(total, lo, hi, 0);
-    }
-  }
-  const int off = chunk == 0 ? 0 : total;
-  // this chunk's entries, in angle order (one wave)
-  if (tid < 64) {
-    const int a = a_own + lane;
+  TileEntry* lst = list + (size_t)tile * g.A;
+  int off = 0, zmin = 0x7fffffff, zmax = -1;
+  for (int a0 = 0; a0 < g.A; a0 += 64) {
+    const int a = a0 + lane;
     bool act = false;
     int zl = 0, zh = 0;
     AngleGeo ag{};
@@ -189,43 +156,114 @@ k_tile_angles(GatherArgs g, int ntx, int nchunk, TileEntry* __restrict__ list,
         zl = clampi((int)round_away(height(R2, dmax) - ag.fz) + g.V / 2, 0, g.V - 1);
         zh = clampi((int)round_away(height(R2, dmin) - ag.fz) + g.V / 2, 0, g.V - 1);
         act = true;
+        zmin = min(zmin, zl);
+        zmax = max(zmax, zh);
       }
     }
     const unsigned long long bal = __ballot(act);
     if (act) {
       const int p = __popcll(bal & ((1ull << lane) - 1ull));
-      list[(size_t)tile * g.A + off + p] =
-          TileEntry{a, zl | (zh << 16), (ag.fx & 0xffff) | (ag.fy << 16), 0};
-      sent[p] = make_int4(a, zl, ag.fx, ag.fy);
-      sfz[p] = ag.fz;
+      lst[off + p] = TileEntry{a, zl | (zh << 16), (ag.fx & 0xffff) | (ag.fy << 16), (float)ag.fz};
     }
-    if (lane == 0) nown = __popcll(bal);
+    off += __popcll(bal);
   }
-  __syncthreads();
-  const int n = nown;
-  // exact hit heights of this thread's column for the chunk's entries
-  uint8_t* zo = zoff + ((size_t)tile * g.A + off) * 256 + tid;
-  for (int i0 = 0; i0 < n; i0 += kPrepBatch) {
-    // branch-free batch (slots past n repeat the last entry and are not stored)
-    double hb[kPrepBatch];
-    bool inp[kPrepBatch];
+  for (int o = 32; o > 0; o >>= 1) {
+    zmin = min(zmin, __shfl_xor(zmin, o, 64));
+    zmax = max(zmax, __shfl_xor(zmax, o, 64));
+  }
+  if (lane == 0) tinfo[tile] = make_int4(off, zmin, zmax, 0);
+}
+
+// 8 bytes of a spectrum at byte offset off from base; an offset past nbytes
+// returns zeros without a memory access (a buffer load's range check)
+__device__ __forceinline__ float2 load_or_zero(const float2* base, uint32_t nbytes, uint32_t off) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)nbytes, 0x00020000);
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+constexpr uint32_t kMissOffset = 0x80000000u;  // past every spectrum (< 2^31 bytes)
+
+// one column's fixed data for the gather's hit tests
+struct GatherCol {
+  int qx, qy;       // lateral column (voxel - V/2)
+  int zbase;        // the window's first height - V/2
+  uint32_t fixed;   // byte offset of the column's element in the unshifted frame
+  float R2f, thr;   // R^2 in FP32 and the FP32 height path's margin
+  double R2;
+};
+
+// A batch of KB list entries: per entry the window slot of this column's hit
+// (-1: none), the angle, the spectrum element's byte offset (kMissOffset for
+// a miss) and, after issue(), the loaded value (and beta when !PRE).
+template <int KB>
+struct GBatch {
+  int kv[KB], ea[KB];
+  uint32_t el[KB];
+  float2 sp[KB];
+  float b[KB];
+  __device__ __forceinline__ void issue(const GatherArgs& g, size_t C2, uint32_t C2b) {
 #pragma unroll
-    for (int t = 0; t < kPrepBatch; ++t) {
-      const int4 e = sent[min(i0 + t, n - 1)];
-      const int fdx = qx + e.z, fdy = qy + e.w;
-      const int r2 = fdx * fdx + fdy * fdy;
-      inp[t] = fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx && r2 <= g.r2max;
-      hb[t] = height(R2, inp[t] ? r2 : 0);
-    }
+    for (int t = 0; t < KB; ++t) sp[t] = load_or_zero(g.spec + (size_t)ea[t] * C2, C2b, el[t]);
+    // keep the loads ahead of what follows (the other batch's sums): the
+    // scheduler would otherwise sink them below those independent LDS ops
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // Unconditional updates (a conditional one lets the compiler sink its
+  // loads behind the branch): a miss adds -0.0 to slot 0, which leaves every
+  // value bit-identical (x + -0.0 == x, also for x = +-0.0), and does not
+  // count.
+  template <bool PRE, int ZW>
+  __device__ __forceinline__ void apply(float2 (&acc)[ZW][256], unsigned short (&cn)[ZW][256],
+                                        int tid) const {
+#pragma clang fp contract(off)
 #pragma unroll
-    for (int t = 0; t < kPrepBatch; ++t) {
-      const int it = min(i0 + t, n - 1);
-      const int z = (int)round_away(hb[t] - sfz[it]) + g.V / 2;
-      const uint8_t o = inp[t] ? (uint8_t)(z - sent[it].y) : (uint8_t)255;
-      if (i0 + t < n) zo[(size_t)(i0 + t) * 256] = o;  // uniform
+    for (int t = 0; t < KB; ++t) {
+      const bool hit = kv[t] >= 0;
+      const int kc = hit ? kv[t] : 0;
+      // value (-b Si, b Sr)
+      const float pim = PRE ? sp[t].y : b[t] * sp[t].y;
+      const float pre = PRE ? sp[t].x : b[t] * sp[t].x;
+      float2 s = acc[kc][tid];
+      s.x = s.x + (hit ? -pim : -0.0f);
+      s.y = s.y + (hit ? pre : -0.0f);
+      acc[kc][tid] = s;
+      cn[kc][tid] = (unsigned short)(cn[kc][tid] + (hit ? 1 : 0));
     }
+  }
+};
+
+// Batch bi of a segment's compacted entries (seg[0 .. total), angle order):
+// entries i0 .. i0 + KB - 1, those past total miss (angle 0, no load). The
+// entries are read from LDS at a uniform address (a broadcast).
+template <int ZW, bool PRE, int KB>
+__device__ __forceinline__ void gather_fill(GBatch<KB>& B, const int4* seg, int i0, int total,
+                                            const GatherArgs& g, const GatherCol& c) {
+#pragma unroll
+  for (int t = 0; t < KB; ++t) {
+    const bool valid = i0 + t < total;  // uniform
+    const int4 en = seg[valid ? i0 + t : 0];
+    const int ea = en.x, fxy = en.y;
+    const float fz32 = __int_as_float(en.z);
+    const int fdx = c.qx + (int)(short)(fxy & 0xffff), fdy = c.qy + (fxy >> 16);
+    const int r2raw = fdx * fdx + fdy * fdy;
+    const bool inp = valid && fdx >= -g.nx && fdx < g.nx && fdy >= -g.nx && fdy < g.nx &&
+                     r2raw <= g.r2max;
+    const int r2 = inp ? r2raw : 0;
+    bool near;
+    int dz = height_off_f32(c.R2f, r2, fz32, c.thr, &near);
+    if (near && inp) dz = (int)round_away(height(c.R2, r2) - g.ageo[ea].fz);  // rare
+    const int k = dz - c.zbase;
+    const bool hit = inp && k >= 0 && k < ZW;
+    B.kv[t] = hit ? k : -1;
+    B.ea[t] = valid ? ea : 0;
+    B.el[t] = !hit ? kMissOffset
+                   : (g.frame_pos ? (uint32_t)((fdy + g.nx) * g.C + (fdx + g.nx)) * 8u : c.fixed);
+    if constexpr (!PRE) B.b[t] = g.beta[r2];
   }
 }
+
+constexpr int kGatherSeg = 256;  // list entries tested (and compacted) per segment
 
 // flags: bit0 = normalise (divide by counts, 0 -> 1) and apply the centring
 // sign (-1)^(x+y+z) of the 3-D inverse FFT; counts are then not written.
@@ -233,32 +271,35 @@ k_tile_angles(GatherArgs g, int ntx, int nchunk, TileEntry* __restrict__ list,
 // the Born forward model).
 // PRE: the spectra are already multiplied by the Ewald factor beta of their
 // pupil pixel (the same FP32 product), so beta is not looked up here.
-template <int ZW, bool PRE>
+// KB: list entries whose loads are issued together.
+template <int ZW, bool PRE, int KB>
 __global__ void __launch_bounds__(256)
-k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
-             const TileEntry* __restrict__ list, const int4* __restrict__ tinfo,
-             const uint8_t* __restrict__ zoff, int flags, const int2* __restrict__ band) {
+k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items, const int* __restrict__ order,
+             const TileEntry* __restrict__ list, const int4* __restrict__ tinfo, int flags,
+             const int2* __restrict__ band) {
   // The sums must be the sequential FP32 sums of the reference: no fused
   // multiply-add. The pragma is lexical, so the products and sums are written
   // here as plain operators (HIP's __fmul_rn/__fadd_rn are header functions
   // outside its scope and would still be contracted).
 #pragma clang fp contract(off)
-  // Blocks are dealt round-robin to the 8 XCDs. Tiles go round-robin too
-  // (the heavy tiles near the lateral origin are spread), and the windows
-  // of one tile stay on one XCD, in order (they read the same spectrum
-  // patches). A bijection when the tile count is a multiple of 8. Windows
-  // w0 .. w0 + nwin - 1 are launched.
+  // Blocks are dealt round-robin to the 8 XCDs. Tiles go to the XCDs in
+  // order of distance from the lateral origin (order[]: the heaviest first,
+  // spread over the XCDs), and the windows of one tile stay on one XCD, in
+  // order (they read the same spectrum patches). A bijection when the tile
+  // count is a multiple of 8. Windows w0 .. w0 + nwin - 1 are launched.
   int tile, w;
   {
     const int b = blockIdx.x, tiles = items / nwin;
+    int rank;
     if (tiles % 8 == 0) {
       const int n = b / 8;
-      tile = (n / nwin) * 8 + b % 8;
+      rank = (n / nwin) * 8 + b % 8;
       w = w0 + n % nwin;
     } else {
-      tile = b / nwin;
+      rank = b / nwin;
       w = w0 + b % nwin;
     }
+    tile = order[rank];
   }
   const int tid = threadIdx.x;
   const int X = (tile % ntx) * GT + tid % GT, Y = (tile / ntx) * GT + tid / GT;
@@ -292,79 +333,62 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items,
 #pragma unroll
   for (int k = 0; k < ZW; ++k) { acc[k][tid] = make_float2(0.f, 0.f); cn[k][tid] = 0; }
   const TileEntry* lst = list + (size_t)tile * g.A;
-  const uint8_t* zo = zoff + (size_t)tile * g.A * 256 + tid;
   // spectrum element read by this column: the unshifted frame holds it at
   // (qy mod C, qx mod C) for every angle; the +posSpec frame at fd + NXMAX
   const int frow = ((qy % g.C) + g.C) % g.C, fcol = ((qx % g.C) + g.C) % g.C;
-  const size_t fixed = (size_t)frow * g.C + fcol;
+  const uint32_t fixed = (uint32_t)(frow * g.C + fcol) * 8u;
+  const uint32_t C2b = (uint32_t)g.C * (uint32_t)g.C * 8u;  // bytes per angle
   const size_t C2 = (size_t)g.C * g.C;
-  // Each wave tests 64 entries against the window at once, then takes the
-  // overlapping ones lowest index first (angle order) kBatch at a time: the
-  // loads of a batch are issued together, the sums applied in order.
-  const int lane = tid & 63;
-  for (int j0 = 0; j0 < info.x; j0 += 64) {
-    const int j = j0 + lane;
-    TileEntry e{0, 0, 0, 0};
+  const double R2 = g.htab[0] * g.htab[0];
+  const float R2f = (float)R2;  // an integer: exact below 2^24
+  // the FP32 height path's margin, 4 ulp(R); everything takes the FP64 path
+  // where R^2 - r2 would not be exact in FP32
+  const float thr = R2 < 16777216.0 ? ldexpf(1.f, ilogbf((float)g.htab[0]) - 21) : 2.f;
+  const int zbase = z0 - g.V / 2;
+  // The tile's list in segments of kGatherSeg entries: each thread tests one
+  // entry against the window, the overlapping ones are compacted into LDS in
+  // list (angle) order, and every wave takes them KB at a time,
+  // software-pipelined two deep: the loads of batch i + 1 are in flight
+  // while the sums of batch i are applied (each voxel's sum still takes its
+  // hits in angle order). Both batches always issue their KB loads (misses
+  // fetch nothing), so the outstanding-load count is the same on every path.
+  __shared__ int4 seg[kGatherSeg];
+  __shared__ int wcount[4];
+  const int lane = tid & 63, wave = tid >> 6;
+  const GatherCol cg{qx, qy, zbase, fixed, R2f, thr, R2};
+  for (int s0 = 0; s0 < info.x; s0 += kGatherSeg) {  // uniform
+    __syncthreads();  // the previous segment is consumed
+    const int j = s0 + tid;
     bool ov = false;
+    TileEntry e{0, 0, 0, 0.f};
     if (j < info.x) {
       e = lst[j];
       const int zl = e.zpack & 0xffff, zh = e.zpack >> 16;
       ov = !(zh < z0 || zl > z1);
     }
-    unsigned long long m = __ballot(ov);  // wave-uniform
-    while (m) {
-      // Straight-line batch (no branch between the loads, so all of them are
-      // in flight together): unused slots repeat the first entry and are
-      // masked out afterwards.
-      int bitv[kBatch];
-      bool valid[kBatch];
+    const unsigned long long bal = __ballot(ov);
+    if (lane == 0) wcount[wave] = __popcll(bal);
+    __syncthreads();
+    int base = 0;
 #pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
-        valid[t] = m != 0;  // uniform; true for t = 0
-        bitv[t] = valid[t] ? __ffsll((unsigned long long)m) - 1 : bitv[0];
-        m &= m - 1;  // m == 0 stays 0
-      }
-      int offv[kBatch], zlv[kBatch];
-      float2 spv[kBatch];
-      float bv[kBatch];
-#pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
-        const int i = j0 + bitv[t];
-        const int ea = __builtin_amdgcn_readlane(e.a, bitv[t]);
-        zlv[t] = __builtin_amdgcn_readlane(e.zpack, bitv[t]) & 0xffff;
-        const int fxy = __builtin_amdgcn_readlane(e.fxy, bitv[t]);
-        const int fdx = qx + (int)(short)(fxy & 0xffff), fdy = qy + (fxy >> 16);
-        offv[t] = zo[(size_t)i * 256];
-        // addresses are clamped so that non-hits read in bounds; a hit has
-        // |fd| inside the crop and r2 <= r2max, where clamping is the identity
-        const int r2 = min(fdx * fdx + fdy * fdy, g.r2max);
-        const size_t sp = (size_t)(clampi(fdy, -g.nx, g.nx - 1) + g.nx) * g.C +
-                          (clampi(fdx, -g.nx, g.nx - 1) + g.nx);
-        spv[t] = g.spec[(size_t)ea * C2 + (g.frame_pos ? sp : fixed)];
-        if constexpr (!PRE) bv[t] = g.beta[r2];
-      }
-#pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
-        if (!valid[t]) offv[t] = 255;
-      }
-      // Unconditional updates (a conditional one lets the compiler sink its
-      // loads behind the branch): a miss adds -0.0 to slot 0, which leaves
-      // every value bit-identical (x + -0.0 == x, also for x = +-0.0), and
-      // does not count.
-#pragma unroll
-      for (int t = 0; t < kBatch; ++t) {
-        const int k = offv[t] == 255 ? -1 : zlv[t] + offv[t] - z0;
-        const bool hit = k >= 0 && k < ZW;
-        const int kc = hit ? k : 0;
-        // value (-b Si, b Sr)
-        const float pim = PRE ? spv[t].y : bv[t] * spv[t].y;
-        const float pre = PRE ? spv[t].x : bv[t] * spv[t].x;
-        float2 s = acc[kc][tid];
-        s.x = s.x + (hit ? -pim : -0.0f);
-        s.y = s.y + (hit ? pre : -0.0f);
-        acc[kc][tid] = s;
-        cn[kc][tid] = (unsigned short)(cn[kc][tid] + (hit ? 1 : 0));
-      }
+    for (int w2 = 0; w2 < 4; ++w2) base += w2 < wave ? wcount[w2] : 0;
+    const int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    if (ov)
+      seg[base + __popcll(bal & ((1ull << lane) - 1ull))] =
+          make_int4(e.a, e.fxy, __float_as_int(e.fz), 0);
+    __syncthreads();
+    const int nb = (total + KB - 1) / KB;
+    GBatch<KB> A, B;
+    gather_fill<ZW, PRE, KB>(A, seg, 0, total, g, cg);
+    A.issue(g, C2, C2b);
+    for (int bi = 0; bi < nb; bi += 2) {
+      gather_fill<ZW, PRE, KB>(B, seg, (bi + 1) * KB, total, g, cg);
+      B.issue(g, C2, C2b);
+      A.template apply<PRE>(acc, cn, tid);
+      if (bi + 1 >= nb) break;
+      gather_fill<ZW, PRE, KB>(A, seg, (bi + 2) * KB, total, g, cg);
+      A.issue(g, C2, C2b);
+      B.template apply<PRE>(acc, cn, tid);
     }
   }
   if (!inb) return;
@@ -433,13 +457,11 @@ hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, i
   return hipGetLastError();
 }
 
-static int prep_chunks(int A) { return std::max(1, (A + kTileChunk - 1) / kTileChunk); }
-
 // the windowed gather with height windows of ZW voxels (LDS: 10 B per
-// voxel of the 256 x ZW block)
-template <int ZW>
-static hipError_t gather_windows(const GatherArgs& g, int ntx, int tiles, const TileEntry* list,
-                                 const int4* tinfo, const uint8_t* zoff, int flags,
+// voxel of the 256 x ZW block) and batches of KB list entries
+template <int ZW, int KB>
+static hipError_t gather_windows(const GatherArgs& g, int ntx, int tiles, const int* order,
+                                 const TileEntry* list, const int4* tinfo, int flags,
                                  const int2* band, int zlo, int zhi, hipStream_t s) {
   const int V = g.V;
   // banded output: only the windows that can meet the band are launched
@@ -452,65 +474,91 @@ static hipError_t gather_windows(const GatherArgs& g, int ntx, int tiles, const 
   const int nwin = w1 - w0 + 1, items = tiles * nwin;
   Timed tm("ewald_gather", s);
   if (flags & 2) {
-    hipLaunchKernelGGL((k_gather_win<ZW, true>), dim3(items), dim3(256), 0, s, g, ntx, w0, nwin,
-                       items, list, tinfo, zoff, flags, band);
+    hipLaunchKernelGGL((k_gather_win<ZW, true, KB>), dim3(items), dim3(256), 0, s, g, ntx, w0,
+                       nwin, items, order, list, tinfo, flags, band);
   } else {
-    hipLaunchKernelGGL((k_gather_win<ZW, false>), dim3(items), dim3(256), 0, s, g, ntx, w0,
-                       nwin, items, list, tinfo, zoff, flags, band);
+    hipLaunchKernelGGL((k_gather_win<ZW, false, KB>), dim3(items), dim3(256), 0, s, g, ntx, w0,
+                       nwin, items, order, list, tinfo, flags, band);
   }
   return hipGetLastError();
 }
 
 bool gather_window_supported(int zw) { return zw == 8 || zw == 16 || zw == 24; }
 
-hipError_t launch_gather(const float2* spec, int frame_pos, const int* posx,
-                         const int* posy, const AngleGeo* ageo, int A, int C,
-                         int nxmax, int V, const double* htab,
-                         const float* beta, int r2max, float2* vol,
-                         int32_t* cnt, void* scratch, int flags, const int2* band,
-                         int zlo, int zhi, bool prepass, int zw, hipStream_t s) {
-  (void)posx;
-  (void)posy;
+// scratch: [tiles][A] entries | [tiles] info
+static void gather_scratch_split(void* scratch, int tiles, int A, TileEntry** list, int4** tinfo) {
+  const size_t nA = (size_t)(A > 0 ? A : 1);
+  *list = (TileEntry*)scratch;
+  *tinfo = (int4*)((char*)scratch + (size_t)tiles * nA * sizeof(TileEntry));
+}
+
+static hipError_t tile_prepass(const GatherArgs& g, int ntx, int tiles, TileEntry* list,
+                               int4* tinfo, hipStream_t s) {
+  Timed tm("ewald_tiles", s);
+  hipLaunchKernelGGL(k_tile_angles, dim3(tiles), dim3(64), 0, s, g, ntx, list, tinfo);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const float2* spec, int frame_pos, const AngleGeo* ageo, int A, int C,
+                         int nxmax, int V, const double* htab, const float* beta, int r2max,
+                         float2* vol, int32_t* cnt, void* scratch, const int* order, int flags,
+                         const int2* band, int zlo, int zhi, bool prepass, int zw, int kb,
+                         hipStream_t s) {
+  if ((size_t)C * C * sizeof(float2) >= ((size_t)1 << 31)) return hipErrorInvalidValue;
   GatherArgs g{spec, frame_pos, ageo, A, C, nxmax, V, htab, beta, r2max, vol, cnt};
   const int ntx = (V + GT - 1) / GT, tiles = ntx * ntx;
-  // scratch: [tiles][A] entries | [tiles][A][256] height offsets | [tiles] info
-  const size_t nA = (size_t)(A > 0 ? A : 1);
-  TileEntry* list = (TileEntry*)scratch;
-  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
-  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
+  TileEntry* list;
+  int4* tinfo;
+  gather_scratch_split(scratch, tiles, A, &list, &tinfo);
   if (prepass) {
-    Timed tm("ewald_tiles", s);
-    hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
-                       prep_chunks(A), list, tinfo, zoff);
+    const hipError_t e = tile_prepass(g, ntx, tiles, list, tinfo, s);
+    if (e != hipSuccess) return e;
+  }
+#define MTD_GW(ZW, KB) \
+  return gather_windows<ZW, KB>(g, ntx, tiles, order, list, tinfo, flags, band, zlo, zhi, s)
+  if (kb == 16) {
+    switch (zw) {
+      case 8: MTD_GW(8, 16);
+      case 24: MTD_GW(24, 16);
+      default: MTD_GW(16, 16);
+    }
   }
   switch (zw) {
-    case 8:
-      return gather_windows<8>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
-    case 24:
-      return gather_windows<24>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
-    default:
-      return gather_windows<16>(g, ntx, tiles, list, tinfo, zoff, flags, band, zlo, zhi, s);
+    case 8: MTD_GW(8, 8);
+    case 24: MTD_GW(24, 8);
+    default: MTD_GW(16, 8);
   }
+#undef MTD_GW
 }
 
 hipError_t launch_gather_prepass(const AngleGeo* ageo, int A, int C, int nxmax, int V,
                                  const double* htab, int r2max, void* scratch, hipStream_t s) {
   GatherArgs g{nullptr, 0, ageo, A, C, nxmax, V, htab, nullptr, r2max, nullptr, nullptr};
   const int ntx = (V + GT - 1) / GT, tiles = ntx * ntx;
-  const size_t nA = (size_t)(A > 0 ? A : 1);
-  TileEntry* list = (TileEntry*)scratch;
-  uint8_t* zoff = (uint8_t*)scratch + (size_t)tiles * nA * sizeof(TileEntry);
-  int4* tinfo = (int4*)(zoff + (size_t)tiles * nA * 256);
-  Timed tm("ewald_tiles", s);
-  hipLaunchKernelGGL(k_tile_angles, dim3(tiles * prep_chunks(A)), dim3(256), 0, s, g, ntx,
-                     prep_chunks(A), list, tinfo, zoff);
-  return hipGetLastError();
+  TileEntry* list;
+  int4* tinfo;
+  gather_scratch_split(scratch, tiles, A, &list, &tinfo);
+  return tile_prepass(g, ntx, tiles, list, tinfo, s);
 }
 
 size_t gather_scratch_bytes(int V, int A) {
   const size_t tiles = (size_t)((V + GT - 1) / GT) * ((V + GT - 1) / GT);
   const size_t nA = (size_t)(A > 0 ? A : 1);
-  return tiles * nA * (sizeof(TileEntry) + 256) + tiles * sizeof(int4);  // 16 + 256 B per entry
+  return tiles * nA * sizeof(TileEntry) + tiles * sizeof(int4);  // 16 B per entry
+}
+
+// The gather's dispatch order of the tiles of a V^3 volume: by distance of
+// the tile centre from the lateral origin, nearest first (the tiles there
+// hold the longest hit lists), ties by index.
+void gather_tile_order_host(int V, int* order) {
+  const int ntx = (V + GT - 1) / GT, tiles = ntx * ntx;
+  std::vector<std::pair<long, int>> key(tiles);
+  for (int t = 0; t < tiles; ++t) {
+    const long dx = 2L * ((t % ntx) * GT) + GT - V, dy = 2L * ((t / ntx) * GT) + GT - V;
+    key[t] = {dx * dx + dy * dy, t};
+  }
+  std::sort(key.begin(), key.end());
+  for (int t = 0; t < tiles; ++t) order[t] = key[t].second;
 }
 
 // ---------------------------------------------------------------------------

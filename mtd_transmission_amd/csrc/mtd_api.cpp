@@ -125,6 +125,19 @@ const float2* twiddle(mtd_ctx* ctx, int L) {
   return d;
 }
 
+// the gather's tile dispatch order for a V^3 volume
+static const int* tile_order(mtd_ctx* ctx, int V) {
+  auto it = ctx->tile_orders.find(V);
+  if (it != ctx->tile_orders.end()) return it->second;
+  const int ntx = (V + 15) / 16;
+  std::vector<int> h((size_t)ntx * ntx);
+  gather_tile_order_host(V, h.data());
+  int* d = upload(h.data(), h.size());
+  if (!d) return nullptr;
+  ctx->tile_orders[V] = d;
+  return d;
+}
+
 // ---------------------------------------------------------------------------
 // runtime-length FFT plans (gen_fft.hpp)
 // ---------------------------------------------------------------------------
@@ -899,6 +912,7 @@ int mtd_ctx_destroy(mtd_ctx* ctx) {
   timing_collect(ctx);
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (auto& kv : ctx->tile_orders) (void)hipFree(kv.second);
   for (auto& kv : ctx->gen_plans) delete kv.second;
   for (auto& kv : ctx->herraez_border) (void)hipFree(kv.second);
   for (auto& kv : ctx->bluestein) {
@@ -965,6 +979,42 @@ int mtd_ctx_set_schedule(mtd_ctx* ctx, int holo_chunk, int gather_window) {
                                                  " not built (8, 16, 24)");
   ctx->holo_chunk_override = holo_chunk;
   ctx->gather_zw_override = gather_window;
+  return MTD_OK;
+}
+
+// the named schedule parameters: their context slot and accepted values
+static int* sched_slot(mtd_ctx* ctx, const std::string& name) {
+  if (name == "holo_chunk") return &ctx->holo_chunk_override;
+  if (name == "gather_window") return &ctx->gather_zw_override;
+  if (name == "gather_batch") return &ctx->gather_kb_override;
+  return nullptr;
+}
+
+static bool sched_value_ok(const std::string& name, int v) {
+  if (v == 0) return true;  // the default
+  if (name == "holo_chunk") return v > 0;
+  if (name == "gather_window") return gather_window_supported(v);
+  if (name == "gather_batch") return v == 8 || v == 16;
+  return false;
+}
+
+int mtd_ctx_set_schedule_param(mtd_ctx* ctx, const char* name, int value) {
+  if (!ctx || !name) return MTD_ERR_ARG;
+  int* slot = sched_slot(ctx, name);
+  if (!slot) return set_err(ctx, MTD_ERR_ARG, std::string("unknown schedule parameter ") + name);
+  if (!sched_value_ok(name, value))
+    return set_err(ctx, MTD_ERR_UNSUPPORTED,
+                   std::string("schedule parameter ") + name + " = " + std::to_string(value) +
+                       " not built");
+  *slot = value;
+  return MTD_OK;
+}
+
+int mtd_ctx_get_schedule_param(mtd_ctx* ctx, const char* name, int* value) {
+  if (!ctx || !name || !value) return MTD_ERR_ARG;
+  int* slot = sched_slot(ctx, name);
+  if (!slot) return set_err(ctx, MTD_ERR_ARG, std::string("unknown schedule parameter ") + name);
+  *value = *slot;
   return MTD_OK;
 }
 
@@ -1221,12 +1271,13 @@ static int scatter_impl(mtd_ctx* ctx, const mtd_geom* g, const float2* spec, int
   const int V = g->dim_final;
   const int zlo = V / 2 + (int)std::round(std::sqrt(t.R2 - (double)t.r2max) - g->rayon);
   const int zhi = V / 2 + (int)std::round(g->rayon);
-  MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, posspec, posspec + A, ag, A,
-                                       2 * g->nxmax, g->nxmax, V, t.htab,
+  const int* order = tile_order(ctx, V);
+  if (!order) return set_err(ctx, MTD_ERR_HIP, "out of device memory (tile order)");
+  MTD_TRY(hip_check(ctx, launch_gather(spec, frame_pos, ag, A, 2 * g->nxmax, g->nxmax, V, t.htab,
                                        weights ? weights : t.beta, t.r2max, (float2*)vol, counts,
-                                       scratch, flags, band,
-                                       zlo, zhi, plan == nullptr,
+                                       scratch, order, flags, band, zlo, zhi, plan == nullptr,
                                        ctx->gather_zw_override > 0 ? ctx->gather_zw_override : 16,
+                                       ctx->gather_kb_override > 0 ? ctx->gather_kb_override : 8,
                                        ctx->stream),
                     "Ewald gather"));
   return MTD_OK;
