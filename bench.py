@@ -392,10 +392,16 @@ def run_batched(args):
     re = [torch.empty((g.dim_final,) * 3, device="cuda") for g in geoms]
     im = [torch.empty_like(r) for r in re]
     pos = [torch.empty((2 * h.shape[0],), dtype=torch.int32, device="cuda") for h in holos]
+    import tempfile
+    cache = args.tune_cache or os.path.join(tempfile.mkdtemp(prefix="mtd_tune_"), "tune.txt")
     ctx = m.Context(0)
-    # eager batch (no graph) for comparison, after a warm-up
+    ctx.set_tune_cache(cache)
+    # eager batch (no graph) for comparison, after a warm-up (which tunes
+    # every shape not in the cache file)
+    t0 = time.perf_counter()
     ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
     torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
@@ -419,8 +425,25 @@ def run_batched(args):
                                          autotune=True, times=True).tolist())
     lat = np.asarray(lat)
     per_acq = [dict(s, ms=round(float(t), 4)) for s, t in zip(shapes, lat[-len(shapes):])]
-    chunks = {f"{g.n}x{g.C}": [ctx.batch_tuned_chunk(g.n, g.C), ctx.batch_tuned_window(g.n, g.C)]
-              for g in geoms}
+    scheds = {}
+    for g in geoms:
+        sc = ctx.batch_tuned_schedule(g.n, g.C, g.dim_final)
+        scheds[f"{g.n}x{g.C}x{g.dim_final}"] = (
+            [sc[k] for k in ("holo_chunk", "gather_window", "gather_batch", "vol_rows_tile",
+                             "vol_cols_tile", "from_cache")] if sc else None)
+    ctx.close()
+    # a second context on the same cache file: the schedules are read back,
+    # nothing is re-tuned
+    ctx = m.Context(0)
+    ctx.set_tune_cache(cache)
+    cached = sum(1 for g in geoms if (ctx.batch_tuned_schedule(g.n, g.C, g.dim_final) or {})
+                 .get("from_cache") == 1)
+    t0 = time.perf_counter()
+    ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
+    torch.cuda.synchronize()
+    second_first_ms = (time.perf_counter() - t0) * 1e3
+    ctx.close()
+    small = batched_small_graph_gain(args, m, synth, torch, np)
     out = {
         "metric": METRIC, "value": round(total_A / (ms / 1e3), 2), "unit": "holograms/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -434,14 +457,56 @@ def run_batched(args):
                     "p99_ms": round(float(np.percentile(lat, 99)), 4),
                     "eager_ms_per_batch": round(eager_ms, 4),
                     "graph_ms_per_batch": round(ms, 4),
-                    "tuned_schedules": {"key": "hologram side x crop",
+                    "tuned_schedules": {"key": "hologram side x crop x volume side",
                                         "value": "[holograms per hologram launch pair, "
-                                                 "gather window height]", **chunks},
+                                                 "gather window height, gather load batch, "
+                                                 "volume row tile variant, volume column "
+                                                 "tile variant, read from cache]", **scheds},
+                    "tune_cache": {"file": cache, "first_call_s_with_tuning": round(tune_s, 2),
+                                   "second_context_shapes_from_cache": cached,
+                                   "second_context_first_call_ms": round(second_first_ms, 2)},
+                    "graph_gain_small_shapes": small,
                     "acquisitions": per_acq},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def batched_small_graph_gain(args, m, synth, torch, np, count=32):
+    """Where the captured graph pays: many small acquisitions (256^2
+    holograms, crop 64, 128^3, 24 angles), whose kernels last a few
+    microseconds each, eager launches against one graph replay."""
+    geoms, holos, masks = [], [], []
+    for i in range(count):
+        w = dict(n=256, nxmax=32, v=128, A=24, seed=5000 + i)
+        n, nx, V, A, pix, cx, cy = geometry(w)
+        kx, ky = synth.scan_fermat(A, nx, 0.95)
+        # the phantom scaled into the small field of view (0.45 n pix)
+        ph = synth.scale_phantom(synth.make_phantom(w["seed"], 2), 0.45 * n * pix / 33.8e-6)
+        h, mask = synth.simulate_torch(n, nx, cx, cy, pix, LAM, N0, kx, ky, ph,
+                                       seed=w["seed"] + 1, lowpass=False)
+        geoms.append(m.geom_derive(n, nx, V, cx, cy, NA, N0, LAM, pix))
+        holos.append(h)
+        masks.append(mask)
+    re = [torch.empty((g.dim_final,) * 3, device="cuda") for g in geoms]
+    im = [torch.empty_like(r) for r in re]
+    pos = [torch.empty((2 * h.shape[0],), dtype=torch.int32, device="cuda") for h in holos]
+    ctx = m.Context(0)
+    res = {}
+    for graph in (False, True):
+        for _ in range(2):
+            ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=graph)
+        torch.cuda.synchronize()
+        res["graph_ms" if graph else "eager_ms"] = round(
+            (time.perf_counter() - t0) * 1e3 / args.steps, 4)
     ctx.close()
+    res["acquisitions"] = count
+    res["shape"] = "256^2 holograms, crop 64, 128^3, 24 angles"
+    return res
 
 
 def main():
@@ -455,6 +520,9 @@ def main():
     ap.add_argument("--cpu-child", nargs=3, help=argparse.SUPPRESS)
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the plain vs adjoint-TV comparison (differentiation form)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="batched workload: the autotuner's cache file (default: a new "
+                         "temporary file, so the first context tunes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device copies of 1 GiB (reference kernel for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling) and exit")

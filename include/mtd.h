@@ -113,8 +113,11 @@ int mtd_ctx_set_schedule(mtd_ctx* ctx, int holo_chunk, int gather_window);
 /* Named schedule parameters of the plain path (no reference counterpart;
  * results are the same for every value): "holo_chunk" (as above),
  * "gather_window" (8, 16, 24), "gather_batch" (list entries whose spectrum
- * loads the gather issues together: 8, 16). 0 restores the default. An
- * unknown name is MTD_ERR_ARG, a value not built MTD_ERR_UNSUPPORTED. */
+ * loads the gather issues together: 8, 16), "vol_rows_tile" (the volume
+ * row pass with 1/2 the lines per block: 1), "vol_cols_tile" (the volume
+ * column passes with 1/2 or 2x the columns per block: 1, 2). 0 restores the
+ * default. An unknown name is MTD_ERR_ARG, a value not built
+ * MTD_ERR_UNSUPPORTED. */
 int mtd_ctx_set_schedule_param(mtd_ctx* ctx, const char* name, int value);
 int mtd_ctx_get_schedule_param(mtd_ctx* ctx, const char* name, int* value);
 /* Restricts the timing to the kernel of that report name (NULL or "": all
@@ -254,10 +257,23 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms,
                           int32_t* const* posspec, float* const* index_re,
                           float* const* index_im, int flags, float* acq_ms);
 /* The hologram chunk and the gather height window (8, 16 or 24 voxels) the
- * autotuner picked for (hologram side n, crop C, dtype); 0 if that shape was
- * not tuned. */
+ * autotuner picked for (hologram side n, crop C, dtype), of the first volume
+ * side tuned for them; 0 if that shape was not tuned. */
 int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype);
 int mtd_batch_tuned_window(mtd_ctx* ctx, int n, int C, int dtype);
+/* The whole schedule picked for (n, C, V, dtype), into sched[6]: hologram
+ * chunk, gather window, gather load batch, volume row-pass tile variant
+ * (0, 1), volume column-pass tile variant (0, 1, 2), and 1 if it was read
+ * from the cache file (0 if tuned in this process). Returns 1 if the shape
+ * is tuned, 0 if not (sched all 0). */
+int mtd_batch_tuned_schedule(mtd_ctx* ctx, int n, int C, int V, int dtype, int* sched);
+/* Schedule cache file of the autotuner (no reference counterpart): its
+ * entries for this library version and device are loaded now, and every
+ * shape tuned from now on is appended to it, keyed by (N, C, V, dtype),
+ * library version and device. NULL or "": no file. A context starts with
+ * the file named by the environment variable MTD_TUNE_CACHE, if set. A
+ * missing file is not an error (nothing tuned yet). */
+int mtd_ctx_set_tune_cache(mtd_ctx* ctx, const char* path);
 
 /* ---- Born forward model (Simul_holo) and its adjoint -------------------- */
 /* The simulator's hologram-spectrum synthesis (Simul_holo/main.cpp:106-109,

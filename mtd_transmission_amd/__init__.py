@@ -87,6 +87,8 @@ SIGNATURES = {
     "mtd_reconstruct_batch": (_I, [_P, _I, _GP, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "mtd_batch_tuned_chunk": (_I, [_P, _I, _I, _I]),
     "mtd_batch_tuned_window": (_I, [_P, _I, _I, _I]),
+    "mtd_batch_tuned_schedule": (_I, [_P, _I, _I, _I, _I, ctypes.POINTER(_I)]),
+    "mtd_ctx_set_tune_cache": (_I, [_P, ctypes.c_char_p]),
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
@@ -465,6 +467,25 @@ class Context:
 
     def batch_tuned_window(self, n, C, dtype=MTD_U16):
         return lib().mtd_batch_tuned_window(self.h, n, C, dtype)
+
+    def batch_tuned_schedule(self, n, C, V, dtype=MTD_U16):
+        """The autotuner's schedule for (n, C, V, dtype) as a dict, or None
+        if that shape is not tuned."""
+        out = (_I * 6)()
+        found = lib().mtd_batch_tuned_schedule(self.h, n, C, V, dtype, out)
+        if found < 0:
+            self._check(found, "batch_tuned_schedule")
+        if found == 0:
+            return None
+        keys = ("holo_chunk", "gather_window", "gather_batch", "vol_rows_tile",
+                "vol_cols_tile", "from_cache")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def set_tune_cache(self, path):
+        """Load the autotuner's cache file (entries of this library version
+        and device) and append what it tunes from now on; None: no file."""
+        self._check(lib().mtd_ctx_set_tune_cache(
+            self.h, None if path is None else os.fsencode(path)), "set_tune_cache")
 
     # ---- Born forward model and adjoint ---------------------------------
     def spectra_to_fields(self, g: Geom, spectra, posspec, fields):

@@ -62,6 +62,19 @@ struct EvPair {
 
 }  // namespace mtd
 
+// A schedule of the plain path picked by the batched form's autotuner (every
+// schedule gives the same results): holograms per hologram-stage chunk, the
+// gather's height window and load batch (0: defaults), the volume passes'
+// tile variants, the time it measured (ms) and whether it came from the
+// cache file.
+struct TunedSched {
+  int chunk, zw, kb, rows_tile, cols_tile;
+  float ms;
+  bool from_cache;
+};
+// (hologram side N, crop side C, volume side V, hologram dtype)
+using TuneKey = std::tuple<int, int, int, int>;
+
 struct mtd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -106,11 +119,15 @@ struct mtd_ctx {
   // captured at another generation holds stale addresses and is recaptured
   uint64_t ws_gen = 0;
   uint64_t batch_gen = 0;
-  // (N, C, dtype) -> (holograms per hologram-stage chunk, gather window)
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> batch_tuned;
+  // (N, C, V, dtype) -> the autotuner's schedule; the cache file it is
+  // persisted to ("": none)
+  std::map<TuneKey, TunedSched> batch_tuned;
+  std::string tune_cache;
   int holo_chunk_override = 0;  // > 0: holograms per hologram-stage chunk
   int gather_zw_override = 0;   // > 0: gather height window (8, 16, 24)
   int gather_kb_override = 0;   // > 0: gather entries per load batch (8, 16)
+  int vol_rows_tile = 0;        // tile variant of the volume row pass (0, 1)
+  int vol_cols_tile = 0;        // tile variant of the volume column passes (0, 1, 2)
   std::map<int, int*> tile_orders;  // the gather's tile dispatch order per volume side
   // geometry plan of the forward model / adjoint (mtd_born_plan): speculars,
   // angle geometry, hit band and the gather's tile lists, in WS_PLAN* slots
@@ -336,6 +353,6 @@ hipError_t launch_cap_sample(const float2* vol, const AngleGeo* ageo, int A, con
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
                            int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
-                           uint8_t* otf, const FftTab& ft, hipStream_t s);
+                           uint8_t* otf, const FftTab& ft, int2 tiles, hipStream_t s);
 
 }  // namespace mtd

@@ -16,6 +16,7 @@
 #include "gen_fft.hpp"
 #include "internal.hpp"
 #include "passes.hpp"
+#include "trig.hpp"
 
 namespace mtd {
 
@@ -261,7 +262,7 @@ __device__ __forceinline__ float2 form_from(float2 u, float phf, double bgp, dou
   const float aa = (float)fabs(A);
   if (mode & 1) {
     float sp, cp;
-    sincosf(pc, &sp, &cp);
+    sincos_cw(pc, &sp, &cp);
     return make_float2(aa * cp - 1.0f, aa * sp);
   }
   return make_float2(logf(aa), pc);
@@ -280,7 +281,7 @@ __device__ __forceinline__ float2 form_born_wrapped(float2 u, double bgp, double
   if (A < -10) A = 1;
   const float aa = (float)fabs(A);
   float sb, cb;
-  sincosf((float)bgp, &sb, &cb);
+  sincos_cw((float)bgp, &sb, &cb);
   const float k = amf > 0.f ? aa * __builtin_amdgcn_rcpf(amf) : 0.f;
   return make_float2(k * (u.x * cb + u.y * sb) - 1.0f, k * (u.y * cb - u.x * sb));
 }
@@ -468,10 +469,10 @@ struct QRowOp {
     ql.begin(a, f.C);
     f.begin(a);
   }
-  __device__ void line(int a, int k) {
-    ql.set_row(a, k, f.C);
-    f.line(a, k);
-  }
+  __device__ void line(int a, int k) { ql.set_row(a, k, f.C); }
+  // the field formation's row state (window factor, background polynomials)
+  // once the row's inverse transform is done
+  __device__ void mid_line(int a, int k) { f.line(a, k); }
   __device__ float2 load(int a, int k, int u) const { return ql.at(a, k, u, f.C); }
   __device__ float2 mid(int, int, int l, float2 iu) const { return f.form_iu(l, iu); }
   __device__ void store(int a, int k, int j, float2 v) const { f.store(a, k, j, v); }

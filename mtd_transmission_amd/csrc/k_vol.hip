@@ -648,24 +648,52 @@ struct ZOutOp {  // lines along z; a = y, col = x, row = z
   Dim<VT> V{};
 };
 
+// the compiled row / column pass of tile variant TV (passes.hpp)
+template <int VT, bool INV, int TV, class Op>
+static void vol_rows_tv(const Op& op, hipStream_t s) {
+  hipLaunchKernelGGL((k_rows<VT, INV, Op, TV>), (rows_grid<VT, TV>(VT)), (rows_block<VT, TV>()),
+                     0, s, op);
+}
+template <int VT, bool INV, int TV, class Op>
+static void vol_cols_tv(const Op& op, hipStream_t s) {
+  hipLaunchKernelGGL((k_cols<VT, INV, Op, TV>), (cols_grid<VT, Op, TV>(VT)),
+                     (cols_block<VT, Op, TV>()), 0, s, op);
+}
+
 // one row pass and two column passes over V batches: compiled register FFTs
-// (VT > 0) or the runtime-length FFTs (VT == 0)
-template <int VT, bool INV, class RX, class CY, class CZ>
+// (VT > 0) or the runtime-length FFTs (VT == 0). TILES: the tile variants
+// tiles.x (rows: 0, 1) and tiles.y (columns: 0, 1, 2) are compiled and
+// picked at run time; otherwise variant 0.
+template <int VT, bool INV, bool TILES, class RX, class CY, class CZ>
 static hipError_t vol_passes(Dim<VT> V, const RX& x, const CY& y, const CZ& z, const FftTab& ft,
-                             const char* nx, const char* ny, const char* nz, hipStream_t s) {
+                             const char* nx, const char* ny, const char* nz, int2 tiles,
+                             hipStream_t s) {
   if constexpr (VT > 0) {
+    const int rt = TILES ? tiles.x : 0, ct = TILES ? tiles.y : 0;
     {
       Timed tm(nx, s);
-      hipLaunchKernelGGL((k_rows<VT, INV, RX>), rows_grid<VT>(VT), rows_block<VT>(), 0, s, x);
+      if constexpr (TILES) {
+        if (rt == 1) vol_rows_tv<VT, INV, 1>(x, s);
+        else vol_rows_tv<VT, INV, 0>(x, s);
+      } else {
+        vol_rows_tv<VT, INV, 0>(x, s);
+      }
     }
+    auto cols = [&](const auto& op) {
+      if constexpr (TILES) {
+        if (ct == 1) vol_cols_tv<VT, INV, 1>(op, s);
+        else if (ct == 2) vol_cols_tv<VT, INV, 2>(op, s);
+        else vol_cols_tv<VT, INV, 0>(op, s);
+      } else {
+        vol_cols_tv<VT, INV, 0>(op, s);
+      }
+    };
     {
       Timed tm(ny, s);
-      hipLaunchKernelGGL((k_cols<VT, INV, CY>), (cols_grid<VT, CY>(VT)), (cols_block<VT, CY>()),
-                         0, s, y);
+      cols(y);
     }
     Timed tm(nz, s);
-    hipLaunchKernelGGL((k_cols<VT, INV, CZ>), (cols_grid<VT, CZ>(VT)), (cols_block<VT, CZ>()), 0,
-                       s, z);
+    cols(z);
     return hipGetLastError();
   } else {
     if (!ft.gen) return hipErrorInvalidValue;
@@ -689,12 +717,12 @@ template <int VT>
 static hipError_t finalize_v(Dim<VT> V, const float2* vol, const int32_t* cnt, int prenorm,
                              const int2* band, float2* work,
                              float df3, float cte, float gsign, float* re,
-                             float* im, uint8_t* otf, const FftTab& ft,
+                             float* im, uint8_t* otf, const FftTab& ft, int2 tiles,
                              hipStream_t s) {
   NormXOp<VT> x{vol, cnt, work, otf, ft.tw, prenorm, band, V};
   PlaneYOp<VT> y{work, ft.tw, band, V};
   ZOutOp<VT> z{work, re, im, ft.tw, df3, cte, gsign, band, 0, V - 1, V};
-  return vol_passes<VT, true>(V, x, y, z, ft, "vol_x", "vol_y", "vol_z", s);
+  return vol_passes<VT, true, true>(V, x, y, z, ft, "vol_x", "vol_y", "vol_z", tiles, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -754,7 +782,8 @@ static hipError_t contrast_spectrum_v(Dim<VT> V, const float* re, const float* i
   ContrastXOp<VT> x{re, im, work, ft.tw, V};
   PlaneYOp<VT> y{work, ft.tw, nullptr, V};
   SpectrumZOp<VT> z{work, ft.tw, scale, band, 0, V - 1, V};
-  return vol_passes<VT, false>(V, x, y, z, ft, "fwd_vol_x", "fwd_vol_y", "fwd_vol_z", s);
+  return vol_passes<VT, false, false>(V, x, y, z, ft, "fwd_vol_x", "fwd_vol_y", "fwd_vol_z",
+                                      make_int2(0, 0), s);
 }
 
 // calcHolo's sampling, in the frame of the per-angle spectra (+posSpec frame:
@@ -826,17 +855,17 @@ k_cap_weights(const int32_t* __restrict__ cnt, const AngleGeo* __restrict__ ageo
 hipError_t launch_finalize(int V, const float2* vol, const int32_t* cnt,
                            int prenorm, const int2* band, float2* work, float df3, float cte,
                            float gsign, float* out_re, float* out_im,
-                           uint8_t* otf, const FftTab& ft, hipStream_t s) {
+                           uint8_t* otf, const FftTab& ft, int2 tiles, hipStream_t s) {
   switch (V) {
 #define X(n)                                                                                   \
   case n:                                                                                      \
     return finalize_v<n>(Dim<n>{}, vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, \
-                         out_im, otf, ft, s);
+                         out_im, otf, ft, tiles, s);
     MTD_VOL_SIZES(X)
 #undef X
   }
   return finalize_v<0>(Dim<0>(V), vol, cnt, prenorm, band, work, df3, cte, gsign, out_re, out_im,
-                       otf, ft, s);
+                       otf, ft, tiles, s);
 }
 
 hipError_t launch_contrast_spectrum(int V, const float* re, const float* im, float2* work,

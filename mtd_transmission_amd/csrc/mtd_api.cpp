@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -766,11 +767,28 @@ static int front_fused(mtd_ctx* ctx, const mtd_geom* g, const float2* q, int n, 
                    "field stage");
 }
 
-// Hologram stage: the N x C row-pass intermediate of a chunk stays within
-// the 256 MiB Infinity Cache. Field stage: about 1 GiB of per-angle buffers
-// per launch, so that its small kernels get hundreds of angles at once.
+// Hologram stage: up to 1 GiB of the N x C row-pass intermediate per chunk
+// for stacks in device memory (fewer, larger launch pairs: measured over
+// 600 holograms of 1024^2, crop 256, chunks of 8 to 600 holograms ran
+// fastest at the largest; a chunk small enough for its intermediate to stay
+// in cache between the passes is slower, the launch tails dominate), 128 MiB
+// for host stacks (their chunks are the upload granularity, overlapped with
+// the passes). Field stage: about 1 GiB of per-angle buffers per launch, so
+// that its small kernels get hundreds of angles at once.
 static const size_t kChunkBudget = 128ull << 20;
+static const size_t kChunkBudgetDevice = 1ull << 30;
 static const size_t kFieldBudget = 1ull << 30;
+
+// holograms per hologram-stage chunk when no schedule sets it: the budget
+// of row-pass intermediate, split into even chunks
+static int default_holo_chunk(int N, int C, int n, bool host) {
+  int hc = chunk_of((size_t)N * C * sizeof(float2), host ? kChunkBudget : kChunkBudgetDevice, n);
+  if (hc < n) {
+    const int nch = (n + hc - 1) / hc;
+    hc = (n + nch - 1) / nch;
+  }
+  return hc;
+}
 
 // device bytes per angle of the field stage (crop, work, IU, and with
 // unwrapping its scratch and the phase)
@@ -820,11 +838,12 @@ static int copy_engine(mtd_ctx* ctx) {
 static int holo_range(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int f0,
                       int n, float2* crop, float2* part, int S, const float2* qtw = nullptr) {
   const int N = g->n, C = 2 * g->nxmax;
+  const bool host = is_host_ptr(holos);
   const int hc = ctx->holo_chunk_override > 0
                      ? std::min(ctx->holo_chunk_override, std::max(n, 1))
-                     : chunk_of((size_t)N * C * sizeof(float2), kChunkBudget, n);
+                     : default_holo_chunk(N, C, n, host);
   auto part_at = [&](int h0) { return part ? part + (size_t)h0 * S : nullptr; };
-  if (!is_host_ptr(holos)) {
+  if (!host) {
     for (int h0 = 0; h0 < n; h0 += hc)
       MTD_TRY(holo_chunk(ctx, g, holos, dtype, f0 + h0, std::min(hc, n - h0),
                          crop + (size_t)h0 * C * C, part_at(h0), qtw));
@@ -879,6 +898,8 @@ extern "C" {
 
 const char* mtd_version(void) { return "mtd-mi355x 0.1 (gfx950)"; }
 
+static int tune_cache_load(mtd_ctx* ctx);
+
 // message of the last failed mtd_ctx_create on this thread
 static thread_local std::string tl_create_err;
 
@@ -900,6 +921,10 @@ int mtd_ctx_create(int device, mtd_ctx** out) {
   mtd_ctx* c = new mtd_ctx();
   c->device = device;
   c->stream = nullptr;
+  if (const char* cache = std::getenv("MTD_TUNE_CACHE")) {
+    c->tune_cache = cache;
+    (void)tune_cache_load(c);  // an unreadable file only means nothing is cached
+  }
   *out = c;
   return MTD_OK;
 }
@@ -987,6 +1012,8 @@ static int* sched_slot(mtd_ctx* ctx, const std::string& name) {
   if (name == "holo_chunk") return &ctx->holo_chunk_override;
   if (name == "gather_window") return &ctx->gather_zw_override;
   if (name == "gather_batch") return &ctx->gather_kb_override;
+  if (name == "vol_rows_tile") return &ctx->vol_rows_tile;
+  if (name == "vol_cols_tile") return &ctx->vol_cols_tile;
   return nullptr;
 }
 
@@ -995,6 +1022,8 @@ static bool sched_value_ok(const std::string& name, int v) {
   if (name == "holo_chunk") return v > 0;
   if (name == "gather_window") return gather_window_supported(v);
   if (name == "gather_batch") return v == 8 || v == 16;
+  if (name == "vol_rows_tile") return v == 1;
+  if (name == "vol_cols_tile") return v == 1 || v == 2;
   return false;
 }
 
@@ -1338,6 +1367,7 @@ static int finalize_impl(mtd_ctx* ctx, const mtd_geom* g, const float* vol,
   const float gsign = ((V / 2) & 1) ? -1.f : 1.f;
   return hip_check(ctx, launch_finalize(V, (const float2*)vol, counts, prenorm, band, work, df3,
                                         (float)cte, gsign, index_re, index_im, otf, twV,
+                                        make_int2(ctx->vol_rows_tile, ctx->vol_cols_tile),
                                         ctx->stream),
                    "finalize");
 }
@@ -1693,38 +1723,111 @@ static uint64_t fnv1a(const uint8_t* p, size_t n) {
   return h;
 }
 
+// the context's schedule parameters, read or applied as one record
+static TunedSched sched_get(const mtd_ctx* ctx) {
+  return TunedSched{ctx->holo_chunk_override, ctx->gather_zw_override, ctx->gather_kb_override,
+                    ctx->vol_rows_tile, ctx->vol_cols_tile, 0.f, false};
+}
+static void sched_set(mtd_ctx* ctx, const TunedSched& t) {
+  ctx->holo_chunk_override = t.chunk;
+  ctx->gather_zw_override = t.zw;
+  ctx->gather_kb_override = t.kb;
+  ctx->vol_rows_tile = t.rows_tile;
+  ctx->vol_cols_tile = t.cols_tile;
+}
+
+static TuneKey tune_key(const mtd_geom* g, int dtype) {
+  return std::make_tuple(g->n, 2 * g->nxmax, g->dim_final, dtype);
+}
+
+// ---------------------------------------------------------------------------
+// Schedule cache file. One text line per tuned shape:
+//   mtd-tune <library version> | <device arch> | N C V dtype chunk zw kb rows cols ms
+// Lines of another library version or device are skipped, the last line of
+// a shape wins; new entries are appended one line per write.
+// ---------------------------------------------------------------------------
+static std::string device_arch(int device) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+  std::string a = p.gcnArchName;
+  const size_t c = a.find(':');  // drop feature flags (":sramecc+:xnack-")
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+static std::string tune_prefix(int device) {
+  return std::string("mtd-tune ") + mtd_version() + " | " + device_arch(device) + " | ";
+}
+
+static int tune_cache_load(mtd_ctx* ctx) {
+  std::FILE* f = std::fopen(ctx->tune_cache.c_str(), "r");
+  if (!f) return MTD_OK;  // no file yet: nothing tuned
+  const std::string pre = tune_prefix(ctx->device);
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    if (std::strncmp(line, pre.c_str(), pre.size()) != 0) continue;
+    int n, C, V, dt, ch, zw, kb, rt, ct;
+    float ms;
+    if (std::sscanf(line + pre.size(), "%d %d %d %d %d %d %d %d %d %f", &n, &C, &V, &dt, &ch,
+                    &zw, &kb, &rt, &ct, &ms) != 10)
+      continue;
+    const TunedSched t{ch, zw, kb, rt, ct, ms, true};
+    // an entry naming a variant this library does not build is ignored
+    if (ch < 0 || (zw && !gather_window_supported(zw)) || (kb && kb != 8 && kb != 16) ||
+        rt < 0 || rt > 1 || ct < 0 || ct > 2)
+      continue;
+    ctx->batch_tuned[std::make_tuple(n, C, V, dt)] = t;
+  }
+  std::fclose(f);
+  return MTD_OK;
+}
+
+static void tune_cache_append(mtd_ctx* ctx, const TuneKey& k, const TunedSched& t) {
+  if (ctx->tune_cache.empty()) return;
+  std::FILE* f = std::fopen(ctx->tune_cache.c_str(), "a");
+  if (!f) return;  // a read-only cache still serves what it holds
+  char line[512];
+  std::snprintf(line, sizeof line, "%s%d %d %d %d %d %d %d %d %d %.4f\n",
+                tune_prefix(ctx->device).c_str(), std::get<0>(k), std::get<1>(k),
+                std::get<2>(k), std::get<3>(k), t.chunk, t.zw, t.kb, t.rows_tile, t.cols_tile,
+                t.ms);
+  std::fputs(line, f);
+  std::fclose(f);
+}
+
 // one acquisition with the schedule picked for its shape
 static int batch_one(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                      const uint8_t* mask, int32_t* pos, float* re, float* im) {
-  const auto it = ctx->batch_tuned.find(std::make_tuple(g->n, 2 * g->nxmax, dtype));
-  const int chunk = ctx->holo_chunk_override, zw = ctx->gather_zw_override;
-  if (it != ctx->batch_tuned.end()) {
-    ctx->holo_chunk_override = it->second.first;
-    ctx->gather_zw_override = it->second.second;
-  }
+  const auto it = ctx->batch_tuned.find(tune_key(g, dtype));
+  const TunedSched user = sched_get(ctx);
+  if (it != ctx->batch_tuned.end()) sched_set(ctx, it->second);
   const int rc = mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im);
-  ctx->holo_chunk_override = chunk;
-  ctx->gather_zw_override = zw;
+  sched_set(ctx, user);
   return rc;
 }
 
-// Times the plain reconstruction of one acquisition under each candidate
-// schedule and keeps the fastest for its (N, C, dtype): first the hologram
-// chunk (holograms per row/column launch pair: the launch count against
-// the tail of each launch) with 16-high gather windows, then the gather's
-// window height (8, 16, 24: LDS per block, and so blocks per CU, against
-// the entries each window rescans) with that chunk. Each candidate runs
-// twice, the second run timed.
-static int tune_holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
-                           const uint8_t* mask, int32_t* pos, float* re, float* im) {
-  const auto key = std::make_tuple(g->n, 2 * g->nxmax, dtype);
+// Picks the plain reconstruction's schedule for one (N, C, V, dtype) by
+// timing candidates on that acquisition, one axis at a time, each axis
+// starting from the best so far: the hologram chunk (holograms per
+// row/column launch pair: the launch count against the tail of each
+// launch), the gather's height window (8, 16, 24: LDS per block, and so
+// blocks per CU, against the entries each window rescans) and its load
+// batch (8, 16 list entries in flight), then the volume passes' tile
+// variants (row pass: 1 or 1/2 the lines per block; column passes: 1, 1/2
+// or 2x the columns per block). Every candidate gives the same results. A
+// candidate runs once untimed and twice timed (the faster counts). The pick
+// is appended to the cache file when the context has one.
+static int tune_schedule(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
+                         const uint8_t* mask, int32_t* pos, float* re, float* im) {
+  const TuneKey key = tune_key(g, dtype);
   if (ctx->batch_tuned.count(key) || A <= 0) return MTD_OK;
+  // chunk candidates: the default (0: the device-stack budget, even
+  // chunks), smaller chunks, and the whole stack up to twice that budget
   const int cap = (int)std::max<size_t>(
-      1, (size_t)2 * kChunkBudget / ((size_t)g->n * 2 * g->nxmax * sizeof(float2)));
-  std::vector<int> cand;
-  for (int c : {16, 32, 64, 128, 256, 512})
-    if (c < A && c <= 4 * cap) cand.push_back(c);
-  cand.push_back(std::min(A, 4 * cap));
+      1, (size_t)2 * kChunkBudgetDevice / ((size_t)g->n * 2 * g->nxmax * sizeof(float2)));
+  std::vector<int> chunks{0};
+  for (int c : {32, 128})
+    if (c < A) chunks.push_back(c);
+  if (std::min(A, cap) > 128) chunks.push_back(std::min(A, cap));
   hipEvent_t e0, e1;
   MTD_TRY(hip_check(ctx, hipEventCreate(&e0), "tuner event"));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1733,36 +1836,54 @@ static int tune_holo_chunk(mtd_ctx* ctx, const mtd_geom* g, const void* holos, i
   }
   auto time_one = [&](float* ms) -> int {
     MTD_TRY(mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im));
-    MTD_TRY(hip_check(ctx, hipEventRecord(e0, ctx->stream), "tuner"));
-    MTD_TRY(mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im));
-    MTD_TRY(hip_check(ctx, hipEventRecord(e1, ctx->stream), "tuner"));
-    MTD_TRY(hip_check(ctx, hipEventSynchronize(e1), "tuner"));
-    return hip_check(ctx, hipEventElapsedTime(ms, e0, e1), "tuner");
+    *ms = 0.f;
+    for (int r = 0; r < 2; ++r) {
+      MTD_TRY(hip_check(ctx, hipEventRecord(e0, ctx->stream), "tuner"));
+      MTD_TRY(mtd_reconstruct_plain(ctx, g, holos, dtype, A, mask, pos, re, im));
+      MTD_TRY(hip_check(ctx, hipEventRecord(e1, ctx->stream), "tuner"));
+      MTD_TRY(hip_check(ctx, hipEventSynchronize(e1), "tuner"));
+      float t = 0.f;
+      MTD_TRY(hip_check(ctx, hipEventElapsedTime(&t, e0, e1), "tuner"));
+      if (r == 0 || t < *ms) *ms = t;
+    }
+    return MTD_OK;
   };
-  const int user_chunk = ctx->holo_chunk_override, user_zw = ctx->gather_zw_override;
-  ctx->gather_zw_override = 16;
-  int best = 0, best_zw = 16;
-  float best_ms = 0.f;
+  const TunedSched user = sched_get(ctx);
+  TunedSched best{0, 0, 0, 0, 0, 0.f, false};
   int rc = MTD_OK;
-  for (int c : cand) {
-    ctx->holo_chunk_override = c;
-    float ms = 0.f;
-    if ((rc = time_one(&ms)) != MTD_OK) break;
-    if (!best || ms < best_ms) { best = c; best_ms = ms; }
-  }
-  ctx->holo_chunk_override = best;
-  for (int zw : {8, 24}) {
-    if (rc != MTD_OK) break;
-    ctx->gather_zw_override = zw;
-    float ms = 0.f;
-    if ((rc = time_one(&ms)) != MTD_OK) break;
-    if (ms < best_ms) { best_zw = zw; best_ms = ms; }
-  }
-  ctx->holo_chunk_override = user_chunk;
-  ctx->gather_zw_override = user_zw;
+  // one axis: every candidate value v of field f, keeping the fastest
+  auto axis = [&](int TunedSched::*f, const std::vector<int>& vals) {
+    for (int v : vals) {
+      if (rc != MTD_OK) return;
+      if (best.ms > 0.f && best.*f == v) continue;  // already timed
+      TunedSched t = best;
+      t.*f = v;
+      sched_set(ctx, t);
+      float ms = 0.f;
+      if ((rc = time_one(&ms)) != MTD_OK) return;
+      if (best.ms <= 0.f || ms < best.ms) {
+        best = t;
+        best.ms = ms;
+      }
+    }
+  };
+  axis(&TunedSched::chunk, chunks);
+  axis(&TunedSched::zw, {0, 8, 24});  // 0: the default (16)
+  axis(&TunedSched::kb, {0, 16});     // 0: the default (8)
+  axis(&TunedSched::rows_tile, {0, 1});
+  axis(&TunedSched::cols_tile, {0, 1, 2});
+  sched_set(ctx, user);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc == MTD_OK) ctx->batch_tuned[key] = std::make_pair(best, best_zw);
+  if (rc == MTD_OK) {
+    // record the defaults the zero entries stand for (same launches)
+    if (best.chunk == 0)
+      best.chunk = default_holo_chunk(g->n, 2 * g->nxmax, A, is_host_ptr(holos));
+    if (best.zw == 0) best.zw = 16;
+    if (best.kb == 0) best.kb = 8;
+    ctx->batch_tuned[key] = best;
+    tune_cache_append(ctx, key, best);
+  }
   return rc;
 }
 
@@ -1833,8 +1954,8 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
   }
   if (flags & MTD_BATCH_AUTOTUNE)
     for (int i = 0; i < n; ++i)
-      MTD_TRY(tune_holo_chunk(ctx, &geoms[i], holos[i], dtypes[i], counts[i], mask_of(i),
-                              pos_of(i), index_re[i], index_im[i]));
+      MTD_TRY(tune_schedule(ctx, &geoms[i], holos[i], dtypes[i], counts[i], mask_of(i),
+                            pos_of(i), index_re[i], index_im[i]));
   // eager pass: the results of this call, and every table, plan and
   // workspace the graph will use is in place before the capture
   MTD_TRY(batch_stamp(ctx, 0));
@@ -1891,16 +2012,49 @@ int mtd_reconstruct_batch(mtd_ctx* ctx, int n, const mtd_geom* geoms, const void
   return MTD_OK;
 }
 
+// the tuned schedule of the first volume side tuned for (n, C, dtype)
+static const TunedSched* tuned_any_v(mtd_ctx* ctx, int n, int C, int dtype) {
+  for (const auto& kv : ctx->batch_tuned)
+    if (std::get<0>(kv.first) == n && std::get<1>(kv.first) == C &&
+        std::get<3>(kv.first) == dtype)
+      return &kv.second;
+  return nullptr;
+}
+
 int mtd_batch_tuned_chunk(mtd_ctx* ctx, int n, int C, int dtype) {
   if (!ctx) return -1;
-  const auto it = ctx->batch_tuned.find(std::make_tuple(n, C, dtype));
-  return it == ctx->batch_tuned.end() ? 0 : it->second.first;
+  const TunedSched* t = tuned_any_v(ctx, n, C, dtype);
+  return t ? t->chunk : 0;
 }
 
 int mtd_batch_tuned_window(mtd_ctx* ctx, int n, int C, int dtype) {
   if (!ctx) return -1;
-  const auto it = ctx->batch_tuned.find(std::make_tuple(n, C, dtype));
-  return it == ctx->batch_tuned.end() ? 0 : it->second.second;
+  const TunedSched* t = tuned_any_v(ctx, n, C, dtype);
+  return t ? t->zw : 0;
+}
+
+int mtd_batch_tuned_schedule(mtd_ctx* ctx, int n, int C, int V, int dtype, int* sched) {
+  if (!ctx || !sched) return MTD_ERR_ARG;
+  const auto it = ctx->batch_tuned.find(std::make_tuple(n, C, V, dtype));
+  if (it == ctx->batch_tuned.end()) {
+    for (int i = 0; i < 6; ++i) sched[i] = 0;
+    return 0;
+  }
+  const TunedSched& t = it->second;
+  sched[0] = t.chunk;
+  sched[1] = t.zw;
+  sched[2] = t.kb;
+  sched[3] = t.rows_tile;
+  sched[4] = t.cols_tile;
+  sched[5] = t.from_cache ? 1 : 0;
+  return 1;
+}
+
+int mtd_ctx_set_tune_cache(mtd_ctx* ctx, const char* path) {
+  if (!ctx) return MTD_ERR_ARG;
+  ctx->tune_cache = path ? path : "";
+  if (ctx->tune_cache.empty()) return MTD_OK;
+  return tune_cache_load(ctx);
 }
 
 }  // extern "C"

@@ -70,6 +70,7 @@ template <int L, int MAXT = 1024>
 constexpr int cols_threads() {
   return cols_nc<L, MAXT>() * Fft2<L, false>::T;
 }
+
 // An operation may cap the threads of its column blocks with a static
 // kColThreads: two half-size blocks per CU overlap one block's barriers with
 // the other's memory traffic, which pays where the stores are whole 128-byte
@@ -90,6 +91,33 @@ struct op_col_threads<Op, std::void_t<decltype(Op::kColThreads)>> {
 template <int L>
 constexpr int col_waves_per_simd() {
   return L >= 1024 ? 2 : L > 512 ? 3 : 4;
+}
+
+// Tile variants TV of the passes (a schedule the autotuner picks; every
+// variant does the same arithmetic in the same order, so results are
+// identical): TV 0 the shapes above; TV 1 half the lines (rows) or columns
+// per block, twice the blocks; TV 2 twice the columns per block where the
+// block still fits one CU at the pass's waves per SIMD, its LDS within
+// 64 KiB and the width within 32 and dividing L (else TV 0's width).
+template <int L, int TV>
+constexpr int rows_nt_tv() {
+  constexpr int nt = rows_nt<L>();
+  return TV == 1 && nt % 2 == 0 ? nt / 2 : nt;
+}
+template <int L, int MAXT, int TV>
+constexpr int cols_nc_tv() {
+  constexpr int nc = cols_nc<L, MAXT>();
+  constexpr int T = Fft2<L, false>::T;
+  constexpr int waves = col_waves_per_simd<L>();
+  if constexpr (TV == 1) {
+    return nc % 2 == 0 ? nc / 2 : nc;
+  } else if constexpr (TV == 2) {
+    constexpr int w = 2 * nc;
+    constexpr bool fits = w <= 32 && L % w == 0 && w * T <= 1024 && w * T <= 256 * waves &&
+                          w * Fft2<L, false>::LDS_COL * 4 <= kLdsBlockMax;
+    return fits ? w : nc;
+  }
+  return nc;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs in dispatch order (linear
@@ -127,10 +155,10 @@ struct has_line : std::false_type {};
 template <class Op>
 struct has_line<Op, std::void_t<decltype(std::declval<Op&>().line(0, 0))>> : std::true_type {};
 
-template <int L, bool INV, class Op>
+template <int L, bool INV, class Op, int TV = 0>
 __global__ void __launch_bounds__(256) k_rows(Op op) {
   using F = Fft2<L, INV>;
-  constexpr int NT = rows_nt<L>();
+  constexpr int NT = rows_nt_tv<L, TV>();
   __shared__ float lds[NT * F::LDS_ROW];
   const int2 ab = xcd_block();
   const int a = ab.x;
@@ -149,11 +177,12 @@ __global__ void __launch_bounds__(256) k_rows(Op op) {
   }
 }
 
-template <int L, bool INV, class Op>
-__global__ void __launch_bounds__((cols_threads<L, op_col_threads<Op>::value>()),
+template <int L, bool INV, class Op, int TV = 0>
+__global__ void __launch_bounds__((cols_nc_tv<L, op_col_threads<Op>::value, TV>() *
+                                   Fft2<L, false>::T),
                                   col_waves_per_simd<L>()) k_cols(Op op) {
   using F = Fft2<L, INV>;
-  constexpr int NC = cols_nc<L, op_col_threads<Op>::value>();
+  constexpr int NC = cols_nc_tv<L, op_col_threads<Op>::value, TV>();
   __shared__ float lds[NC * F::LDS_COL];
   const int2 ab = xcd_block();
   const int a = ab.x, c0 = ab.y * NC;
@@ -170,13 +199,30 @@ __global__ void __launch_bounds__((cols_threads<L, op_col_threads<Op>::value>())
   }
 }
 
+// An operation may define mid_line(batch, line), called once per transform
+// between the first transform and its mid() calls: per-line state that only
+// mid() reads is then not held in registers through the first transform.
+template <class Op, class = void>
+struct has_mid_line : std::false_type {};
+template <class Op>
+struct has_mid_line<Op, std::void_t<decltype(std::declval<Op&>().mid_line(0, 0))>>
+    : std::true_type {};
+
 // Two chained transforms per row: the INV1 transform of the loaded line,
 // op.mid(batch, line, element, value) on every output element, the values
 // moved back to the input layout through the exchange storage, the INV2
 // transform, op.store. The line set up by op.line(batch, line) is kept for
 // the whole kernel (one line per NT-th of the block).
+// waves per SIMD asked of the chained row pass: 4 (at most 128 VGPRs) for
+// 128- and 256-point lines, which then fit without spilling (other lengths
+// of that size would spill); the rest leave the register budget to the
+// compiler
+template <int L>
+constexpr int rows2_waves_per_simd() {
+  return (L == 128 || L == 256) ? 4 : 1;
+}
 template <int L, bool INV1, bool INV2, class Op>
-__global__ void __launch_bounds__(256) k_rows2(Op op) {
+__global__ void __launch_bounds__(256, rows2_waves_per_simd<L>()) k_rows2(Op op) {
   using F1 = Fft2<L, INV1>;
   using F2 = Fft2<L, INV2>;
   constexpr int NT = rows_nt<L>();
@@ -192,6 +238,7 @@ __global__ void __launch_bounds__(256) k_rows2(Op op) {
   for (int n1 = 0; n1 < F1::L1; ++n1) v[n1] = op.load(a, line, F1::L2 * n1 + n2);
   const typename F1::Row x{lds + r * F1::LDS_ROW};
   F1::run(v, u, n2, x, op.tw);
+  if constexpr (has_mid_line<Op>::value) op.mid_line(a, line);
   if (n2 < F1::L1) {
 #pragma unroll
     for (int k2 = 0; k2 < F1::L2; ++k2) u[k2] = op.mid(a, line, n2 + F1::L1 * k2, u[k2]);
@@ -204,21 +251,21 @@ __global__ void __launch_bounds__(256) k_rows2(Op op) {
   }
 }
 
-template <int L>
+template <int L, int TV = 0>
 inline dim3 rows_grid(int A) {
-  return dim3(L / rows_nt<L>(), A);
+  return dim3(L / rows_nt_tv<L, TV>(), A);
 }
-template <int L>
+template <int L, int TV = 0>
 inline dim3 rows_block() {
-  return dim3(rows_nt<L>() * Fft2<L, false>::T);
+  return dim3(rows_nt_tv<L, TV>() * Fft2<L, false>::T);
 }
-template <int L, class Op = void>
+template <int L, class Op = void, int TV = 0>
 inline dim3 cols_grid(int A) {
-  return dim3(L / cols_nc<L, op_col_threads<Op>::value>(), A);
+  return dim3(L / cols_nc_tv<L, op_col_threads<Op>::value, TV>(), A);
 }
-template <int L, class Op = void>
+template <int L, class Op = void, int TV = 0>
 inline dim3 cols_block() {
-  return dim3(cols_threads<L, op_col_threads<Op>::value>());
+  return dim3(cols_nc_tv<L, op_col_threads<Op>::value, TV>() * Fft2<L, false>::T);
 }
 
 }  // namespace mtd

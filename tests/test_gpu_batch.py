@@ -6,7 +6,7 @@ mtd_reconstruct_plain on the same inputs (which the parity tests hold against
 the oracle), after the eager first call, after each graph replay, and after
 new hologram contents are written into the captured buffers; the "small"
 member is also checked against the FP32 oracle (rel-L2 1e-5). The
-autotuned chunk changes launch granularity only, so its results are
+autotuned schedules change launch shapes only, so their results are
 bit-identical too."""
 import numpy as np
 import pytest
@@ -106,26 +106,78 @@ def test_batch_eager_graph_and_new_contents(gpu_ctx):
     _same(list(zip(re, im, pos)), want2)
 
 
-def test_batch_autotune_same_results(gpu_ctx):
+def test_batch_autotune_same_results(gpu_ctx, tmp_path):
+    """The autotuner's schedules (hologram chunk, gather window and batch,
+    volume tile variants) leave every bit unchanged; its picks go to the
+    cache file, and a new context reads them back instead of tuning."""
     import torch
-    _say("batched form: autotuned hologram chunks")
+    import mtd_transmission_amd as m
+    _say("batched form: autotuned schedules, persisted")
     acqs = [_acq(*s) for s in SHAPES[:3]]
     geoms = [a[0] for a in acqs]
     holos = [a[1] for a in acqs]
     masks = [a[2] for a in acqs]
     want = _plain(gpu_ctx, geoms, holos, masks)
-    re, im, pos = _outputs(geoms, holos)
-    gpu_ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
-    torch.cuda.synchronize()
-    _same(list(zip(re, im, pos)), want)
-    chunks = [gpu_ctx.batch_tuned_chunk(g.n, g.C) for g in geoms]
-    windows = [gpu_ctx.batch_tuned_window(g.n, g.C) for g in geoms]
-    _say(f"  tuned chunks {chunks}, gather windows {windows}")
-    assert all(c > 0 for c in chunks)
-    assert all(w in (8, 16, 24) for w in windows)
-    gpu_ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
-    torch.cuda.synchronize()
-    _same(list(zip(re, im, pos)), want)
+    cache = tmp_path / "tune.txt"
+    ctx = m.Context(0)
+    try:
+        ctx.set_tune_cache(str(cache))
+        re, im, pos = _outputs(geoms, holos)
+        ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
+        torch.cuda.synchronize()
+        _same(list(zip(re, im, pos)), want)
+        scheds = [ctx.batch_tuned_schedule(g.n, g.C, g.dim_final) for g in geoms]
+        _say(f"  tuned schedules {scheds}")
+        for sc, g in zip(scheds, geoms):
+            assert sc is not None and sc["from_cache"] == 0
+            assert sc["holo_chunk"] > 0 and sc["gather_window"] in (8, 16, 24)
+            assert sc["gather_batch"] in (8, 16) and sc["vol_rows_tile"] in (0, 1)
+            assert sc["vol_cols_tile"] in (0, 1, 2)
+            assert ctx.batch_tuned_chunk(g.n, g.C) == sc["holo_chunk"]
+            assert ctx.batch_tuned_window(g.n, g.C) == sc["gather_window"]
+        ctx.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=True, autotune=True)
+        torch.cuda.synchronize()
+        _same(list(zip(re, im, pos)), want)
+    finally:
+        ctx.close()
+    lines = cache.read_text().splitlines()
+    _say(f"  cache file: {len(lines)} entries, first: {lines[0] if lines else None}")
+    assert len(lines) == len({(g.n, g.C, g.dim_final) for g in geoms})
+    assert all(ln.startswith("mtd-tune ") for ln in lines)
+    ctx2 = m.Context(0)
+    try:
+        ctx2.set_tune_cache(str(cache))
+        for sc, g in zip(scheds, geoms):
+            got = ctx2.batch_tuned_schedule(g.n, g.C, g.dim_final)
+            assert got is not None and got["from_cache"] == 1
+            assert {k: v for k, v in got.items() if k != "from_cache"} == \
+                {k: v for k, v in sc.items() if k != "from_cache"}
+        re, im, pos = _outputs(geoms, holos)
+        ctx2.reconstruct_batch(geoms, holos, masks, re, im, pos, graph=False, autotune=True)
+        torch.cuda.synchronize()
+        _same(list(zip(re, im, pos)), want)
+        assert len(cache.read_text().splitlines()) == len(lines), "cached shapes were re-tuned"
+    finally:
+        ctx2.close()
+
+
+@pytest.mark.parametrize("shape", [(256, 32, 128, 9, 21), (512, 80, 320, 7, 22),
+                                   (768, 144, 576, 5, 23), (512, 96, 384, 5, 24)])
+def test_volume_tile_variants_same_bits(gpu_ctx, shape):
+    """Every tile variant of the volume passes gives the default's bits."""
+    import torch
+    g, h, mask = _acq(*shape)
+    _say(f"volume tile variants: n {g.n} C {g.C} V {g.dim_final}")
+    want = _plain(gpu_ctx, [g], [h], [mask])
+    try:
+        for rt, ct in ((1, 0), (0, 1), (0, 2), (1, 2)):
+            gpu_ctx.set_schedule_param("vol_rows_tile", rt)
+            gpu_ctx.set_schedule_param("vol_cols_tile", ct)
+            got = _plain(gpu_ctx, [g], [h], [mask])
+            _same(got, want)
+    finally:
+        gpu_ctx.set_schedule_param("vol_rows_tile", 0)
+        gpu_ctx.set_schedule_param("vol_cols_tile", 0)
 
 
 def test_batch_graph_refuses_host_stacks(gpu_ctx):
