@@ -309,8 +309,8 @@ hipError_t launch_angle_geo(const int* posx, const int* posy, int A, int nxmax,
 hipError_t launch_band(const AngleGeo* ageo, int A, int V, const double* htab, int r2max,
                        int2* band, hipStream_t s);
 // band == nullptr: every voxel of the volume (and counts) is written;
-// otherwise only the height windows that meet the band are, and [zlo, zhi]
-// is a host-side bound of the band (only windows meeting it are launched).
+// otherwise only the planes of the band are, and [zlo, zhi] is a host-side
+// bound of the band (only height windows meeting it are launched).
 // order: the tile dispatch order (gather_tile_order); zw: height window
 // (8, 16, 24; others fall back to 16); kb: list entries per load batch (8,
 // 16). The crop side must keep one spectrum below 2 GiB.

@@ -209,26 +209,26 @@ struct GBatch {
     // scheduler would otherwise sink them below those independent LDS ops
     __builtin_amdgcn_sched_barrier(0);
   }
-  // Unconditional updates (a conditional one lets the compiler sink its
-  // loads behind the branch): a miss adds -0.0 to slot 0, which leaves every
-  // value bit-identical (x + -0.0 == x, also for x = +-0.0), and does not
-  // count.
+  // Only hits update their LDS slot (a lane-masked read-modify-write: the
+  // misses of a wave move no LDS data); the loads were issued before the
+  // scheduling barrier in issue(), so the branch cannot sink them.
   template <bool PRE, int ZW>
   __device__ __forceinline__ void apply(float2 (&acc)[ZW][256], unsigned short (&cn)[ZW][256],
                                         int tid) const {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int t = 0; t < KB; ++t) {
-      const bool hit = kv[t] >= 0;
-      const int kc = hit ? kv[t] : 0;
-      // value (-b Si, b Sr)
-      const float pim = PRE ? sp[t].y : b[t] * sp[t].y;
-      const float pre = PRE ? sp[t].x : b[t] * sp[t].x;
-      float2 s = acc[kc][tid];
-      s.x = s.x + (hit ? -pim : -0.0f);
-      s.y = s.y + (hit ? pre : -0.0f);
-      acc[kc][tid] = s;
-      cn[kc][tid] = (unsigned short)(cn[kc][tid] + (hit ? 1 : 0));
+      if (kv[t] >= 0) {
+        const int kc = kv[t];
+        // value (-b Si, b Sr)
+        const float pim = PRE ? sp[t].y : b[t] * sp[t].y;
+        const float pre = PRE ? sp[t].x : b[t] * sp[t].x;
+        float2 s = acc[kc][tid];
+        s.x = s.x + -pim;
+        s.y = s.y + pre;
+        acc[kc][tid] = s;
+        cn[kc][tid] = (unsigned short)(cn[kc][tid] + 1);
+      }
     }
   }
 };
@@ -311,15 +311,20 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items, const int* __re
   // ballots are per wave) but write nothing
   const bool inb = X < g.V && Y < g.V;
   const int z0 = w * ZW, z1 = min(z0 + ZW, g.V) - 1;
-  if (band) {  // banded output: windows outside the hit band are not written
+  // heights this block writes: the window, cut to the hit band when the
+  // output is banded (no pass reads a plane outside the band)
+  int zs0 = z0, zs1 = z1;
+  if (band) {
     const int2 b = *band;
     if (z1 < b.x || z0 > b.y) return;
+    zs0 = max(z0, b.x);
+    zs1 = min(z1, b.y);
   }
   const int4 info = tinfo[tile];
   const bool norm = flags & 1, sgn = flags & 5;
   if (info.x == 0 || z1 < info.y || z0 > info.z) {
     if (inb) {
-      for (int z = z0; z <= z1; ++z) {
+      for (int z = zs0; z <= zs1; ++z) {
         g.vol[z * plane + col] = make_float2(0.f, 0.f);
         if (!sgn) g.cnt[z * plane + col] = 0;
       }
@@ -395,7 +400,8 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items, const int* __re
 #pragma unroll
   for (int k = 0; k < ZW; ++k) {
     const int z = z0 + k;
-    if (z > z1) break;
+    if (z > zs1) break;
+    if (z < zs0) continue;
     const float2 s = acc[k][tid];
     const int c = cn[k][tid];
     if (sgn) {
