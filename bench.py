@@ -93,7 +93,7 @@ def kernel_bytes(name, n, C, V, A, P, M, B, b_in=2):
         "holo_rows": A * b_in * n * n,
         "holo_cols": A * 8 * C * C,
         "demod_rows": A * 8 * C * C,
-        "fwd_rows_field_q": A * 8 * C * C,
+        "fwd_rows_field": A * 8 * C * C,
         "fwd_cols": A * 8 * C * C,
         "ewald_gather": A * 8 * P + 8 * B * V * V,
         "vol_x": 8 * B * V * V,
@@ -540,6 +540,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the PCIe-inclusive leg (host stacks upload in smaller "
+                         "chunks; profiles averaging per-dispatch counters skip it)")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (the reference CPU path over every hologram)")
     ap.add_argument("--cpu-child", nargs=3, help=argparse.SUPPRESS)
@@ -671,23 +674,25 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     print(f"[bench] {args.steps} steps, {ms:.3f} ms/volume", file=sys.stderr, flush=True)
 
-    # PCIe-inclusive rate: the holograms start in pinned host memory and the
-    # library streams them to HBM inside each timed step, each chunk's copy
-    # overlapped with the previous chunk's transform (the executables' case)
-    host = torch.empty(holos.shape, dtype=holos.dtype, pin_memory=True)
-    host.copy_(holos)
-    ctx.reconstruct_plain(g, host, mask, re, im, pos)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    pcie = None
+    if not args.no_pcie:
+        # PCIe-inclusive rate: the holograms start in pinned host memory and the
+        # library streams them to HBM inside each timed step, each chunk's copy
+        # overlapped with the previous chunk's transform (the executables' case)
+        host = torch.empty(holos.shape, dtype=holos.dtype, pin_memory=True)
+        host.copy_(holos)
         ctx.reconstruct_plain(g, host, mask, re, im, pos)
-    torch.cuda.synchronize()
-    ms_pcie = (time.perf_counter() - t0) * 1e3 / args.steps
-    del host
-    pcie = {"value": round(A / (ms_pcie / 1e3), 2), "unit": "holograms/s",
-            "ms_per_step": round(ms_pcie, 4),
-            "note": "pinned host holograms streamed to HBM by the library inside the step "
-                    "(copies overlapped with the hologram stage)"}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.reconstruct_plain(g, host, mask, re, im, pos)
+        torch.cuda.synchronize()
+        ms_pcie = (time.perf_counter() - t0) * 1e3 / args.steps
+        del host
+        pcie = {"value": round(A / (ms_pcie / 1e3), 2), "unit": "holograms/s",
+                "ms_per_step": round(ms_pcie, 4),
+                "note": "pinned host holograms streamed to HBM by the library inside the step "
+                        "(copies overlapped with the hologram stage)"}
 
     # roofline of the dominant kernel, from the timed region's events
     launches, tot = kt[dom]
