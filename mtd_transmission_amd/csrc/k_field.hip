@@ -273,9 +273,11 @@ __device__ __forceinline__ float2 form_from(float2 u, float phf, double bgp, dou
 //   aa e^{i pc} - 1 = (aa/|u|) u e^{-i bgp} - 1
 // gives the same field without the arctangent (an FP32 rounding apart:
 // rel-L2 about 2e-7 against form_from on a 512^3 reconstruction); u = 0 keeps
-// the reference's -1 (atan2f(0, 0) = 0, aa = 0).
+// the reference's -1 (atan2f(0, 0) = 0, aa = 0). |u| by the hardware square
+// root of x^2 + y^2 (within about 2 ulp of hypotf; field values here stay far
+// from the float range's ends, and |u| cancels in aa / |u| unless clamped).
 __device__ __forceinline__ float2 form_born_wrapped(float2 u, double bgp, double bga) {
-  const float amf = hypotf(u.x, u.y);
+  const float amf = __builtin_amdgcn_sqrtf(u.x * u.x + u.y * u.y);
   double A = (double)amf / (bga + 0.00000001);
   if (A > 10) A = 1;
   if (A < -10) A = 1;

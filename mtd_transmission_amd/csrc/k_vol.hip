@@ -97,15 +97,16 @@ __device__ __forceinline__ double height(double R2, int r2) { return sqrt(R2 - (
 
 // Height offset round(h(r2) - fz) of a hit (before + V/2), as the reference
 // computes it: FP64, round half away from zero. First in FP32: with R^2 <
-// 2^24, R^2 - r2 is an exact float, sqrtf is correctly rounded, fz32 is fz
+// 2^24, R^2 - r2 is an exact float; the hardware square root (v_sqrt_f32,
+// within 1 ulp, not correctly rounded) of it is within 1 ulp(R), fz32 is fz
 // rounded once and the difference is rounded once, each within half an ulp
 // of R (h, fz and |h - fz| are at most R): the FP32 difference is within
-// 1.5 ulp(R) of the FP64 one. Wherever the fractional part of |difference|
-// is farther than thr = 4 ulp(R) from 1/2 both round alike; near 1/2
-// (*near is set) the caller takes the FP64 path.
+// 2 ulp(R) of the FP64 one. Wherever the fractional part of |difference| is
+// farther than thr = 8 ulp(R) from 1/2 both round alike; near 1/2 (*near is
+// set) the caller takes the FP64 path.
 __device__ __forceinline__ int height_off_f32(float R2f, int r2, float fz32, float thr,
                                               bool* near) {
-  const float d = sqrtf(R2f - (float)r2) - fz32;
+  const float d = __builtin_amdgcn_sqrtf(R2f - (float)r2) - fz32;
   const float ad = fabsf(d);
   const float fl = floorf(ad);
   const float fr = ad - fl;  // exact (Sterbenz)
@@ -346,9 +347,9 @@ k_gather_win(GatherArgs g, int ntx, int w0, int nwin, int items, const int* __re
   const size_t C2 = (size_t)g.C * g.C;
   const double R2 = g.htab[0] * g.htab[0];
   const float R2f = (float)R2;  // an integer: exact below 2^24
-  // the FP32 height path's margin, 4 ulp(R); everything takes the FP64 path
+  // the FP32 height path's margin, 8 ulp(R); everything takes the FP64 path
   // where R^2 - r2 would not be exact in FP32
-  const float thr = R2 < 16777216.0 ? ldexpf(1.f, ilogbf((float)g.htab[0]) - 21) : 2.f;
+  const float thr = R2 < 16777216.0 ? ldexpf(1.f, ilogbf((float)g.htab[0]) - 20) : 2.f;
   const int zbase = z0 - g.V / 2;
   // The tile's list in segments of kGatherSeg entries: each thread tests one
   // entry against the window, the overlapping ones are compacted into LDS in
