@@ -190,6 +190,15 @@ int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops,
                       int A, const uint8_t* mask, int32_t* posspec,
                       float* fields);
 
+/* calc_Uborn2 (pretraitement_hors_axe_CPU/fonctions.cpp:348-374): the
+ * demodulated field U of each crop (circular shift by -posSpec, unnormalised
+ * backward 2-D FFT, shift by (NXMAX, NXMAX)), u[A][C][C] complex64 in the
+ * reference's frame; posspec as mtd_crop_to_field writes it (x then y). The
+ * ArrayFire variant saves its phase atan2(U) (phasePI_atan2.raw,
+ * pretraitement_hors_axe_GPU/main.cpp:212-214). Device arrays. */
+int mtd_demodulate(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A,
+                   const int32_t* posspec, float* u);
+
 /* Both preprocessing loops (main.cpp:130-342) from holograms. */
 int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos,
                    int dtype, int A, const uint8_t* mask, int32_t* posspec,

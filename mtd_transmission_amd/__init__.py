@@ -89,6 +89,7 @@ SIGNATURES = {
     "mtd_batch_tuned_window": (_I, [_P, _I, _I, _I]),
     "mtd_batch_tuned_schedule": (_I, [_P, _I, _I, _I, _I, ctypes.POINTER(_I)]),
     "mtd_ctx_set_tune_cache": (_I, [_P, ctypes.c_char_p]),
+    "mtd_demodulate": (_I, [_P, _P, _P, _I, _P, _P]),
     "mtd_forward_spectra": (_I, [_P, _GP, _P, _P, _P, _I, _P]),
     "mtd_adjoint_spectra": (_I, [_P, _GP, _P, _P, _I, _P, _P]),
     "mtd_cap_weights": (_I, [_P, _GP, _P, _I, _P]),
@@ -372,6 +373,16 @@ class Context:
         self._check(lib().mtd_crop_to_field(self.h, ctypes.byref(g), _dev(crops, "c32", n, "crops"),
                                             A, mp, _dev(posspec, "i32", 2 * A, "posspec"),
                                             _dev(fields, "c32", n, "fields")), "crop_to_field")
+
+    def demodulate(self, g: Geom, crops, posspec, u):
+        """calc_Uborn2: the demodulated field of each crop for its specular
+        (U frame), complex64 [A][C][C]."""
+        self._stream()
+        A = crops.shape[0]
+        n = A * g.C * g.C
+        self._check(lib().mtd_demodulate(self.h, ctypes.byref(g), _dev(crops, "c32", n, "crops"),
+                                         A, _dev(posspec, "i32", 2 * A, "posspec"),
+                                         _dev(u, "c32", n, "u")), "demodulate")
 
     def preprocess(self, g: Geom, holos, mask, posspec, fields):
         self._stream()

@@ -252,6 +252,9 @@ size_t argmax_scratch_bytes(int A);
 hipError_t launch_demod(int C, const float2* crop, const int* posx,
                         const int* posy, int A, const FftTab& ft,
                         float2* work, float2* iu, hipStream_t s);
+// the demodulated fields moved from the IU frame to the U frame (a shift by
+// C/2 in both axes), A crops of C x C
+hipError_t launch_iu_to_u(int C, const float2* iu, int A, float2* u, hipStream_t s);
 // The simulator's field synthesis: shift by -posSpec, unnormalised inverse
 // FFT, shift by C/2, times scale (delta_f^2): fields[A][C][C].
 hipError_t launch_spectra_to_fields(int C, const float2* spec, const int* posx, const int* posy,

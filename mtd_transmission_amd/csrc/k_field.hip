@@ -654,6 +654,23 @@ hipError_t launch_demod(int C, const float2* crop, const int* posx,
   return demod_c<0>(Dim<0>(C), crop, posx, posy, A, ft, work, iu, s);
 }
 
+// IU frame to the U frame of calc_Uborn2's output: U[y][x] =
+// IU[(y + C/2) % C][(x + C/2) % C]
+__global__ void __launch_bounds__(256)
+k_iu_to_u(const float2* __restrict__ iu, int C, float2* __restrict__ u) {
+  const int a = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= C * C) return;
+  const int x = i % C, y = i / C, h = C / 2;
+  u[(size_t)a * C * C + i] = iu[((size_t)a * C + (y + h) % C) * C + (x + h) % C];
+}
+
+hipError_t launch_iu_to_u(int C, const float2* iu, int A, float2* u, hipStream_t s) {
+  if (A <= 0) return hipSuccess;
+  Timed tm("iu_to_u", s);
+  hipLaunchKernelGGL(k_iu_to_u, dim3((C * C + 255) / 256, A), dim3(256), 0, s, iu, C, u);
+  return hipGetLastError();
+}
+
 hipError_t launch_lsq(int C, const float2* iu, const float* phu, int A, const LsqPlan& plan,
                       double* coef, hipStream_t s) {
   switch (C) {

@@ -1143,6 +1143,32 @@ int mtd_crop_to_field(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A
   return MTD_OK;
 }
 
+int mtd_demodulate(mtd_ctx* ctx, const mtd_geom* g, const float* crops, int A,
+                   const int32_t* posspec, float* u) {
+  CtxScope scope(ctx);
+  MTD_TRY(validate(ctx, g, A, 0));
+  if (A == 0) return MTD_OK;
+  if (!crops || !posspec || !u) return set_err(ctx, MTD_ERR_ARG, "null array");
+  const int C = 2 * g->nxmax;
+  FftTab twC;
+  const bool tabs = fft_tab(ctx, C, crop_len_supported(C), &twC);
+  const int chunk = chunk_of(2 * (size_t)C * C * sizeof(float2), kFieldBudget, A);
+  float2* work = (float2*)ws_get(ctx, WS_WORK, (size_t)chunk * C * C * sizeof(float2));
+  float2* iu = (float2*)ws_get(ctx, WS_IU, (size_t)chunk * C * C * sizeof(float2));
+  if (!tabs || !work || !iu) return set_err(ctx, MTD_ERR_HIP, "out of device memory (demodulation)");
+  for (int a0 = 0; a0 < A; a0 += chunk) {
+    const int n = std::min(chunk, A - a0);
+    MTD_TRY(hip_check(ctx, launch_demod(C, (const float2*)crops + (size_t)a0 * C * C,
+                                        posspec + a0, posspec + A + a0, n, twC, work, iu,
+                                        ctx->stream),
+                      "demodulation"));
+    MTD_TRY(hip_check(ctx, launch_iu_to_u(C, iu, n, (float2*)u + (size_t)a0 * C * C,
+                                          ctx->stream),
+                      "demodulation frame"));
+  }
+  return MTD_OK;
+}
+
 int mtd_preprocess(mtd_ctx* ctx, const mtd_geom* g, const void* holos, int dtype, int A,
                    const uint8_t* mask, int32_t* posspec, float* fields) {
   CtxScope scope(ctx);

@@ -264,7 +264,11 @@ def test_shipped_configurations(tmp_path, name, exe):
     seeded synthetic holograms: speculars bit-exact; the index volume (at 400
     sampled voxels of the oracle's exact sum) within rel-L2 1e-5 of the FP32
     oracle; the Rytov fields within 1e-5 of the FP32 oracle wherever the
-    phase is defined. The phase, the Rytov field's imaginary part, is
+    phase is defined. Under the ArrayFire variant's name the oracle runs
+    with that variant's windows (alpha 0.025, the (1+i) crop window), and its
+    two extra outputs are checked: phasePI_atan2.raw against atan2 of the
+    oracle's demodulated fields (wherever the phase is defined), and
+    UBornAmpFinal.raw against exp(Re) of the last field (Rytov). The phase, the Rytov field's imaginary part, is
     ill-conditioned where the demodulated field nearly vanishes (the phase
     singularities of a strongly scattering object): an error du moves it by
     du/|u|. On these inputs over 99% of the squared difference from the FP64
@@ -309,7 +313,11 @@ def test_shipped_configurations(tmp_path, name, exe):
     r = subprocess.run([os.path.join(m.BIN_DIR, exe)], env=env, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    f32, p32 = ref.preprocess(holos, go, mask, "f32")
+    variant = exe == "tomo_pretraitement_GPU"
+    if variant:  # pretraitement_hors_axe_GPU/main.cpp:55-60
+        import dataclasses
+        go = dataclasses.replace(go, alpha_holo=0.025, alpha_crop=0.025)
+    f32, p32, c32 = ref.preprocess(holos, go, mask, "f32", return_crops=True)
     assert np.array_equal(np.fromfile(res / "tab_posSpec.raw", "<f8"), p32.astype(float))
     dim = f"{C}x{C}x{A}"
     got = (np.fromfile(res / f"UBornfinal_Re{dim}.raw", "<f8")
@@ -321,6 +329,17 @@ def test_shipped_configurations(tmp_path, name, exe):
     defined = au >= 0.01 * np.median(au)
     ef = rel_l2(got[defined], f32[defined])
     print(f"  phase defined at {defined.mean():.4f} of the pixels", flush=True)
+    if variant:
+        u32 = np.stack([ref.demod(c32[a], int(p32[a]), int(p32[a + A]), "f32")
+                        for a in range(A)])
+        ph = np.fromfile(res / "phasePI_atan2.raw", "<f4").reshape(A, C, C)
+        dph = np.angle(np.exp(1j * (ph - np.angle(u32))))  # difference mod 2 pi
+        eph = float(np.abs(dph[defined]).max())
+        amp = np.fromfile(res / "UBornAmpFinal.raw", "<f4").reshape(C, C)
+        ea = rel_l2(amp, np.exp(f32[-1].real))
+        print(f"  phasePI_atan2 max |diff| {eph:.2e} rad; UBornAmpFinal rel-L2 {ea:.2e}",
+              flush=True)
+        assert eph < 1e-4 and ea < 1e-5
     r = subprocess.run([REC], env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     ire = read_tiff_f32(res / "indice.tif")
@@ -331,13 +350,14 @@ def test_shipped_configurations(tmp_path, name, exe):
     ei = rel_l2(ire.ravel()[idx] + 1j * iim.ravel()[idx], want)
     print(f"  fields rel-L2 vs f32 {ef:.3e}; index at 400 voxels {ei:.3e}", flush=True)
     assert ef < 1e-5 and ei < 1e-5
-    if exe != "tomo_pretraitement":  # the GPU variant's name runs the same program
+    if variant:  # the CPU name with -arrayfire gives the same fields
         res2 = tmp_path / "result_cpu_name"
         res2.mkdir()
         (home / ".config" / "gui_tomo.conf").write_text(
             f"CHEMIN_RESULT {res2}\nCHEMIN_ACQUIS {acq}\nCHEMIN_CONFIG {cfg}\n"
             f"CHEMIN_CONFIG_PC_ACQUIS {cfg}\n")
-        r = subprocess.run([PRE], env=env, capture_output=True, text=True, timeout=300)
+        r = subprocess.run([PRE, "-arrayfire"], env=env, capture_output=True, text=True,
+                           timeout=300)
         assert r.returncode == 0, r.stdout + r.stderr
         for f in (f"UBornfinal_Re{dim}.raw", f"UBornfinal_Im{dim}.raw", "tab_posSpec.raw",
                   "parametres.raw"):

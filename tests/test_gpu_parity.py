@@ -568,3 +568,32 @@ def test_timing_only_one_kernel(gpu_ctx):
     finally:
         gpu_ctx.set_timing(False)
         gpu_ctx.timing_reset()
+
+
+@pytest.mark.parametrize("name", ["small", "cfg1"])
+def test_demodulate(gpu_ctx, name):
+    """mtd_demodulate (calc_Uborn2) on the library's own crops and speculars:
+    within rel-L2 1e-5 of the FP32 oracle's demodulation of the oracle's
+    crops, and within the FP64 bound of the FP64 oracle's."""
+    import torch
+    case = make_case(name)
+    _say(f"demodulate case={name}")
+    g = case.lib_geom()
+    A, C = case.acq.holos.shape[0], g.C
+    crops = _empty((A, C, C), torch.complex64)
+    gpu_ctx.holo_to_crop(g, _t(case.acq.holos), crops)
+    pos = _empty((2 * A,), torch.int32)
+    fields = _empty((A, C, C), torch.complex64)
+    gpu_ctx.preprocess(g, _t(case.acq.holos), case.acq.mask, pos, fields)
+    u = _empty((A, C, C), torch.complex64)
+    gpu_ctx.demodulate(g, crops, pos, u)
+    torch.cuda.synchronize()
+    _, p32, c32 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f32", return_crops=True)
+    _, p64, c64 = ref.preprocess(case.acq.holos, case.g, case.acq.mask, "f64", return_crops=True)
+    assert np.array_equal(pos.cpu().numpy(), p32)
+    u32 = np.stack([ref.demod(c32[a], int(p32[a]), int(p32[a + A]), "f32") for a in range(A)])
+    u64 = np.stack([ref.demod(c64[a], int(p64[a]), int(p64[a + A]), "f64") for a in range(A)])
+    got = u.cpu().numpy()
+    e32, e64 = rel_l2(got, u32), rel_l2(got, u64)
+    _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert e32 < TOL and e64 <= fp64_bound(u32, u64)
