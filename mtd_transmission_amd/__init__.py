@@ -320,7 +320,8 @@ class Context:
 
     def set_schedule_param(self, name, value):
         """One named schedule parameter ("holo_chunk", "gather_window",
-        "gather_batch"; 0: its default); results do not depend on it."""
+        "gather_batch", "vol_rows_tile", "vol_cols_tile"; 0: its default);
+        results do not depend on it."""
         self._check(lib().mtd_ctx_set_schedule_param(self.h, name.encode(), int(value)),
                     "set_schedule_param")
 

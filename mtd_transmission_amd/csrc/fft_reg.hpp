@@ -193,14 +193,16 @@ struct Fft2 {
   // hold u[k2] = X[n2 + L1*k2]. x: this transform's exchange storage. The
   // transpose moves the real parts, then the imaginary parts, through a
   // float buffer (half the LDS of a complex one). Contains three barriers:
-  // every thread of the workgroup must call it.
-  template <class X>
+  // every thread of the workgroup must call it. tw: the L-point twiddle
+  // table, or with TS > 1 the (TS*L)-point one (entry TS*j is entry j of the
+  // L-point table).
+  template <int TS = 1, class X>
   __device__ __forceinline__ static void run(float2 (&v)[L1], float2 (&u)[L2], int n2, X x,
                                              const float2* __restrict__ tw) {
     if constexpr (L1 > 1) RegFFT<L1, INV>::run(v);
 #pragma unroll
     for (int k1 = 1; k1 < L1; ++k1) {
-      float2 w = tw[n2 * k1];
+      float2 w = tw[TS * n2 * k1];
       if (INV) w.y = -w.y;
       v[k1] = cmul(v[k1], w);
     }

@@ -7,7 +7,8 @@
 // become sign flips applied on load and on store.
 //
 // Pass 1 (rows): two real rows are packed into one complex FFT of length N
-// (two-step register FFT); only the C bins kx in [x0, x0+C) are separated and
+// (two-step register FFT; from N = 2048, each row's even and odd samples into
+// one of length N/2); only the C bins kx in [x0, x0+C) are separated and
 // written, to t1 (N x C per hologram) in column tiles of W bins (W = 32 when
 // it divides C): t1[a][d / W][y][d % W]. A row pass stores 256-byte runs, and
 // the blocks of the column pass that share a tile read it as one contiguous
@@ -44,11 +45,11 @@ constexpr int holo_pairs() {  // row pairs per block: <= 256 threads, <= 64 KiB
   return divisor_le(N / 2, by_lds < cap ? (by_lds > 0 ? by_lds : 1) : cap);
 }
 
-// waves per SIMD the row kernel is compiled for (blocks of at most 256
-// threads; four such blocks fit in the LDS of a CU)
+// waves per SIMD the two-row kernel is compiled for (blocks of at most 256
+// threads; four such blocks fit in the LDS of a CU; sides up to 1024)
 template <int N>
 constexpr int holo_rows_waves() {
-  return N == 768 ? 3 : N <= 1024 ? 4 : 2;
+  return N == 768 ? 3 : 4;
 }
 
 template <int N, typename TIn>
@@ -134,6 +135,114 @@ k_holo_rows(const TIn* __restrict__ holos, const float* __restrict__ ws,
         } else {
           o0[e + 1] = zk - zm;
           o1[e] = zk + zm;
+        }
+      }
+    }
+  }
+}
+
+// Hologram sides from 2048: each real row is one complex transform of half
+// the length, z[n] = x[2n] + i x[2n+1], n < H = N/2. At N = 2048 that is the
+// balanced 32 x 32 register FFT (the two-row packing needs the 32 x 64
+// split: 64 values per thread, half the threads idle in its second step, two
+// waves per SIMD). With Z = FFT_H(z) and w = exp(-2 pi i / N), the row's bins
+// follow from the even and odd samples' spectra:
+//   X[k] = A[k] + w^k B[k],  A = (Z[k] + conj Z[m]) / 2,
+//   B = (Z[k] - conj Z[m]) / 2i,  m = H - k,  Z's indices mod H.
+// With E = Z[k] + conj Z[m] and D = Z[k] - conj Z[m], the stored value is
+//   2 X[k] = (E.x + w.x D.y + w.y D.x, E.y - w.x D.x + w.y D.y)
+// (2 X: as in the two-row pass, the 1/2 is in the column pass's scale).
+template <int N>
+constexpr bool holo_rows_half() {
+  return N >= 2048;
+}
+
+__device__ __forceinline__ float2 load_pair(const uint8_t* p) {
+  const uchar2 q = *(const uchar2*)p;
+  return make_float2((float)q.x, (float)q.y);
+}
+__device__ __forceinline__ float2 load_pair(const uint16_t* p) {
+  const ushort2 q = *(const ushort2*)p;
+  return make_float2((float)q.x, (float)q.y);
+}
+__device__ __forceinline__ float2 load_pair(const float* p) { return *(const float2*)p; }
+
+template <int N, typename TIn>
+__global__ void __launch_bounds__(256, 4)
+k_holo_rows_half(const TIn* __restrict__ holos, const float* __restrict__ ws,
+                 const float2* __restrict__ tw, int x0, int C, int cp, int lw,
+                 float2* __restrict__ t1) {
+  constexpr int H = N / 2;
+  using F = Fft2<H, false>;
+  constexpr int NR = 256 / F::T;  // rows per block
+  static_assert(F::LDS_ROW >= H, "exchange storage must hold a full line");
+  __shared__ float lds[NR * F::LDS_ROW];
+  const int2 ab = xcd_block();
+  const int a = ab.x;
+  const int r = threadIdx.x / F::T, n2 = threadIdx.x % F::T;
+  const int y = ab.y * NR + r;
+  const TIn* h = holos + ((size_t)a * N + y) * N;
+  // ws holds (-1)^i (float)tuk(i): window and centring sign in one product
+  const float wy = ws[y];
+  float2 v[F::L1], u[F::L2];
+#pragma unroll
+  for (int n1 = 0; n1 < F::L1; ++n1) {
+    const int n = F::L2 * n1 + n2;
+    const float2 s = load_pair(h + 2 * n);
+    const float2 wx = *(const float2*)(ws + 2 * n);
+    v[n1] = make_float2(s.x * (wx.x * wy), s.y * (wx.y * wy));
+  }
+  float* buf = lds + r * F::LDS_ROW;
+  // tw is the N-point table: stride 2 gives the H-point one
+  F::template run<2>(v, u, n2, typename F::Row{buf}, tw);
+  // Z goes through the float storage twice, real parts (E.x, D.x) then
+  // imaginary parts (E.y, D.y); crops up to N/4 keep the first pass's values
+  // in registers, wider ones stage them in their own output slots.
+  const int W = 1 << lw;
+  float2* q = t1 + (size_t)a * N * cp + (size_t)y * W;
+  auto tile = [lw, W](int d) { return (((d >> lw) * N) << lw) + (d & (W - 1)); };
+  constexpr int DK = (N / 4 + F::T - 1) / F::T;
+  float ex[DK], dx[DK];
+  const bool keep = C <= N / 4;  // uniform
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    __syncthreads();  // the storage is free
+    if (n2 < F::L1) {
+#pragma unroll
+      for (int k2 = 0; k2 < F::L2; ++k2) buf[n2 + F::L1 * k2] = part ? u[k2].y : u[k2].x;
+    }
+    __syncthreads();
+    if (keep) {
+#pragma unroll
+      for (int j = 0; j < DK; ++j) {
+        const int d = n2 + F::T * j;
+        if (d < C) {
+          const int k = x0 + d;
+          const int kk = k & (H - 1);
+          const float zk = buf[kk], zm = buf[(H - kk) & (H - 1)];
+          if (part == 0) {
+            ex[j] = zk + zm;
+            dx[j] = zk - zm;
+          } else {
+            const float2 w = tw[k];
+            const float ey = zk - zm, dy = zk + zm;
+            q[tile(d)] = make_float2(ex[j] + w.x * dy + w.y * dx[j], ey - w.x * dx[j] + w.y * dy);
+          }
+        }
+      }
+    } else {
+      for (int d = n2; d < C; d += F::T) {
+        const int k = x0 + d;
+        const int kk = k & (H - 1);
+        const float zk = buf[kk], zm = buf[(H - kk) & (H - 1)];
+        float2* o = q + tile(d);
+        if (part == 0) {
+          *o = make_float2(zk + zm, zk - zm);
+        } else {
+          const float2 e = *o;  // this thread's own store of the first pass
+          const float2 w = tw[k];
+          const float ey = zk - zm, dy = zk + zm;
+          *o = make_float2(e.x + w.x * dy + w.y * e.y, ey - w.x * e.y + w.y * dy);
         }
       }
     }
@@ -273,29 +382,38 @@ static int t1_cols_n(int C) {
   return (C + W - 1) / W * W;
 }
 
+template <int N, typename TIn>
+static void rows_launch(const void* holos, int A, const float* win, const float2* tw, int x0,
+                        int C, int cp, int lw, float2* t1, hipStream_t s) {
+  if constexpr (holo_rows_half<N>()) {
+    constexpr int NR = 256 / Fft2<N / 2, false>::T;
+    hipLaunchKernelGGL((k_holo_rows_half<N, TIn>), dim3(N / NR, A), dim3(NR * Fft2<N / 2, false>::T),
+                       0, s, (const TIn*)holos, win, tw, x0, C, cp, lw, t1);
+  } else {
+    constexpr int NP = holo_pairs<N>();
+    hipLaunchKernelGGL((k_holo_rows<N, TIn>), dim3(N / (2 * NP), A), dim3(NP * Fft2<N, false>::T),
+                       0, s, (const TIn*)holos, win, tw, x0, C, cp, lw, t1);
+  }
+}
+
 template <int N>
 static hipError_t rows_n(int dtype, const void* holos, int A, const float* win,
                          const float2* tw, int x0, int C, float2* t1,
                          hipStream_t s) {
-  constexpr int NP = holo_pairs<N>();
   const int W = holo_tile_w(C, holo_cols_nc<N>());
   const int cp = t1_cols_n<N>(C);
   int lw = 0;
   while ((1 << lw) < W) ++lw;  // W is a power of two
-  dim3 grid(N / (2 * NP), A), block(NP * Fft2<N, false>::T);
   Timed tm("holo_rows", s);
   switch (dtype) {
     case MTD_U8:
-      hipLaunchKernelGGL((k_holo_rows<N, uint8_t>), grid, block, 0, s,
-                         (const uint8_t*)holos, win, tw, x0, C, cp, lw, t1);
+      rows_launch<N, uint8_t>(holos, A, win, tw, x0, C, cp, lw, t1, s);
       break;
     case MTD_U16:
-      hipLaunchKernelGGL((k_holo_rows<N, uint16_t>), grid, block, 0, s,
-                         (const uint16_t*)holos, win, tw, x0, C, cp, lw, t1);
+      rows_launch<N, uint16_t>(holos, A, win, tw, x0, C, cp, lw, t1, s);
       break;
     default:
-      hipLaunchKernelGGL((k_holo_rows<N, float>), grid, block, 0, s,
-                         (const float*)holos, win, tw, x0, C, cp, lw, t1);
+      rows_launch<N, float>(holos, A, win, tw, x0, C, cp, lw, t1, s);
       break;
   }
   return hipGetLastError();

@@ -45,6 +45,46 @@ def test_preprocess_2048_crop_512(gpu_ctx):
     assert e < 1e-5
 
 
+@pytest.mark.parametrize("dtype,nx,cx,cy", [("u8", 64, 700, 1400), ("u16", 320, 1020, 1030),
+                                            ("f32", 300, 333, 1700), ("u16", 512, 600, 1500)])
+def test_holo_to_crop_2048(gpu_ctx, dtype, nx, cx, cy):
+    """The 2048 row pass (each row's even and odd samples as one 1024-point
+    complex transform) for every input type, with crops up to N/4 (the
+    separation's first pass kept in registers) and wider (staged in the
+    output slots; crop 1024 reaches bins past N/2). Fringes of the carrier at
+    (cx, cy) with a seeded speckle and noise; crops within rel-L2 1e-5 of the
+    FP32 oracle and within the FP64 bound of the FP64 oracle."""
+    import torch
+    import mtd_transmission_amd as m
+    from lengths import fp64_bound
+    n, A, C = 2048, 2, 2 * nx
+    _say(f"holo_to_crop N=2048 {dtype} crop={C} carrier=({cx},{cy})")
+    rng = np.random.default_rng(C + cx)
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
+    fr = 2 * np.pi * ((cx - n // 2) * xx + (cy - n // 2) * yy) / n
+    top = {"u8": 250.0, "u16": 60000.0, "f32": 4000.0}[dtype]
+    holos = []
+    for a in range(A):
+        ph = rng.uniform(-0.5, 0.5, (n // 64, n // 64)).repeat(64, 0).repeat(64, 1)
+        h = 0.5 + 0.35 * np.cos(fr + ph + a) + 0.05 * rng.standard_normal((n, n))
+        holos.append(np.clip(h, 0, 1) * top)
+    h = np.stack(holos)
+    h = {"u8": h.astype(np.uint8), "u16": h.astype(np.uint16),
+         "f32": h.astype(np.float32)}[dtype]
+    pix = 1e-7
+    gl = m.geom_derive(n, nx, 2 * C, cx, cy, NA, N0, LAM, pix)
+    go = ref.derive_geom(n, nx, 2 * C, cx, cy, NA, N0, LAM, pix, 1, 1, 1)
+    crops = torch.empty((A, C, C), dtype=torch.complex64, device="cuda")
+    gpu_ctx.holo_to_crop(gl, torch.from_numpy(h).cuda(), crops)
+    torch.cuda.synchronize()
+    got = crops.cpu().numpy()
+    exp32 = np.stack([ref.holo_to_crop(x, go, "f32") for x in h])
+    exp64 = np.stack([ref.holo_to_crop(x, go, "f64") for x in h])
+    e32, e64 = rel_l2(got, exp32), rel_l2(got, exp64)
+    _say(f"  rel-L2 vs f32 {e32:.3e}  vs f64 {e64:.3e}")
+    assert e32 < 1e-5 and e64 <= fp64_bound(exp32, exp64)
+
+
 def test_finalize_1024_deltas_closed_form(gpu_ctx):
     import torch
     import mtd_transmission_amd as m
