@@ -261,6 +261,77 @@ struct Fft2 {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Balanced two-step transform, for L = L1*L2 with L1 < L2 and L1 | L2 (512,
+// 128, ...): Fft2's second step runs on L1 of its L2 threads; here both steps
+// run on all T = L1 threads of a line, R = L2/L1 transforms each in the
+// second:
+//   step 1  thread t holds x[L1*m + t], m < L2, and does the L2-point DFT
+//           over m: Y[t][k2];
+//   twiddle Y[t][k2] *= W_L^(t*k2) (table);
+//   exchange (real parts, then imaginary parts) through the storage seen as
+//           an L2 x L1 matrix (k2, t);
+//   step 2  thread s does the L1-point DFTs over t of rows k2 = R*s + j,
+//           j < R, and holds X[k2 + L2*k1] at v[j*L1 + k1].
+// ---------------------------------------------------------------------------
+// Exchange storage of a block's columns for Fft2B: NC columns interleaved
+// (column fastest, as the lanes are). With NC = 32 a wave holds two threads
+// t of each column; in the second step they read rows R*s + j of different
+// s parity, so the low bit of t is flipped on rows whose s = k2/R is odd,
+// which puts the two reads on different halves of the banks.
+template <int L1, int L2, int NC>
+struct ColB {
+  static constexpr int R = L2 / L1;
+  float* p;  // base + column
+  __device__ __forceinline__ float* at(int k2, int t) const {
+    return p + (k2 * L1 + (t ^ ((k2 / R) & 1))) * NC;
+  }
+};
+
+template <int L, bool INV>
+struct Fft2B {
+  static constexpr int L1 = fft2_l1(L);
+  static constexpr int L2 = fft2_l2(L);
+  static constexpr int T = L1;  // threads per transform
+  static constexpr int R = L2 / L1;
+  static_assert(L1 > 1 && L1 < L2 && L2 % L1 == 0, "balanced split needs L1 | L2, L1 < L2");
+  template <int NC>
+  using Col = ColB<L1, L2, NC>;
+
+  // v[m] = x[L1*m + t] on entry (thread t); on exit v[j*L1 + k1] =
+  // X[R*t + j + L2*k1]. Three barriers: every thread of the workgroup calls
+  // it. The exchange reuses v: each component is read back into the slots
+  // whose values were just written out.
+  template <class X>
+  __device__ __forceinline__ static void run(float2 (&v)[L2], int t, X x,
+                                             const float2* __restrict__ tw) {
+    RegFFT<L2, INV>::run(v);
+#pragma unroll
+    for (int k2 = 1; k2 < L2; ++k2) {
+      float2 w = tw[t * k2];
+      if (INV) w.y = -w.y;
+      v[k2] = cmul(v[k2], w);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < L2; ++k2) *x.at(k2, t) = v[k2].x;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int i = 0; i < L1; ++i) v[j * L1 + i].x = *x.at(R * t + j, i);
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < L2; ++k2) *x.at(k2, t) = v[k2].y;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int i = 0; i < L1; ++i) v[j * L1 + i].y = *x.at(R * t + j, i);
+#pragma unroll
+    for (int j = 0; j < R; ++j) RegFFT<L1, INV>::run(v + j * L1);
+  }
+};
+
 // Moves a line from the output layout of Fft2::run (threads k1 < L1 hold
 // u[k2] at position k1 + L1*k2) to its input layout (thread n2 holds v[n1]
 // at position L2*n1 + n2), one component at a time. Starts and ends with a

@@ -625,6 +625,7 @@ struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
 
 template <int VT>
 struct ZOutOp {  // lines along z; a = y, col = x, row = z
+  static constexpr bool kBalancedCols = true;
   const float2* buf;
   float* re;
   float* im;
@@ -700,7 +701,12 @@ static hipError_t vol_passes(Dim<VT> V, const RX& x, const CY& y, const CZ& z, c
       cols(y);
     }
     Timed tm(nz, s);
-    cols(z);
+    if constexpr (op_cols_balanced<CZ>::value && cols_b_fits<VT>()) {
+      hipLaunchKernelGGL((k_cols_b<VT, INV, CZ>), cols_b_grid<VT>(VT), cols_b_block<VT>(), 0, s,
+                         z);
+    } else {
+      cols(z);
+    }
     return hipGetLastError();
   } else {
     if (!ft.gen) return hipErrorInvalidValue;

@@ -199,6 +199,54 @@ __global__ void __launch_bounds__((cols_nc_tv<L, op_col_threads<Op>::value, TV>(
   }
 }
 
+// The column pass on the balanced transform (Fft2B): 32 adjacent columns per
+// block (128-byte rows even for float outputs), T = L1 threads per column,
+// at most 64 KiB of LDS, compiled for 4 waves per SIMD (two 512-thread blocks
+// per CU at L = 512, where k_cols runs one 1024-thread block and leaves half
+// of each column's threads idle in its second step). An operation opts in
+// with a static constexpr bool kBalancedCols = true.
+template <class Op, class = void>
+struct op_cols_balanced : std::false_type {};
+template <class Op>
+struct op_cols_balanced<Op, std::void_t<decltype(Op::kBalancedCols)>>
+    : std::integral_constant<bool, Op::kBalancedCols> {};
+
+constexpr int kColsBNC = 32;
+template <int L>
+constexpr bool cols_b_fits() {
+  constexpr int l1 = fft2_l1(L), l2 = fft2_l2(L);
+  return l1 > 1 && l1 < l2 && l2 % l1 == 0 && L % kColsBNC == 0 &&
+         kColsBNC * L * 4 <= kLdsBlockMax;
+}
+template <int L, bool INV, class Op>
+__global__ void __launch_bounds__((kColsBNC * Fft2B<L, false>::T), 4) k_cols_b(Op op) {
+  using F = Fft2B<L, INV>;
+  constexpr int NC = kColsBNC;
+  __shared__ float lds[NC * L];
+  const int2 ab = xcd_block();
+  const int a = ab.x, c0 = ab.y * NC;
+  if (skip_batch(op, a)) return;  // block-uniform, before the barrier
+  const int c = threadIdx.x % NC, t = threadIdx.x / NC;
+  op.begin(a);
+  float2 v[F::L2];
+#pragma unroll
+  for (int m = 0; m < F::L2; ++m) v[m] = op.load(a, c0 + c, F::L1 * m + t);
+  F::run(v, t, typename F::template Col<NC>{lds + c}, op.tw);
+#pragma unroll
+  for (int j = 0; j < F::R; ++j)
+#pragma unroll
+    for (int k1 = 0; k1 < F::L1; ++k1)
+      op.store(a, c0 + c, F::R * t + j + F::L2 * k1, v[j * F::L1 + k1]);
+}
+template <int L>
+inline dim3 cols_b_grid(int A) {
+  return dim3(L / kColsBNC, A);
+}
+template <int L>
+inline dim3 cols_b_block() {
+  return dim3(kColsBNC * Fft2B<L, false>::T);
+}
+
 // An operation may define mid_line(batch, line), called once per transform
 // between the first transform and its mid() calls: per-line state that only
 // mid() reads is then not held in registers through the first transform.
