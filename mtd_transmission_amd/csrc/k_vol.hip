@@ -613,6 +613,7 @@ struct NormXOp {  // rows along x; a = z, line = y
 template <int VT>
 struct PlaneYOp {  // columns of each z plane; a = z, col = x, row = y
   static constexpr int kColThreads = 512;  // complex in and out
+  static constexpr bool kBalancedCols = true;
   float2* buf;
   const float2* tw;
   const int2* band;
@@ -688,7 +689,11 @@ static hipError_t vol_passes(Dim<VT> V, const RX& x, const CY& y, const CZ& z, c
       }
     }
     auto cols = [&](const auto& op) {
-      if constexpr (TILES) {
+      using Op = std::decay_t<decltype(op)>;
+      if constexpr (op_cols_balanced<Op>::value && cols_b_fits<VT>()) {
+        hipLaunchKernelGGL((k_cols_b<VT, INV, Op>), cols_b_grid<VT>(VT), cols_b_block<VT>(), 0,
+                           s, op);
+      } else if constexpr (TILES) {
         if (ct == 1) vol_cols_tv<VT, INV, 1>(op, s);
         else if (ct == 2) vol_cols_tv<VT, INV, 2>(op, s);
         else vol_cols_tv<VT, INV, 0>(op, s);
@@ -701,12 +706,7 @@ static hipError_t vol_passes(Dim<VT> V, const RX& x, const CY& y, const CZ& z, c
       cols(y);
     }
     Timed tm(nz, s);
-    if constexpr (op_cols_balanced<CZ>::value && cols_b_fits<VT>()) {
-      hipLaunchKernelGGL((k_cols_b<VT, INV, CZ>), cols_b_grid<VT>(VT), cols_b_block<VT>(), 0, s,
-                         z);
-    } else {
-      cols(z);
-    }
+    cols(z);
     return hipGetLastError();
   } else {
     if (!ft.gen) return hipErrorInvalidValue;
