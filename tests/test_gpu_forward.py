@@ -148,12 +148,15 @@ def test_adjoint_spectra(gpu_ctx, name):
     assert e64 < TOL
 
 
-@pytest.mark.parametrize("nx,V,A", [(16, 64, 24), (64, 256, 400), (256, 1024, 32)])
+@pytest.mark.parametrize("nx,V,A", [(16, 64, 24), (8, 32, 10), (32, 128, 60), (64, 256, 400),
+                                    (256, 1024, 32)])
 def test_gpu_pair_is_adjoint(gpu_ctx, nx, V, A):
     """<A x, y> = <x, A^H y> with the GPU forward and adjoint (complex x);
-    the second size is the differentiation workload (400 angles, 128^2
-    spectra, 256^3 volume), the third the large configuration's 512^2
-    spectra and 1024^3 volume (its forward and inverse 1024-point passes)."""
+    volumes 32^3 and 128^3 take the balanced column passes (4 x 8 and
+    8 x 16 points on all of a column's threads) both ways; 256^3 is the
+    differentiation workload (400 angles, 128^2 spectra), 1024^3 the large
+    configuration's 512^2 spectra and 1024^3 volume (its forward and inverse
+    1024-point passes)."""
     import torch
     _say(f"adjointness nx={nx} V={V} A={A}")
     g, gl = _geoms(nx, V)
