@@ -262,10 +262,11 @@ struct Fft2 {
 };
 
 // ---------------------------------------------------------------------------
-// Balanced two-step transform, for L = L1*L2 with L1 < L2 and L1 | L2 (512,
-// 128, ...): Fft2's second step runs on L1 of its L2 threads; here both steps
-// run on all T = L1 threads of a line, R = L2/L1 transforms each in the
-// second:
+// Balanced two-step transform, for L = L1*L2 with L1 | L2 (512, 128, ...):
+// Fft2's second step runs on L1 of its L2 threads (when L1 < L2); here both
+// steps run on all T = L1 threads of a line, R = L2/L1 transforms each in the
+// second, and the exchange reuses the registers of the values it has just
+// written out (one array of L2 values per thread):
 //   step 1  thread t holds x[L1*m + t], m < L2, and does the L2-point DFT
 //           over m: Y[t][k2];
 //   twiddle Y[t][k2] *= W_L^(t*k2) (table);
@@ -275,16 +276,19 @@ struct Fft2 {
 //           j < R, and holds X[k2 + L2*k1] at v[j*L1 + k1].
 // ---------------------------------------------------------------------------
 // Exchange storage of a block's columns for Fft2B: NC columns interleaved
-// (column fastest, as the lanes are). With NC = 32 a wave holds two threads
-// t of each column; in the second step they read rows R*s + j of different
-// s parity, so the low bit of t is flipped on rows whose s = k2/R is odd,
-// which puts the two reads on different halves of the banks.
+// (column fastest, as the lanes are). A wave holds G = 64/NC consecutive
+// threads t of each column; in the second step they read rows R*s + j of G
+// consecutive s, so t is XORed with (k2/R) mod G, which puts the G reads on
+// distinct groups of NC banks (the writes, one row k2 per instruction, stay
+// on distinct groups too).
 template <int L1, int L2, int NC>
 struct ColB {
   static constexpr int R = L2 / L1;
+  static constexpr int G = 64 / NC;
+  static_assert(NC <= 64 && 64 % NC == 0 && G <= L1, "swizzle within a row");
   float* p;  // base + column
   __device__ __forceinline__ float* at(int k2, int t) const {
-    return p + (k2 * L1 + (t ^ ((k2 / R) & 1))) * NC;
+    return p + (k2 * L1 + (t ^ ((k2 / R) & (G - 1)))) * NC;
   }
 };
 
@@ -294,7 +298,7 @@ struct Fft2B {
   static constexpr int L2 = fft2_l2(L);
   static constexpr int T = L1;  // threads per transform
   static constexpr int R = L2 / L1;
-  static_assert(L1 > 1 && L1 < L2 && L2 % L1 == 0, "balanced split needs L1 | L2, L1 < L2");
+  static_assert(L1 > 1 && L2 % L1 == 0, "balanced split needs L1 | L2");
   template <int NC>
   using Col = ColB<L1, L2, NC>;
 

@@ -199,29 +199,36 @@ __global__ void __launch_bounds__((cols_nc_tv<L, op_col_threads<Op>::value, TV>(
   }
 }
 
-// The column pass on the balanced transform (Fft2B): 32 adjacent columns per
-// block (128-byte rows even for float outputs), T = L1 threads per column,
-// at most 64 KiB of LDS, compiled for 4 waves per SIMD (two 512-thread blocks
-// per CU at L = 512, where k_cols runs one 1024-thread block and leaves half
-// of each column's threads idle in its second step). An operation opts in
-// with a static constexpr bool kBalancedCols = true.
+// The column pass on the balanced transform (Fft2B): NC adjacent columns per
+// block (32: 128-byte rows even for float outputs), T = L1 threads per
+// column, compiled for 4 waves per SIMD: two blocks per CU at L = 512 (512
+// threads, 64 KiB), where k_cols runs one 1024-thread block and leaves half
+// of each column's threads idle in its second step. An operation opts in with
+// a static constexpr bool kBalancedCols = true.
 template <class Op, class = void>
 struct op_cols_balanced : std::false_type {};
 template <class Op>
 struct op_cols_balanced<Op, std::void_t<decltype(Op::kBalancedCols)>>
     : std::integral_constant<bool, Op::kBalancedCols> {};
 
-constexpr int kColsBNC = 32;
+template <int L>
+constexpr int cols_b_nc() {
+  return 32 * L * 4 <= kLdsBlockMax ? 32 : 16;
+}
+// lengths that take the balanced pass: L1 | L2 with L1 < L2, where Fft2
+// idles half a line's threads in its second step. At L = 1024 (32 x 32, 16
+// columns per block) it took 128 VGPRs with 12-19 spilled and was slower
+// than k_cols (1024^3 z pass 4.4-5.05 -> 5.4 ms, y pass 1.08 -> 1.21 ms).
 template <int L>
 constexpr bool cols_b_fits() {
-  constexpr int l1 = fft2_l1(L), l2 = fft2_l2(L);
-  return l1 > 1 && l1 < l2 && l2 % l1 == 0 && L % kColsBNC == 0 &&
-         kColsBNC * L * 4 <= kLdsBlockMax;
+  constexpr int l1 = fft2_l1(L), l2 = fft2_l2(L), nc = cols_b_nc<L>();
+  return l1 > 1 && l2 % l1 == 0 && l1 < l2 && L % nc == 0 && nc * L * 4 <= kLdsBlockMax &&
+         64 / nc <= l1;
 }
 template <int L, bool INV, class Op>
-__global__ void __launch_bounds__((kColsBNC * Fft2B<L, false>::T), 4) k_cols_b(Op op) {
+__global__ void __launch_bounds__((cols_b_nc<L>() * Fft2B<L, false>::T), 4) k_cols_b(Op op) {
   using F = Fft2B<L, INV>;
-  constexpr int NC = kColsBNC;
+  constexpr int NC = cols_b_nc<L>();
   __shared__ float lds[NC * L];
   const int2 ab = xcd_block();
   const int a = ab.x, c0 = ab.y * NC;
@@ -240,11 +247,11 @@ __global__ void __launch_bounds__((kColsBNC * Fft2B<L, false>::T), 4) k_cols_b(O
 }
 template <int L>
 inline dim3 cols_b_grid(int A) {
-  return dim3(L / kColsBNC, A);
+  return dim3(L / cols_b_nc<L>(), A);
 }
 template <int L>
 inline dim3 cols_b_block() {
-  return dim3(kColsBNC * Fft2B<L, false>::T);
+  return dim3(cols_b_nc<L>() * Fft2B<L, false>::T);
 }
 
 // An operation may define mid_line(batch, line), called once per transform
