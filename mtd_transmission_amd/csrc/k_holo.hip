@@ -9,10 +9,10 @@
 // Pass 1 (rows): two real rows are packed into one complex FFT of length N
 // (two-step register FFT; from N = 2048, each row's even and odd samples into
 // one of length N/2); only the C bins kx in [x0, x0+C) are separated and
-// written, to t1 (N x C per hologram) in column tiles of W bins (W = 32 when
+// written, to t1 (N x C per hologram) in column tiles of W bins (W = 16 when
 // it divides C; from N = 2048 the column block width):
-// t1[a][d / W][y][d % W]. A row pass stores runs of W bins (256 bytes at
-// W = 32), and the blocks of the column pass that share a tile read it as
+// t1[a][d / W][y][d % W]. A row pass stores runs of W bins (128 bytes at
+// W = 16), and the blocks of the column pass that share a tile read it as
 // one contiguous N x W slab.
 // Pass 2 (columns): each of the C columns of t1 is transformed along y and
 // only the C bins ky in [y0, y0+C) are kept, scaled by 1/N^2 and signed.
@@ -31,10 +31,10 @@ constexpr int holo_cols_nc() {  // a power of two <= 16: divides every crop side
   return divisor_le(16, by_thr < by_lds ? by_thr : by_lds);
 }
 
-// width of the column tiles of t1: the largest power of two from 32 down to
+// width of the column tiles of t1: the largest power of two from 16 down to
 // the column block width nc (a power of two dividing C) that divides C
 static int holo_tile_w(int C, int nc) {
-  for (int w = 32; w > nc; w /= 2)
+  for (int w = 16; w > nc; w /= 2)
     if (C % w == 0) return w;
   return nc;
 }
@@ -43,8 +43,10 @@ static int holo_tile_w(int C, int nc) {
 // width itself (4 columns there), so that each column block reads one
 // contiguous N x 4 slab instead of 32-byte pieces of 256-byte rows (column
 // pass 4.32 -> 3.76 ms, row pass 5.24 -> 5.37 ms per 1000 holograms; 8-bin
-// tiles 3.90 + 5.29 ms); up to 1024 the widest of 32, 16, ... (8-bin tiles
-// at 1024: column pass -28 us, row pass +52 us per volume)
+// tiles 3.90 + 5.29 ms); below, 16 bins where they divide C: the row pass
+// stores whole 128-byte lines and two column blocks share a slab (at 1024,
+// against 32-bin tiles: column pass -12 us per volume, row pass unchanged;
+// 8-bin tiles: -28 us and +52 us)
 template <int N>
 static int holo_w(int C) {
   if constexpr (N >= 2048) return holo_cols_nc<N>();
