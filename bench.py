@@ -537,8 +537,8 @@ def batched_small_graph_gain(args, m, synth, torch, np, count=32):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-pcie", action="store_true",
                     help="skip the PCIe-inclusive leg (host stacks upload in smaller "
