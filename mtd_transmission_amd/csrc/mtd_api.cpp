@@ -767,16 +767,17 @@ static int front_fused(mtd_ctx* ctx, const mtd_geom* g, const float2* q, int n, 
                    "field stage");
 }
 
-// Hologram stage: up to 1 GiB of the N x C row-pass intermediate per chunk
+// Hologram stage: up to 2 GiB of the N x C row-pass intermediate per chunk
 // for stacks in device memory (fewer, larger launch pairs: measured over
 // 600 holograms of 1024^2, crop 256, chunks of 8 to 600 holograms ran
-// fastest at the largest; a chunk small enough for its intermediate to stay
-// in cache between the passes is slower, the launch tails dominate), 128 MiB
+// fastest at the largest, 600 in one launch pair 10 us per volume faster
+// than two of 300; a chunk small enough for its intermediate to stay in
+// cache between the passes is slower, the launch tails dominate), 128 MiB
 // for host stacks (their chunks are the upload granularity, overlapped with
 // the passes). Field stage: about 1 GiB of per-angle buffers per launch, so
 // that its small kernels get hundreds of angles at once.
 static const size_t kChunkBudget = 128ull << 20;
-static const size_t kChunkBudgetDevice = 1ull << 30;
+static const size_t kChunkBudgetDevice = 2ull << 30;
 static const size_t kFieldBudget = 1ull << 30;
 
 // holograms per hologram-stage chunk when no schedule sets it: the budget
