@@ -771,17 +771,18 @@ static int front_fused(mtd_ctx* ctx, const mtd_geom* g, const float2* q, int n, 
 // for stacks in device memory (fewer, larger launch pairs: measured over
 // 600 holograms of 1024^2, crop 256, chunks of 8 to 600 holograms ran
 // fastest at the largest, 600 in one launch pair 10 us per volume faster
-// than two of 300; at 2048^2, crop 512, the field stage's chunks of about
-// 167 angles bound the hologram chunks, one launch pair each instead of two
-// at 1 GiB: 9.24 -> 9.03 ms per 1000 holograms; a chunk small enough for
-// its intermediate to stay in cache between the passes is slower, the
-// launch tails dominate), 128 MiB
+// than two of 300; at 2048^2, crop 512, the field stage's chunks bound the
+// hologram chunks; a chunk small enough for its intermediate to stay in
+// cache between the passes is slower, the launch tails dominate), 128 MiB
 // for host stacks (their chunks are the upload granularity, overlapped with
-// the passes). Field stage: about 1 GiB of per-angle buffers per launch, so
-// that its small kernels get hundreds of angles at once.
+// the passes). Field stage: up to 4 GiB of per-angle buffers per launch, so
+// that its small kernels get hundreds of angles at once (1000 angles at
+// crop 512, 6 MiB each: two chunks instead of six at 1 GiB, the field
+// stage's kernels 4.46 -> 3.97 ms and the hologram stage's 8.88 -> 8.80 ms
+// per volume).
 static const size_t kChunkBudget = 128ull << 20;
 static const size_t kChunkBudgetDevice = 2ull << 30;
-static const size_t kFieldBudget = 1ull << 30;
+static const size_t kFieldBudget = 4ull << 30;
 
 // holograms per hologram-stage chunk when no schedule sets it: the budget
 // of row-pass intermediate, split into even chunks
