@@ -298,7 +298,7 @@ def cpu_baseline_child(path_holos, path_out, threads):
 
 def host_cpu():
     """nproc and the CPU model of this host (lscpu), and the threads the
-    baseline may use: the host's share of one GPU box is 16 cores."""
+    baseline may use: at most 16, the host cores that go with one GPU."""
     import shutil
     nproc = os.cpu_count() or 1
     try:

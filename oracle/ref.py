@@ -4,8 +4,9 @@
 DEROUL 0, plus the Rytov and C_ABER 0 branches).
 
 TEST INFRASTRUCTURE. Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
-checker (or the timed CPU baseline). The product path never calls it.
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as what
+results are compared against (or the timed CPU baseline). The product path
+never calls it.
 
 Two precisions are restated:
 
