@@ -564,8 +564,8 @@ def main():
         return
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(it is measured on a single MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batched":
