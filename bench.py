@@ -350,7 +350,7 @@ def cpu_baseline(holos_dev, mask, A):
 
 def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     """Device-to-device copy of nbytes (read + write): the achievable HBM
-    figure on this box, and the reference kernel for counter scaling."""
+    figure on this GPU, and the reference kernel for counter scaling."""
     src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     dst.copy_(src)
